@@ -1,0 +1,221 @@
+/*
+ * liborbfe -- MI355X (gfx950) ORB front-end: pyramid, FAST-style corner
+ * score, the reference's corner culling, rotated BRIEF, and Hamming
+ * matching, as HIP kernels behind a plain C ABI.
+ *
+ * Conventions (the C form of the reference's `std::optional<std::string>`
+ * error convention, compute/include/compute/pass.hpp:24-37):
+ *   - every entry point returns ORBFE_OK (0) or a negative ORBFE_E* code;
+ *   - on failure orbfe_last_error(ctx) returns the message (owned by ctx,
+ *     valid until the next call on ctx);
+ *   - "_async" entry points take DEVICE pointers and only enqueue work on
+ *     `stream`; the others take HOST pointers and return after the results
+ *     are in host memory;
+ *   - calls on one context must be issued from one host thread at a time
+ *     and, for the async forms, on one stream (the context owns scratch
+ *     memory that consecutive calls reuse).
+ *
+ * Layouts (bit-compatible with the reference):
+ *   - orbfe_gpu_feature == gpumodels::Feature (wslam/src/models.hpp:31-69);
+ *   - a "feature array" is gpumodels::FeatureArray (models.hpp:72-77):
+ *     `levels` blocks of {uint32 count; uint32 pad; orbfe_gpu_feature
+ *     v[cap]}, block stride 8 + 48*cap bytes.  count is the number of corner
+ *     survivors at that level (it can exceed cap, as the reference's
+ *     atomicAdd counter can); v[0 .. min(count, cap)) hold the features in
+ *     raster order (y, then x) and the remaining slots are zero;
+ *   - orbfe_feature == wslam::Feature (models.hpp:86-111);
+ *   - a pyramid buffer holds levels 1..levels-1 as tightly packed row-major
+ *     float32 images, back to back (level 0 is the input frame / 255).
+ */
+#ifndef ORBFE_H
+#define ORBFE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORBFE_OK 0
+#define ORBFE_EINVAL (-1)
+#define ORBFE_EHIP (-2)
+#define ORBFE_ENOMEM (-3)
+#define ORBFE_ERANGE (-4)
+
+#define ORBFE_MAX_LEVELS 8
+
+/* Tunables of the reference's passes. */
+typedef struct orbfe_config {
+    uint32_t frame_width;   /* W0: WSLAM_FRAME_WIDTH, wslam/include/wslam/common.hpp:11-13 */
+    uint32_t frame_height;  /* H0: WSLAM_FRAME_HEIGHT, common.hpp:15-17 */
+    uint32_t levels;        /* GPUConst::levels_of_detail = 6, common.hpp:43 */
+    double scale_factor;    /* GPUConst::lod_scale_factor = 1.2, common.hpp:51 */
+    float fast_threshold;   /* THRESHOLD override, detect_corners.wgsl:22 (0.15) */
+    uint32_t fast_min_run;  /* N_SIMILLAR_MIN override, detect_corners.wgsl:23 (9) */
+    int32_t orientation_radius; /* ORIENTATION_R, generate_features.wgsl:46 (3) */
+    uint32_t features_per_level; /* FEATURE_BLOCK_SZ; 0 = W0*H0/250, common.hpp:47 */
+    uint32_t border_zero;   /* textureLoad out of range: 0 = Tint clamp
+                               (robustness.cc:344-355), 1 = read zero */
+    float lowe_ratio;       /* kLoweTestThreshold, match_features_cpu.cpp:75 (0.85) */
+} orbfe_config;
+
+typedef struct orbfe_gpu_feature {
+    float x;
+    float y;
+    uint32_t strength;
+    float orientation;
+    uint32_t descriptor[8];
+} orbfe_gpu_feature;
+
+typedef struct orbfe_feature {
+    uint32_t x;
+    uint32_t y;
+    uint32_t lod;
+    uint32_t strength;
+    float orientation;
+    uint32_t descriptor[8];
+} orbfe_feature;
+
+/* One frame of a batched detect call (device pointers). */
+typedef struct orbfe_frame {
+    const uint8_t* image;   /* u8 grayscale, height rows of `pitch` bytes */
+    size_t pitch;
+    uint32_t width;
+    uint32_t height;
+    uint32_t cap;           /* features per level; 0 = width*height/250 */
+    uint32_t reserved;
+    float* pyramid;         /* nullable: levels 1.. output */
+    void* features;         /* feature array output, stride 8 + 48*cap */
+} orbfe_frame;
+
+/* One frame pair of a batched pipeline-match call (device pointers). */
+typedef struct orbfe_match_pair {
+    const void* prev_features; /* feature array of the reference frame */
+    const void* curr_features; /* feature array of the current frame */
+    int32_t* prev_match;       /* levels*cap entries, see orbfe_match_pipeline_async */
+    uint32_t width;            /* frame extent both arrays were detected at */
+    uint32_t height;
+    uint32_t cap;              /* cap both arrays were detected with */
+    uint32_t reserved;
+} orbfe_match_pair;
+
+/* Defaults of the reference for a W0 x H0 frame. */
+int orbfe_config_default(orbfe_config* cfg, uint32_t width, uint32_t height);
+
+/* Creates a context: builds the per-resolution plan (level extents, the
+ * cull mask of cull_corners.wgsl) and allocates scratch for one frame.
+ * Replaces the initialize() chain of FillPyramidPass / PassDetectCorners /
+ * CullCornersPass / GenerateFeaturesPass
+ * (wslam/src/feature_detect/fill_pyramid.cpp, detect_corners.cpp:27-58,
+ * cull_corners.cpp:38-60, generate_features.cpp:71-101). */
+int orbfe_create(const orbfe_config* cfg, int device, struct orbfe_ctx** out);
+void orbfe_destroy(struct orbfe_ctx* ctx);
+const char* orbfe_last_error(const struct orbfe_ctx* ctx);
+
+/* Level extents (texture sizes of wslam/src/common.cpp:38-45). */
+int orbfe_level_dims(const struct orbfe_ctx* ctx, uint32_t* widths, uint32_t* heights);
+/* Bytes of one feature array for this context (6 * (8 + 48*cap) by default). */
+size_t orbfe_feature_array_bytes(const struct orbfe_ctx* ctx);
+/* Floats of one pyramid buffer (levels 1..L-1). */
+size_t orbfe_pyramid_floats(const struct orbfe_ctx* ctx);
+uint32_t orbfe_cap(const struct orbfe_ctx* ctx);
+
+/* Feature-detect stage of one frame: FillPyramidPass::prepareExecute
+ * (fill_pyramid.cpp:206-314), PassDetectCorners::prepareExecute
+ * (detect_corners.cpp:355-385), CullCornersPass::prepareExecute
+ * (cull_corners.cpp:292-353), GenerateFeaturesPass::prepareExecute
+ * (generate_features.cpp:495-527), enqueued on `stream`.  image is the u8
+ * frame (the reference uploads it as float(u8)/255, provider_base.cpp:121-124;
+ * the conversion happens in-kernel).  pyramid may be NULL. */
+int orbfe_detect_async(struct orbfe_ctx* ctx, const uint8_t* image, size_t pitch,
+                       float* pyramid, void* features, hipStream_t stream);
+
+/* Host-memory form of orbfe_detect_async plus the readback of
+ * LoadDataCPUPass::loadFeatures (load_features_cpu.cpp:164-188): copies the
+ * frame in, runs the stage, copies the feature array (and the pyramid when
+ * pyramid != NULL) out. */
+int orbfe_detect(struct orbfe_ctx* ctx, const uint8_t* image, size_t pitch,
+                 float* pyramid, void* features);
+
+/* The same stage for many frames of arbitrary sizes, one launch per kernel
+ * for the whole batch.  Frames may differ in size and cap; all share the
+ * context's other tunables.  Results are identical to per-frame
+ * orbfe_detect_async.  The frame table is uploaded through a small ring of
+ * pinned staging slots, so this form suits a serving loop that forms a new
+ * batch every tick; it is not capturable into a HIP graph. */
+int orbfe_detect_batch_async(struct orbfe_ctx* ctx, const orbfe_frame* frames,
+                             uint32_t n_frames, hipStream_t stream);
+
+/* A prepared batch: the frame table and scratch are set up once (this call
+ * synchronizes), after which orbfe_batch_detect_async only enqueues the
+ * kernels, so it can be captured into a HIP graph and replayed.  The
+ * batch's buffers belong to the caller and must stay valid. */
+int orbfe_batch_create(struct orbfe_ctx* ctx, const orbfe_frame* frames,
+                       uint32_t n_frames, struct orbfe_batch** out);
+int orbfe_batch_detect_async(struct orbfe_ctx* ctx, struct orbfe_batch* batch,
+                             hipStream_t stream);
+void orbfe_batch_destroy(struct orbfe_batch* batch);
+
+/* FromGpuModel over a host feature array (models.hpp:113-120,
+ * load_features_cpu.cpp:180-185): writes the FeatureSet flattened level by
+ * level into out (room for levels*cap entries) and counts[l] = min(count,
+ * cap).  Pure host code. */
+int orbfe_load_features(const void* features, uint32_t levels, uint32_t cap,
+                        orbfe_feature* out, uint32_t* counts);
+
+/* FindMatches(prev, curr) of match_features_cpu.cpp:152-207 on two device
+ * feature arrays of this context's frame size: LOD+-1 candidate pools, the
+ * 10% spatial window, the Lowe ratio, the mutual check.  prev_match gets
+ * levels*cap int32: entry [l*cap + i] is, for prev feature i of level l, the
+ * matched current feature as (lod << 24 | index), or -1.  This is the
+ * MatchResult of the reference (result[l] maps that current feature to prev
+ * feature i). */
+int orbfe_match_pipeline_async(struct orbfe_ctx* ctx, const void* prev_features,
+                               const void* curr_features, int32_t* prev_match,
+                               hipStream_t stream);
+
+/* Batched forms: many frame pairs, possibly of different sizes, one launch
+ * per kernel; dynamic (pinned staging ring) and prepared (capturable). */
+int orbfe_match_pipeline_batch_async(struct orbfe_ctx* ctx,
+                                     const orbfe_match_pair* pairs,
+                                     uint32_t n_pairs, hipStream_t stream);
+int orbfe_match_batch_create(struct orbfe_ctx* ctx, const orbfe_match_pair* pairs,
+                             uint32_t n_pairs, struct orbfe_match_batch** out);
+int orbfe_match_batch_async(struct orbfe_ctx* ctx, struct orbfe_match_batch* batch,
+                            hipStream_t stream);
+void orbfe_match_batch_destroy(struct orbfe_match_batch* batch);
+
+/* Host-memory form of orbfe_match_pipeline_async (MatchFeaturesCPU::execute,
+ * match_features_cpu.cpp:217-268): host feature arrays in, prev_match out;
+ * n_pairs (nullable) gets the number of matches. */
+int orbfe_match_pipeline(struct orbfe_ctx* ctx, const void* prev_features,
+                         const void* curr_features, int32_t* prev_match,
+                         uint32_t* n_pairs);
+
+/* Brute-force Hamming search of nq query against nt train 256-bit
+ * descriptors (8 uint32 words each, device memory): for each query the
+ * lowest-index train at the minimum distance (idx, d1) and the second
+ * smallest distance d2 over all trains (0xFFFFFFFF when nt < 2; idx =
+ * 0xFFFFFFFF and d1 = 0xFFFFFFFF when nt == 0).  The exact distance of
+ * HammingDistanceBounded (match_features_cpu.cpp:26-55) without a window.
+ * nt must be < 2^23. */
+int orbfe_match_bf_async(struct orbfe_ctx* ctx, const uint32_t* query, uint32_t nq,
+                         const uint32_t* train, uint32_t nt, uint32_t* idx,
+                         uint32_t* d1, uint32_t* d2, hipStream_t stream);
+
+/* Kernel timing: when enabled, every kernel launch of the context is
+ * bracketed by HIP events on its stream; orbfe_timing_read synchronizes and
+ * returns, for kernel `name`, the launch count and summed duration (ms). */
+int orbfe_timing_enable(struct orbfe_ctx* ctx, int enable);
+int orbfe_timing_read(struct orbfe_ctx* ctx, const char* name, uint32_t* launches,
+                      double* total_ms);
+int orbfe_timing_reset(struct orbfe_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

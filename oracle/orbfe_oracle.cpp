@@ -1,0 +1,647 @@
+// orbfe oracle -- TEST INFRASTRUCTURE ONLY (see orbfe_oracle.h).
+//
+// Plain scalar C++ restatement of the reference path.  Built with
+// -ffp-contract=off so that every float operation is the one written here.
+// Not linked into, and never called by, the product library.
+
+#include "orbfe_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <tuple>
+#include <utility>
+#include <vector>
+
+#include "../uni_slam_amd/csrc/brief_pattern.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// D3: own float math.  Cody & Waite (1980) reduction of x by pi/2 with a
+// three-part split of pi/2, Taylor polynomials of sin/cos on [-pi/4, pi/4],
+// atan by the identity atan(t) = pi/4 + atan((t-1)/(t+1)) for t > tan(pi/8)
+// and the Taylor series of atan on [-tan(pi/8), tan(pi/8)].  The constants are
+// the f32 roundings of the stated real numbers.
+// ---------------------------------------------------------------------------
+constexpr float kTwoOverPi = 0x1.45f306p-1f;   // 2/pi
+constexpr float kPio2Hi = 0x1.921fb6p+0f;      // pi/2 rounded
+constexpr float kPio2Mid = -0x1.777a5cp-25f;   // pi/2 - kPio2Hi rounded
+constexpr float kPio2Lo = -0x1.ee59dap-50f;    // remaining tail rounded
+constexpr float kPio4 = 0x1.921fb6p-1f;        // pi/4
+constexpr float kPi = 0x1.921fb6p+1f;          // pi
+constexpr float kTanPi8 = 0x1.a8279ap-2f;      // tan(pi/8)
+
+// 1/(2k+1) with alternating sign, k = 1..10
+constexpr float kAtanT[10] = {
+    -0x1.555556p-2f, 0x1.99999ap-3f,  -0x1.24924ap-3f, 0x1.c71c72p-4f,
+    -0x1.745d18p-4f, 0x1.3b13b2p-4f,  -0x1.111112p-4f, 0x1.e1e1e2p-5f,
+    -0x1.af286cp-5f, 0x1.861862p-5f};
+// sin: -1/3!, 1/5!, -1/7!, 1/9!
+constexpr float kSinS[4] = {-0x1.555556p-3f, 0x1.111112p-7f, -0x1.a01a02p-13f,
+                            0x1.71de3ap-19f};
+// cos: -1/2!, 1/4!, -1/6!, 1/8!, -1/10!
+constexpr float kCosC[5] = {-0x1p-1f, 0x1.555556p-5f, -0x1.6c16c2p-10f,
+                            0x1.a01a02p-16f, -0x1.27e4fcp-22f};
+
+float atan_poly(float t) {
+    const float t2 = t * t;
+    float p = kAtanT[9];
+    for (int k = 8; k >= 0; --k) p = std::fmaf(p, t2, kAtanT[k]);
+    const float t3 = t2 * t;
+    return std::fmaf(p, t3, t);
+}
+
+float atan2_own(float y, float x) {
+    const float ax = std::fabs(x);
+    const float ay = std::fabs(y);
+    const bool swap = ay > ax;
+    const float mx = swap ? ay : ax;
+    const float mn = swap ? ax : ay;
+    const float t = mx == 0.0f ? 0.0f : mn / mx;
+    float a;
+    if (t > kTanPi8) {
+        const float u = (t - 1.0f) / (t + 1.0f);
+        a = kPio4 + atan_poly(u);
+    } else {
+        a = atan_poly(t);
+    }
+    if (swap) a = kPio2Hi - a;
+    if (std::signbit(x)) a = kPi - a;
+    if (std::signbit(y)) a = -a;
+    return a;
+}
+
+void sincos_own(float x, float* s, float* c) {
+    const float k = std::rint(x * kTwoOverPi);
+    float r = std::fmaf(-k, kPio2Hi, x);
+    r = std::fmaf(-k, kPio2Mid, r);
+    r = std::fmaf(-k, kPio2Lo, r);
+    const float r2 = r * r;
+    float ps = std::fmaf(kSinS[3], r2, kSinS[2]);
+    ps = std::fmaf(ps, r2, kSinS[1]);
+    ps = std::fmaf(ps, r2, kSinS[0]);
+    const float r3 = r2 * r;
+    const float sn = std::fmaf(ps, r3, r);
+    float pc = std::fmaf(kCosC[4], r2, kCosC[3]);
+    pc = std::fmaf(pc, r2, kCosC[2]);
+    pc = std::fmaf(pc, r2, kCosC[1]);
+    pc = std::fmaf(pc, r2, kCosC[0]);
+    const float cs = std::fmaf(pc, r2, 1.0f);
+    const int q = static_cast<int>(k) & 3;
+    switch (q) {
+        case 0: *s = sn; *c = cs; break;
+        case 1: *s = cs; *c = -sn; break;
+        case 2: *s = -sn; *c = -cs; break;
+        default: *s = -cs; *c = sn; break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Texture access.
+// ---------------------------------------------------------------------------
+struct Tex {
+    const float* data;
+    uint32_t w;
+    uint32_t h;
+    float at(uint32_t x, uint32_t y) const {
+        return data[static_cast<size_t>(y) * w + x];
+    }
+};
+
+// textureLoad with Tint's robustness clamp: min(u32(c), dim - 1)
+// (vendor/dawn/src/tint/lang/core/ir/transform/robustness.cc:344-355), or
+// zero for out-of-range coordinates in zero mode (D4).
+float texture_load(const Tex& t, int64_t x, int64_t y, uint32_t border_zero) {
+    if (border_zero) {
+        if (x < 0 || y < 0 || x >= t.w || y >= t.h) return 0.0f;
+        return t.at(static_cast<uint32_t>(x), static_cast<uint32_t>(y));
+    }
+    const uint32_t ux = static_cast<uint32_t>(x);
+    const uint32_t uy = static_cast<uint32_t>(y);
+    return t.at(std::min(ux, t.w - 1), std::min(uy, t.h - 1));
+}
+
+// One axis of a linear, clamp-to-edge sampler (D1).
+void axis_taps(float coord, uint32_t n, uint32_t* i0, uint32_t* i1,
+               float* frac) {
+    const float u = coord * static_cast<float>(n);
+    const float uu = u - 0.5f;
+    const float f = std::floor(uu);
+    *frac = uu - f;
+    const float hi = static_cast<float>(n - 1);
+    const float f1 = f + 1.0f;
+    const float c0 = f < 0.0f ? 0.0f : (f > hi ? hi : f);
+    const float c1 = f1 < 0.0f ? 0.0f : (f1 > hi ? hi : f1);
+    *i0 = static_cast<uint32_t>(c0);
+    *i1 = static_cast<uint32_t>(c1);
+}
+
+// textureSampleLevel(tex, linear clamp-to-edge sampler, (u, v), 0) (D1).
+float sample_bilinear(const Tex& t, float u, float v) {
+    uint32_t x0, x1, y0, y1;
+    float a, b;
+    axis_taps(u, t.w, &x0, &x1, &a);
+    axis_taps(v, t.h, &y0, &y1, &b);
+    const float t00 = t.at(x0, y0);
+    const float t10 = t.at(x1, y0);
+    const float t01 = t.at(x0, y1);
+    const float t11 = t.at(x1, y1);
+    const float top = t00 + a * (t10 - t00);
+    const float bot = t01 + a * (t11 - t01);
+    return top + b * (bot - top);
+}
+
+// detect_corners.wgsl:31-32
+constexpr int kCircleX[16] = {0, 1, 2, 3, 3, 3, 2, 1, 0, -1, -2, -3, -3, -3, -2, -1};
+constexpr int kCircleY[16] = {-3, -3, -2, -1, 0, 1, 2, 3, 3, 3, 2, 1, 0, -1, -2, -3};
+
+// detect_corners.wgsl:58-81
+uint32_t corner_strength(const Tex& t, uint32_t x, uint32_t y, float threshold,
+                         uint32_t min_run, uint32_t border_zero) {
+    uint32_t curr = 1;
+    uint32_t streak = 1;
+    for (int i = 0; i < 16; ++i) {
+        const int64_t cx = static_cast<int64_t>(x) + kCircleX[i];
+        const int64_t cy = static_cast<int64_t>(y) + kCircleY[i];
+        const float center = t.at(x, y);
+        const float counter = texture_load(t, cx, cy, border_zero);
+        const bool is_different = std::fabs(center - counter) > threshold;
+        curr = is_different ? curr + 1 : 1;
+        streak = std::max(curr, streak);
+    }
+    return streak > min_run ? streak : 0u;
+}
+
+// generate_features.wgsl:80-99
+float orientation(const Tex& t, uint32_t px, uint32_t py, int32_t r,
+                  uint32_t border_zero) {
+    float m10 = 0.0f;
+    float m01 = 0.0f;
+    const int64_t fx = static_cast<int32_t>(px);
+    const int64_t fy = static_cast<int32_t>(py);
+    for (int32_t y = -r; y <= r; ++y) {
+        for (int32_t x = -r; x <= r; ++x) {
+            if (x * x + y * y > r * r) continue;
+            const float pixel = texture_load(t, fx + x, fy + y, border_zero);
+            m10 += static_cast<float>(x) * pixel;
+            m01 += static_cast<float>(y) * pixel;
+        }
+    }
+    return atan2_own(m01, m10);
+}
+
+// generate_features.wgsl:149-172
+float gaussian_tap9(const Tex& t, uint32_t px, uint32_t py) {
+    static const float kOffsets[3] = {-1.2f, 0.0f, 1.2f};
+    static const float kWeights[3] = {0.3125f, 0.375f, 0.3125f};
+    const float texel_x = 1.0f / static_cast<float>(t.w);
+    const float texel_y = 1.0f / static_cast<float>(t.h);
+    const float uvx = (static_cast<float>(px) + 0.5f) * texel_x;
+    const float uvy = (static_cast<float>(py) + 0.5f) * texel_y;
+    float sum = 0.0f;
+    for (int j = 0; j < 3; ++j) {
+        for (int i = 0; i < 3; ++i) {
+            const float ox = kOffsets[i] * texel_x;
+            const float oy = kOffsets[j] * texel_y;
+            const float s = sample_bilinear(t, uvx + ox, uvy + oy);
+            sum = sum + s * kWeights[i] * kWeights[j];
+        }
+    }
+    return sum;
+}
+
+// generate_features.wgsl:101-144
+void compute_descriptor(const Tex& t, int32_t fx, int32_t fy, float theta,
+                        uint32_t desc[8]) {
+    float s, c;
+    sincos_own(theta, &s, &c);
+    for (int k = 0; k < 8; ++k) desc[k] = 0;
+    for (uint32_t i = 0; i < 256; ++i) {
+        const float pax = kOrbfeBriefPattern[i][0];
+        const float pay = kOrbfeBriefPattern[i][1];
+        const float pbx = kOrbfeBriefPattern[i][2];
+        const float pby = kOrbfeBriefPattern[i][3];
+        const int32_t ax = static_cast<int32_t>(std::rint(pax * c - pay * s));
+        const int32_t ay = static_cast<int32_t>(std::rint(pax * s + pay * c));
+        const int32_t bx = static_cast<int32_t>(std::rint(pbx * c - pby * s));
+        const int32_t by = static_cast<int32_t>(std::rint(pbx * s + pby * c));
+        const uint32_t cax = static_cast<uint32_t>(fx + ax);
+        const uint32_t cay = static_cast<uint32_t>(fy + ay);
+        const uint32_t cbx = static_cast<uint32_t>(fx + bx);
+        const uint32_t cby = static_cast<uint32_t>(fy + by);
+        const float sa = gaussian_tap9(t, cax, cay);
+        const float sb = gaussian_tap9(t, cbx, cby);
+        const uint32_t test = sa > sb ? 1u : 0u;
+        desc[i / 32] |= test << (i % 32);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+float oracle_atan2f(float y, float x) { return atan2_own(y, x); }
+
+void oracle_sincosf(float x, float* s, float* c) { sincos_own(x, s, c); }
+
+void oracle_level_dims(uint32_t W0, uint32_t H0, uint32_t levels, double scale,
+                       uint32_t* w, uint32_t* h) {
+    for (uint32_t lod = 0; lod < levels; ++lod) {
+        const double factor =
+            lod == 0 ? 1.0 : std::pow(scale, static_cast<double>(lod));
+        w[lod] = static_cast<uint32_t>(W0 / factor);
+        h[lod] = static_cast<uint32_t>(H0 / factor);
+    }
+}
+
+void oracle_lod0(const uint8_t* img, size_t pitch, uint32_t W0, uint32_t H0,
+                 float* out) {
+    for (uint32_t y = 0; y < H0; ++y)
+        for (uint32_t x = 0; x < W0; ++x)
+            out[static_cast<size_t>(y) * W0 + x] =
+                static_cast<float>(img[y * pitch + x]) /
+                static_cast<float>(std::numeric_limits<uint8_t>::max());
+}
+
+void oracle_pyramid_level(const float* lod0, uint32_t W0, uint32_t H0,
+                          uint32_t w, uint32_t h, float* out) {
+    const Tex src{lod0, W0, H0};
+    const float dw = static_cast<float>(w);
+    const float dh = static_cast<float>(h);
+    for (uint32_t y = 0; y < h; ++y) {
+        for (uint32_t x = 0; x < w; ++x) {
+            const float u = (static_cast<float>(x) + 0.5f) / dw;
+            const float v = (static_cast<float>(y) + 0.5f) / dh;
+            out[static_cast<size_t>(y) * w + x] = sample_bilinear(src, u, v);
+        }
+    }
+}
+
+void oracle_detect(const float* lvl, uint32_t w, uint32_t h, float threshold,
+                   uint32_t min_run, uint32_t border_zero, uint32_t* S) {
+    const Tex t{lvl, w, h};
+    for (uint32_t y = 0; y < h; ++y)
+        for (uint32_t x = 0; x < w; ++x)
+            S[static_cast<size_t>(y) * w + x] =
+                corner_strength(t, x, y, threshold, min_run, border_zero);
+}
+
+void oracle_cull(const uint32_t* S_all, uint32_t W0, uint32_t H0,
+                 uint32_t levels, const uint32_t* dims_w,
+                 const uint32_t* dims_h, uint32_t* O_all) {
+    const uint64_t N0 = static_cast<uint64_t>(W0) * H0;  // CORNER_BLOCK_SZ
+    const uint32_t region = 3;                           // REGION_SIZE
+    const uint32_t wg_x = 64;                            // kWgSize[0]
+    auto ceil_div = [](uint64_t n, uint64_t d) { return (n + d - 1) / d; };
+    std::vector<uint32_t> T(static_cast<size_t>(N0));
+
+    // Dispatch extents, cull_corners.cpp:332-353.
+    const uint64_t hor_gx = wg_x * ceil_div(ceil_div(W0, region), wg_x);
+    const uint64_t hor_gy = H0;
+    const uint64_t ver_gx = wg_x * ceil_div(W0, wg_x);
+    const uint64_t ver_gy = ceil_div(H0, region);
+
+    for (uint32_t lod = 0; lod < levels; ++lod) {
+        const uint32_t* src = S_all + lod * N0;
+        uint32_t* dst = O_all + lod * N0;
+        const uint64_t w = dims_w[lod];
+        const uint64_t h = dims_h[lod];
+        std::fill(T.begin(), T.end(), 0u);
+        std::fill(dst, dst + N0, 0u);
+
+        // main_horizontal, cull_corners.wgsl:21-57
+        for (uint64_t gy = 0; gy < hor_gy; ++gy) {
+            for (uint64_t gx = 0; gx < hor_gx; ++gx) {
+                const uint64_t x = gx * region;
+                const uint64_t start = x * region + gy * w;
+                const uint64_t end = std::min(start + region, w);
+                if (start >= w * h) continue;
+                uint64_t max_idx = start;
+                bool did_find = src[start] > 0;
+                for (uint64_t xx = start; xx < end; ++xx) {
+                    if (src[xx] > src[max_idx]) {
+                        max_idx = xx;
+                        did_find = true;
+                    }
+                    T[xx] = 0u;
+                }
+                if (did_find) T[max_idx] = src[max_idx];
+            }
+        }
+
+        // main_vertical, cull_corners.wgsl:60-109
+        for (uint64_t gy = 0; gy < ver_gy; ++gy) {
+            for (uint64_t gx = 0; gx < ver_gx; ++gx) {
+                const uint64_t x = gx;
+                const uint64_t y = gy * region;
+                const uint64_t start = x * region + y * w;
+                if (start >= w * h) continue;
+                const uint64_t step = w;
+                const uint64_t left = N0 - start;
+                const uint64_t step_count = std::min<uint64_t>(left / step, region);
+                const uint64_t end = start + step * step_count;
+                uint64_t max_idx = start;
+                bool did_find = T[start] > 0;
+                for (uint64_t i = start; i < end; i += step) {
+                    if (T[i] > T[max_idx]) {
+                        max_idx = i;
+                        did_find = true;
+                    }
+                    dst[i] = T[i];
+                }
+                if (did_find) dst[max_idx] = T[max_idx];
+            }
+        }
+    }
+}
+
+uint32_t oracle_generate(const float* lvl, uint32_t w, uint32_t h,
+                         const uint32_t* O, uint32_t cap,
+                         int32_t orientation_radius, uint32_t border_zero,
+                         oracle_gpu_feature* out) {
+    const Tex t{lvl, w, h};
+    uint32_t count = 0;
+    for (uint32_t y = 0; y < h; ++y) {
+        for (uint32_t x = 0; x < w; ++x) {
+            const uint32_t strength = O[static_cast<size_t>(w) * y + x];
+            if (strength == 0) continue;
+            if (count < cap) {
+                oracle_gpu_feature f;
+                f.orientation = orientation(t, x, y, orientation_radius,
+                                            border_zero);
+                compute_descriptor(t, static_cast<int32_t>(x),
+                                   static_cast<int32_t>(y), f.orientation,
+                                   f.descriptor);
+                f.x = static_cast<float>(x);
+                f.y = static_cast<float>(y);
+                f.strength = strength;
+                out[count] = f;
+            }
+            ++count;
+        }
+    }
+    return count;
+}
+
+int oracle_detect_frame(const uint8_t* img, size_t pitch,
+                        const oracle_params* p, float* pyramid_out,
+                        void* feature_array_out) {
+    const uint32_t L = p->levels;
+    if (L == 0 || L > ORACLE_MAX_LEVELS) return -1;
+    const uint32_t W0 = p->frame_width;
+    const uint32_t H0 = p->frame_height;
+    const size_t N0 = static_cast<size_t>(W0) * H0;
+    uint32_t dw[ORACLE_MAX_LEVELS], dh[ORACLE_MAX_LEVELS];
+    oracle_level_dims(W0, H0, L, p->scale_factor, dw, dh);
+
+    std::vector<std::vector<float>> pyr(L);
+    pyr[0].resize(N0);
+    oracle_lod0(img, pitch, W0, H0, pyr[0].data());
+    size_t pyr_off = 0;
+    for (uint32_t l = 1; l < L; ++l) {
+        pyr[l].resize(static_cast<size_t>(dw[l]) * dh[l]);
+        oracle_pyramid_level(pyr[0].data(), W0, H0, dw[l], dh[l], pyr[l].data());
+        if (pyramid_out) {
+            std::memcpy(pyramid_out + pyr_off, pyr[l].data(),
+                        pyr[l].size() * sizeof(float));
+            pyr_off += pyr[l].size();
+        }
+    }
+
+    std::vector<uint32_t> S(L * N0, 0u), O(L * N0, 0u);
+    for (uint32_t l = 0; l < L; ++l)
+        oracle_detect(pyr[l].data(), dw[l], dh[l], p->fast_threshold,
+                      p->fast_min_run, p->border_zero, S.data() + l * N0);
+    oracle_cull(S.data(), W0, H0, L, dw, dh, O.data());
+
+    const size_t block_bytes = 8 + sizeof(oracle_gpu_feature) * p->cap;
+    uint8_t* base = static_cast<uint8_t*>(feature_array_out);
+    std::memset(base, 0, block_bytes * L);
+    for (uint32_t l = 0; l < L; ++l) {
+        uint8_t* block = base + block_bytes * l;
+        auto* feats = reinterpret_cast<oracle_gpu_feature*>(block + 8);
+        const uint32_t count =
+            oracle_generate(pyr[l].data(), dw[l], dh[l], O.data() + l * N0,
+                            p->cap, p->orientation_radius, p->border_zero, feats);
+        std::memcpy(block, &count, 4);
+    }
+    return 0;
+}
+
+void oracle_load_features(const void* feature_array, uint32_t levels,
+                          uint32_t cap, oracle_feature* out, uint32_t* counts) {
+    const size_t block_bytes = 8 + sizeof(oracle_gpu_feature) * cap;
+    const uint8_t* base = static_cast<const uint8_t*>(feature_array);
+    size_t n = 0;
+    for (uint32_t lod = 0; lod < levels; ++lod) {
+        const uint8_t* block = base + block_bytes * lod;
+        uint32_t count;
+        std::memcpy(&count, block, 4);
+        const uint32_t take = std::min(count, cap);
+        const auto* feats = reinterpret_cast<const oracle_gpu_feature*>(block + 8);
+        for (uint32_t i = 0; i < take; ++i) {
+            oracle_feature f;
+            f.x = static_cast<uint32_t>(feats[i].x);
+            f.y = static_cast<uint32_t>(feats[i].y);
+            f.lod = lod;
+            f.strength = feats[i].strength;
+            f.orientation = feats[i].orientation;
+            std::memcpy(f.descriptor, feats[i].descriptor, 32);
+            out[n++] = f;
+        }
+        counts[lod] = take;
+    }
+}
+
+uint32_t oracle_hamming_bounded(const uint32_t* a, const uint32_t* b,
+                                uint32_t cap) {
+    uint32_t lower = 0;
+    for (int k = 0; k < 4; ++k) lower += __builtin_popcount(a[k] ^ b[k]);
+    if (lower >= cap) return cap;
+    uint32_t upper = 0;
+    for (int k = 4; k < 8; ++k) upper += __builtin_popcount(a[k] ^ b[k]);
+    return lower + upper;
+}
+
+}  // extern "C"
+
+namespace {
+
+// A feature together with its position in its FeatureSet, so that the
+// matcher's Feature-valued results can be reported as indices.
+struct Tagged {
+    oracle_feature f;
+    uint32_t index;
+};
+
+// wslam::Feature's defaulted operator<=> (models.hpp:109): members in
+// declaration order, descriptor words lexicographically.
+bool feature_less(const oracle_feature& a, const oracle_feature& b) {
+    if (a.x != b.x) return a.x < b.x;
+    if (a.y != b.y) return a.y < b.y;
+    if (a.lod != b.lod) return a.lod < b.lod;
+    if (a.strength != b.strength) return a.strength < b.strength;
+    if (a.orientation != b.orientation) return a.orientation < b.orientation;
+    for (int k = 0; k < 8; ++k)
+        if (a.descriptor[k] != b.descriptor[k])
+            return a.descriptor[k] < b.descriptor[k];
+    return false;
+}
+
+bool feature_equal(const oracle_feature& a, const oracle_feature& b) {
+    return !feature_less(a, b) && !feature_less(b, a);
+}
+
+struct FeatureLess {
+    bool operator()(const oracle_feature& a, const oracle_feature& b) const {
+        return feature_less(a, b);
+    }
+};
+
+// toLod0, match_features_cpu.cpp:58-64
+std::pair<float, float> to_lod0(const oracle_feature& f) {
+    float scale = 1.0f;
+    for (uint32_t i = 0; i < f.lod; ++i) scale *= static_cast<float>(1.2);
+    return {static_cast<float>(f.x) * scale, static_cast<float>(f.y) * scale};
+}
+
+// FindMatchesInLod, match_features_cpu.cpp:66-150.  Returns (query, pool
+// member) pairs.
+std::vector<std::pair<Tagged, Tagged>> find_matches_in_lod(
+    const std::vector<Tagged>& set_a, const std::vector<Tagged>& set_b,
+    uint32_t W0, uint32_t H0, float ratio) {
+    const float max_dx = 0.1f * static_cast<float>(W0);
+    const float max_dy = 0.1f * static_cast<float>(H0);
+    std::vector<std::pair<Tagged, Tagged>> matches;
+    if (set_b.size() < 2) return matches;
+    std::vector<std::pair<float, float>> b_px;
+    b_px.reserve(set_b.size());
+    for (const auto& f : set_b) b_px.push_back(to_lod0(f.f));
+    std::vector<uint32_t> order(set_b.size());
+    for (uint32_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](uint32_t l, uint32_t r) {
+        return b_px[l].first < b_px[r].first;
+    });
+    for (const auto& curr : set_a) {
+        const auto [cx, cy] = to_lod0(curr.f);
+        uint32_t best = std::numeric_limits<uint32_t>::max();
+        uint32_t second = std::numeric_limits<uint32_t>::max();
+        size_t best_idx = 0;
+        bool best_found = false;
+        const float lo = cx - max_dx;
+        auto it = std::lower_bound(order.begin(), order.end(), lo,
+                                   [&](uint32_t idx, float v) {
+                                       return b_px[idx].first < v;
+                                   });
+        for (; it != order.end(); ++it) {
+            const size_t i = *it;
+            const auto [bx, by] = b_px[i];
+            if (bx > cx + max_dx) break;
+            if (std::fabs(cy - by) > max_dy) continue;
+            const uint32_t d = oracle_hamming_bounded(
+                curr.f.descriptor, set_b[i].f.descriptor, second);
+            if (d < best) {
+                second = best;
+                best = d;
+                best_idx = i;
+                best_found = true;
+            } else if (d < second) {
+                second = d;
+            }
+        }
+        if (best_found &&
+            static_cast<float>(best) < ratio * static_cast<float>(second))
+            matches.emplace_back(curr, set_b[best_idx]);
+    }
+    return matches;
+}
+
+}  // namespace
+
+extern "C" int oracle_find_matches(const oracle_feature* prev,
+                                   const uint32_t* prev_counts,
+                                   const oracle_feature* curr,
+                                   const uint32_t* curr_counts, uint32_t levels,
+                                   uint32_t W0, uint32_t H0, float ratio,
+                                   int32_t* prev_match, uint32_t* n_pairs) {
+    if (levels == 0 || levels > ORACLE_MAX_LEVELS) return -1;
+    // FeatureSet as array<vector<Feature>, L> with per-level indices.
+    std::vector<std::vector<Tagged>> set_a(levels), set_b(levels);
+    size_t off_a = 0, off_b = 0;
+    std::vector<size_t> prev_off(levels);
+    for (uint32_t l = 0; l < levels; ++l) {
+        prev_off[l] = off_a;
+        for (uint32_t i = 0; i < prev_counts[l]; ++i)
+            set_a[l].push_back({prev[off_a + i], i});
+        off_a += prev_counts[l];
+        for (uint32_t i = 0; i < curr_counts[l]; ++i)
+            set_b[l].push_back({curr[off_b + i], i});
+        off_b += curr_counts[l];
+    }
+    for (size_t i = 0; i < off_a; ++i) prev_match[i] = -1;
+
+    // adjacent(), match_features_cpu.cpp:164-177
+    auto adjacent = [&](const std::vector<std::vector<Tagged>>& set,
+                        uint32_t lod) {
+        std::vector<Tagged> out(set[lod]);
+        if (lod > 0) out.insert(out.end(), set[lod - 1].begin(), set[lod - 1].end());
+        if (lod + 1 < levels)
+            out.insert(out.end(), set[lod + 1].begin(), set[lod + 1].end());
+        return out;
+    };
+
+    // match_features_cpu.cpp:188-194
+    std::map<oracle_feature, oracle_feature, FeatureLess> b_to_a;
+    for (uint32_t lod = 0; lod < levels; ++lod)
+        for (auto& [b, a] :
+             find_matches_in_lod(set_b[lod], adjacent(set_a, lod), W0, H0, ratio))
+            b_to_a.emplace(b.f, a.f);
+
+    // match_features_cpu.cpp:196-205
+    uint32_t pairs = 0;
+    for (uint32_t lod = 0; lod < levels; ++lod) {
+        std::map<oracle_feature, oracle_feature, FeatureLess> result;
+        for (auto& [a, b] :
+             find_matches_in_lod(set_a[lod], adjacent(set_b, lod), W0, H0, ratio)) {
+            const auto it = b_to_a.find(b.f);
+            if (it != b_to_a.end() && feature_equal(it->second, a.f)) {
+                if (result.emplace(b.f, a.f).second) {
+                    prev_match[prev_off[lod] + a.index] =
+                        static_cast<int32_t>((b.f.lod << 24) | b.index);
+                    ++pairs;
+                }
+            }
+        }
+    }
+    *n_pairs = pairs;
+    return 0;
+}
+
+extern "C" void oracle_match_bf(const uint32_t* q, uint32_t nq,
+                                const uint32_t* t, uint32_t nt, uint32_t* idx,
+                                uint32_t* d1, uint32_t* d2) {
+    for (uint32_t i = 0; i < nq; ++i) {
+        uint32_t best = std::numeric_limits<uint32_t>::max();
+        uint32_t second = std::numeric_limits<uint32_t>::max();
+        uint32_t best_idx = std::numeric_limits<uint32_t>::max();
+        const uint32_t* a = q + 8ull * i;
+        for (uint32_t j = 0; j < nt; ++j) {
+            const uint32_t* b = t + 8ull * j;
+            uint32_t d = 0;
+            for (int k = 0; k < 8; ++k) d += __builtin_popcount(a[k] ^ b[k]);
+            if (d < best) {
+                second = best;
+                best = d;
+                best_idx = j;
+            } else if (d < second) {
+                second = d;
+            }
+        }
+        idx[i] = best_idx;
+        d1[i] = best;
+        d2[i] = second;
+    }
+}

@@ -1,0 +1,951 @@
+// liborbfe host side: plans (level extents and the static cull mask),
+// scratch memory, batch tables, kernel timing and the C entry points of
+// include/orbfe.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/orbfe.h"
+#include "orbfe_device.h"
+
+namespace orbfe {
+hipError_t upload_pattern();
+hipError_t launch_detect(uint32_t blocks, BatchView bv, const Job& job0, const Params& prm,
+                         hipStream_t s);
+hipError_t launch_compact(uint32_t blocks, BatchView bv, const Job& job0, uint32_t levels,
+                          hipStream_t s);
+hipError_t launch_describe(uint32_t blocks, BatchView bv, const Job& job0, const Params& prm,
+                           hipStream_t s);
+hipError_t launch_match_dir(bool mutual, uint32_t blocks, MatchBatchView bv,
+                            const MatchJob& job0, float ratio, hipStream_t s);
+uint32_t bf_queries_per_block();
+hipError_t launch_bf(const uint32_t* q, uint32_t nq, const uint32_t* t, uint32_t nt,
+                     uint32_t splits, uint32_t split_len, uint2* part, uint32_t* idx,
+                     uint32_t* d1, uint32_t* d2, hipStream_t s, int phase);
+}  // namespace orbfe
+
+using orbfe::BatchView;
+using orbfe::Job;
+using orbfe::LevelGeom;
+using orbfe::MatchBatchView;
+using orbfe::MatchJob;
+using orbfe::Params;
+using orbfe::Plan;
+
+namespace {
+
+constexpr size_t kAlign = 256;
+
+size_t align_up(size_t v, size_t a = kAlign) { return (v + a - 1) / a * a; }
+
+uint32_t ceil_div(uint64_t n, uint64_t d) { return static_cast<uint32_t>((n + d - 1) / d); }
+
+// Level extents as the reference creates its textures:
+// width = uint32(W0 / pow(scale, lod)) (wslam/src/common.cpp:38-45).
+void level_dims(uint32_t W0, uint32_t H0, uint32_t L, double scale, uint32_t* w, uint32_t* h) {
+    for (uint32_t lod = 0; lod < L; ++lod) {
+        const double factor = lod == 0 ? 1.0 : std::pow(scale, static_cast<double>(lod));
+        w[lod] = static_cast<uint32_t>(W0 / factor);
+        h[lod] = static_cast<uint32_t>(H0 / factor);
+    }
+}
+
+// Static part of the corner cull of cull_corners.wgsl for one level, as a
+// bitmap in the survivor-word layout (bit x&63 of word y*wpr + x/64).
+//
+// main_horizontal (cull_corners.wgsl:21-57) runs over gid in
+// [0, 64*ceil(ceil(W0/3)/64)) x [0, H0) (cull_corners.cpp:332-341) and
+// starts at s = 9*gid.x + gid.y*w.  For s >= w its loop is empty and it
+// copies S[s] alone; for s < w it keeps the first maximum of S over
+// [s, min(s+3, w)).  main_vertical (cull_corners.wgsl:60-98) runs over
+// [0, 64*ceil(W0/64)) x [0, ceil(H0/3)) (cull_corners.cpp:343-350), starts at
+// s = 3*gid.x + 3*gid.y*w and copies T at s and at s + k*w for
+// k < min((W0*H0 - s)/w, 3) (getVerticalEnd, cull_corners.wgsl:101-109).
+// A survivor is a position p < w*h with O[p] > 0, so for p >= w the static
+// condition is "p is a horizontal start and p is vertically copied", and for
+// p < w it is "p lies in a 3-wide window at a multiple of 9 and p is
+// vertically copied"; the window maximum is evaluated on the scores.
+void build_level_mask(uint32_t W0, uint32_t H0, uint32_t w, uint32_t h, uint32_t wpr,
+                      uint64_t* words) {
+    const uint64_t N0 = static_cast<uint64_t>(W0) * H0;
+    const uint64_t n = static_cast<uint64_t>(w) * h;
+    std::vector<uint8_t> hor(n, 0), ver(n, 0);
+    const uint64_t hor_gx = 64ull * ceil_div(ceil_div(W0, 3), 64);
+    for (uint64_t gy = 0; gy < H0; ++gy) {
+        for (uint64_t gx = 0; gx < hor_gx; ++gx) {
+            const uint64_t s = 9 * gx + gy * w;
+            if (s >= n) break;
+            hor[s] = 1;
+        }
+    }
+    const uint64_t ver_gx = 64ull * ceil_div(W0, 64);
+    const uint64_t ver_gy = ceil_div(H0, 3);
+    for (uint64_t gy = 0; gy < ver_gy; ++gy) {
+        for (uint64_t gx = 0; gx < ver_gx; ++gx) {
+            const uint64_t s = 3 * gx + 3 * gy * w;
+            if (s >= n) break;
+            ver[s] = 1;
+            const uint64_t steps = std::min<uint64_t>((N0 - s) / w, 3);
+            for (uint64_t k = 0; k < steps; ++k)
+                if (s + k * w < n) ver[s + k * w] = 1;
+        }
+    }
+    std::fill(words, words + static_cast<size_t>(h) * wpr, 0ull);
+    for (uint64_t p = 0; p < n; ++p) {
+        const bool keep = p < w ? (ver[p] && (p % 9) < 3) : (ver[p] && hor[p]);
+        if (!keep) continue;
+        const uint64_t y = p / w, x = p % w;
+        words[y * wpr + x / 64] |= 1ull << (x % 64);
+    }
+}
+
+struct PlanEntry {
+    Plan host{};
+    Plan* dev = nullptr;
+    uint64_t* mask = nullptr;
+    // per-frame scratch sizes (bytes), 256-aligned
+    size_t surv_bytes = 0, kp_bytes = 0, pyr_bytes = 0, feat_bytes = 0, match_bytes = 0;
+};
+
+struct Event2 {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+struct StageSlot {
+    void* host = nullptr;
+    void* dev = nullptr;
+    size_t bytes = 0;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+};
+
+}  // namespace
+
+struct orbfe_ctx {
+    int device = 0;
+    orbfe_config cfg{};
+    Params prm{};
+    std::string err;
+    std::map<std::tuple<uint32_t, uint32_t, uint32_t>, std::unique_ptr<PlanEntry>> plans;
+    PlanEntry* main = nullptr;
+    hipStream_t stream = nullptr;  // for the host-memory entry points
+    // single-frame scratch
+    uint64_t* surv = nullptr;
+    uint32_t* kp = nullptr;
+    float* pyr_scratch = nullptr;
+    int32_t* b_to_a = nullptr;
+    // host-memory entry points: device staging
+    uint8_t* d_img = nullptr;
+    float* d_pyr = nullptr;
+    uint8_t* d_feat[2] = {nullptr, nullptr};
+    int32_t* d_match = nullptr;
+    // brute-force partials
+    uint2* bf_part = nullptr;
+    size_t bf_part_bytes = 0;
+    // dynamic batches
+    StageSlot ring[4];
+    uint32_t ring_next = 0;
+    uint8_t* arena = nullptr;
+    size_t arena_bytes = 0;
+    hipStream_t arena_stream = nullptr;
+    // timing
+    bool timing = false;
+    std::vector<Event2> pending;
+    std::vector<hipEvent_t> pool;
+    std::map<std::string, std::pair<uint32_t, double>> totals;
+};
+
+struct orbfe_batch {
+    orbfe_ctx* ctx = nullptr;
+    uint32_t n = 0;
+    uint32_t tiles = 0, desc_blocks = 0;
+    void* table = nullptr;  // device: jobs, tile prefix, desc prefix
+    uint8_t* arena = nullptr;
+    BatchView view{};
+};
+
+struct orbfe_match_batch {
+    orbfe_ctx* ctx = nullptr;
+    uint32_t n = 0;
+    uint32_t blocks = 0;
+    void* table = nullptr;
+    uint8_t* arena = nullptr;
+    MatchBatchView view{};
+};
+
+namespace {
+
+int fail(orbfe_ctx* ctx, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->err = buf;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                          \
+    do {                                                                            \
+        hipError_t e_ = (expr);                                                     \
+        if (e_ != hipSuccess)                                                       \
+            return fail((ctx), ORBFE_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+int dalloc(orbfe_ctx* ctx, T** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = kAlign;
+    void* v = nullptr;
+    const hipError_t e = hipMalloc(&v, bytes);
+    if (e != hipSuccess)
+        return fail(ctx, ORBFE_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    *p = static_cast<T*>(v);
+    return ORBFE_OK;
+}
+
+template <typename T>
+void dfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+uint32_t default_cap(uint32_t W0, uint32_t H0) {
+    // GPUConst::max_features_per_lod = W*H/250 (wslam/include/wslam/common.hpp:47)
+    return static_cast<uint32_t>(static_cast<uint64_t>(W0) * H0 / 250);
+}
+
+int get_plan(orbfe_ctx* ctx, uint32_t W0, uint32_t H0, uint32_t cap, PlanEntry** out) {
+    if (cap == 0) cap = default_cap(W0, H0);
+    const auto key = std::make_tuple(W0, H0, cap);
+    auto it = ctx->plans.find(key);
+    if (it != ctx->plans.end()) {
+        *out = it->second.get();
+        return ORBFE_OK;
+    }
+    const uint32_t L = ctx->cfg.levels;
+    if (W0 == 0 || H0 == 0 || W0 > 65535 || H0 > 65535)
+        return fail(ctx, ORBFE_ERANGE, "frame %ux%u outside 1..65535", W0, H0);
+    if (cap == 0 || cap >= (1u << orbfe::kMatchIdxBits))
+        return fail(ctx, ORBFE_ERANGE, "cap %u outside 1..%u", cap,
+                    (1u << orbfe::kMatchIdxBits) - 1);
+    uint32_t dw[ORBFE_MAX_LEVELS], dh[ORBFE_MAX_LEVELS];
+    level_dims(W0, H0, L, ctx->cfg.scale_factor, dw, dh);
+    auto pe = std::make_unique<PlanEntry>();
+    Plan& P = pe->host;
+    P.W0 = W0;
+    P.H0 = H0;
+    P.L = L;
+    P.cap = cap;
+    P.feat_stride = 8 + 48 * cap;
+    P.desc_groups = ceil_div(cap, 4);
+    P.max_dx = 0.1f * static_cast<float>(W0);
+    P.max_dy = 0.1f * static_cast<float>(H0);
+    uint32_t tiles = 0, words = 0, pyr = 0;
+    float scale = 1.0f;
+    for (uint32_t l = 0; l < L; ++l) {
+        if (dw[l] == 0 || dh[l] == 0)
+            return fail(ctx, ORBFE_ERANGE, "level %u of a %ux%u frame is empty", l, W0, H0);
+        LevelGeom& G = P.lv[l];
+        G.w = dw[l];
+        G.h = dh[l];
+        G.wpr = ceil_div(G.w, 64);
+        G.tiles_x = ceil_div(G.w, orbfe::kTileW);
+        G.tiles_y = ceil_div(G.h, orbfe::kTileH);
+        G.tile_base = tiles;
+        tiles += G.tiles_x * G.tiles_y;
+        G.pyr_off = l == 0 ? 0 : pyr;
+        if (l > 0) pyr += G.w * G.h;
+        G.word_off = words;
+        words += G.h * G.wpr;
+        G.feat_off = l * P.feat_stride;
+        G.kp_off = l * cap;
+        G.lod0_scale = scale;  // toLod0 (match_features_cpu.cpp:58-64)
+        scale *= static_cast<float>(1.2);
+    }
+    P.tiles = tiles;
+    P.words = words;
+    P.pyr_floats = pyr;
+    std::vector<uint64_t> mask(words ? words : 1, 0ull);
+    for (uint32_t l = 0; l < L; ++l)
+        build_level_mask(W0, H0, P.lv[l].w, P.lv[l].h, P.lv[l].wpr,
+                         mask.data() + P.lv[l].word_off);
+    int rc = dalloc(ctx, &pe->mask, mask.size() * 8);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipMemcpy(pe->mask, mask.data(), mask.size() * 8, hipMemcpyHostToDevice));
+    P.mask = pe->mask;
+    rc = dalloc(ctx, &pe->dev, sizeof(Plan));
+    if (rc) return rc;
+    HIP_TRY(ctx, hipMemcpy(pe->dev, &P, sizeof(Plan), hipMemcpyHostToDevice));
+    pe->surv_bytes = align_up(static_cast<size_t>(words) * 8);
+    pe->kp_bytes = align_up(static_cast<size_t>(L) * cap * 4);
+    pe->pyr_bytes = align_up(static_cast<size_t>(pyr) * 4);
+    pe->feat_bytes = align_up(static_cast<size_t>(L) * P.feat_stride);
+    pe->match_bytes = align_up(static_cast<size_t>(L) * cap * 4);
+    *out = pe.get();
+    ctx->plans.emplace(key, std::move(pe));
+    return ORBFE_OK;
+}
+
+template <typename F>
+hipError_t timed(orbfe_ctx* ctx, const char* name, hipStream_t s, F&& launch) {
+    if (!ctx->timing) return launch();
+    hipEvent_t ev[2];
+    for (auto& e : ev) {
+        if (!ctx->pool.empty()) {
+            e = ctx->pool.back();
+            ctx->pool.pop_back();
+        } else {
+            const hipError_t err = hipEventCreate(&e);
+            if (err != hipSuccess) return err;
+        }
+    }
+    hipError_t err = hipEventRecord(ev[0], s);
+    if (err != hipSuccess) return err;
+    err = launch();
+    if (err != hipSuccess) return err;
+    err = hipEventRecord(ev[1], s);
+    ctx->pending.push_back({name, ev[0], ev[1]});
+    return err;
+}
+
+int launch_detect_stage(orbfe_ctx* ctx, const BatchView& bv, const Job& job0, uint32_t tiles,
+                        uint32_t blocks_compact, uint32_t desc_blocks, hipStream_t s) {
+    const Params prm = ctx->prm;
+    HIP_TRY(ctx, timed(ctx, "k_detect", s, [&] {
+                return orbfe::launch_detect(tiles, bv, job0, prm, s);
+            }));
+    HIP_TRY(ctx, timed(ctx, "k_compact", s, [&] {
+                return orbfe::launch_compact(blocks_compact, bv, job0, ctx->cfg.levels, s);
+            }));
+    HIP_TRY(ctx, timed(ctx, "k_describe", s, [&] {
+                return orbfe::launch_describe(desc_blocks, bv, job0, prm, s);
+            }));
+    return ORBFE_OK;
+}
+
+int launch_match_stage(orbfe_ctx* ctx, const MatchBatchView& bv, const MatchJob& job0,
+                       uint32_t blocks, hipStream_t s) {
+    const float ratio = ctx->prm.ratio;
+    HIP_TRY(ctx, timed(ctx, "k_match_dir", s, [&] {
+                return orbfe::launch_match_dir(false, blocks, bv, job0, ratio, s);
+            }));
+    HIP_TRY(ctx, timed(ctx, "k_match_mutual", s, [&] {
+                return orbfe::launch_match_dir(true, blocks, bv, job0, ratio, s);
+            }));
+    return ORBFE_OK;
+}
+
+// Layout of a detect batch: table (jobs, tile prefix, desc prefix) and per
+// frame scratch in one arena.
+struct DetectLayout {
+    std::vector<Job> jobs;
+    std::vector<uint32_t> tile_prefix, desc_prefix;
+    size_t table_bytes = 0, arena_bytes = 0;
+    size_t off_tiles = 0, off_desc = 0;
+};
+
+int layout_detect(orbfe_ctx* ctx, const orbfe_frame* frames, uint32_t n, DetectLayout& lay,
+                  uint8_t* arena) {
+    lay.jobs.resize(n);
+    lay.tile_prefix.assign(n + 1, 0);
+    lay.desc_prefix.assign(n + 1, 0);
+    size_t off = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const orbfe_frame& f = frames[i];
+        if (!f.image || !f.features)
+            return fail(ctx, ORBFE_EINVAL, "frame %u: null image or features", i);
+        if (f.pitch < f.width)
+            return fail(ctx, ORBFE_EINVAL, "frame %u: pitch %zu < width %u", i, f.pitch, f.width);
+        PlanEntry* pe = nullptr;
+        const int rc = get_plan(ctx, f.width, f.height, f.cap, &pe);
+        if (rc) return rc;
+        Job& J = lay.jobs[i];
+        J.plan = pe->dev;
+        J.img = f.image;
+        J.pitch = f.pitch;
+        J.feat = static_cast<uint8_t*>(f.features);
+        J.surv = reinterpret_cast<uint64_t*>(arena + off);
+        off += pe->surv_bytes;
+        J.kp = reinterpret_cast<uint32_t*>(arena + off);
+        off += pe->kp_bytes;
+        if (f.pyramid) {
+            J.pyr = f.pyramid;
+        } else {
+            J.pyr = reinterpret_cast<float*>(arena + off);
+            off += pe->pyr_bytes;
+        }
+        lay.tile_prefix[i + 1] = lay.tile_prefix[i] + pe->host.tiles;
+        lay.desc_prefix[i + 1] = lay.desc_prefix[i] + pe->host.L * pe->host.desc_groups;
+    }
+    lay.arena_bytes = off;
+    lay.off_tiles = align_up(sizeof(Job) * n);
+    lay.off_desc = lay.off_tiles + align_up(4 * (n + 1));
+    lay.table_bytes = lay.off_desc + align_up(4 * (n + 1));
+    return ORBFE_OK;
+}
+
+void pack_detect_table(const DetectLayout& lay, uint8_t* host) {
+    std::memcpy(host, lay.jobs.data(), sizeof(Job) * lay.jobs.size());
+    std::memcpy(host + lay.off_tiles, lay.tile_prefix.data(), 4 * lay.tile_prefix.size());
+    std::memcpy(host + lay.off_desc, lay.desc_prefix.data(), 4 * lay.desc_prefix.size());
+}
+
+BatchView detect_view(const DetectLayout& lay, const uint8_t* dev) {
+    BatchView v;
+    v.jobs = reinterpret_cast<const Job*>(dev);
+    v.tile_prefix = reinterpret_cast<const uint32_t*>(dev + lay.off_tiles);
+    v.desc_prefix = reinterpret_cast<const uint32_t*>(dev + lay.off_desc);
+    v.n = static_cast<uint32_t>(lay.jobs.size());
+    return v;
+}
+
+struct MatchLayout {
+    std::vector<MatchJob> jobs;
+    std::vector<uint32_t> prefix;
+    size_t table_bytes = 0, arena_bytes = 0, off_prefix = 0;
+};
+
+int layout_match(orbfe_ctx* ctx, const orbfe_match_pair* pairs, uint32_t n, MatchLayout& lay,
+                 uint8_t* arena) {
+    lay.jobs.resize(n);
+    lay.prefix.assign(n + 1, 0);
+    size_t off = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const orbfe_match_pair& p = pairs[i];
+        if (!p.prev_features || !p.curr_features || !p.prev_match)
+            return fail(ctx, ORBFE_EINVAL, "pair %u: null pointer", i);
+        PlanEntry* pe = nullptr;
+        const int rc = get_plan(ctx, p.width, p.height, p.cap, &pe);
+        if (rc) return rc;
+        MatchJob& M = lay.jobs[i];
+        M.plan = pe->dev;
+        M.prev = static_cast<const uint8_t*>(p.prev_features);
+        M.curr = static_cast<const uint8_t*>(p.curr_features);
+        M.prev_match = p.prev_match;
+        M.b_to_a = reinterpret_cast<int32_t*>(arena + off);
+        off += pe->match_bytes;
+        lay.prefix[i + 1] = lay.prefix[i] + pe->host.L * pe->host.desc_groups;
+    }
+    lay.arena_bytes = off;
+    lay.off_prefix = align_up(sizeof(MatchJob) * n);
+    lay.table_bytes = lay.off_prefix + align_up(4 * (n + 1));
+    return ORBFE_OK;
+}
+
+void pack_match_table(const MatchLayout& lay, uint8_t* host) {
+    std::memcpy(host, lay.jobs.data(), sizeof(MatchJob) * lay.jobs.size());
+    std::memcpy(host + lay.off_prefix, lay.prefix.data(), 4 * lay.prefix.size());
+}
+
+MatchBatchView match_view(const MatchLayout& lay, const uint8_t* dev) {
+    MatchBatchView v;
+    v.jobs = reinterpret_cast<const MatchJob*>(dev);
+    v.block_prefix = reinterpret_cast<const uint32_t*>(dev + lay.off_prefix);
+    v.n = static_cast<uint32_t>(lay.jobs.size());
+    return v;
+}
+
+// Dynamic batches: the arena grows on demand (after the stream that used it
+// has drained) and tables go through a ring of pinned staging slots.
+int ensure_arena(orbfe_ctx* ctx, size_t bytes, hipStream_t s) {
+    if (bytes <= ctx->arena_bytes) return ORBFE_OK;
+    if (ctx->arena) {
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->arena_stream));
+        HIP_TRY(ctx, hipStreamSynchronize(s));
+        dfree(ctx->arena);
+    }
+    const size_t want = align_up(std::max(bytes, ctx->arena_bytes * 2), 1 << 20);
+    const int rc = dalloc(ctx, &ctx->arena, want);
+    if (rc) return rc;
+    ctx->arena_bytes = want;
+    ctx->arena_stream = s;
+    return ORBFE_OK;
+}
+
+int stage_table(orbfe_ctx* ctx, size_t bytes, hipStream_t s, StageSlot** out) {
+    StageSlot& slot = ctx->ring[ctx->ring_next];
+    ctx->ring_next = (ctx->ring_next + 1) % 4;
+    if (slot.pending) {
+        HIP_TRY(ctx, hipEventSynchronize(slot.ev));
+        slot.pending = false;
+    }
+    if (slot.bytes < bytes) {
+        if (slot.host) (void)hipHostFree(slot.host);
+        dfree(slot.dev);
+        slot.host = nullptr;
+        const size_t want = align_up(std::max<size_t>(bytes, 64 << 10), 64 << 10);
+        HIP_TRY(ctx, hipHostMalloc(&slot.host, want, hipHostMallocDefault));
+        const int rc = dalloc(ctx, &slot.dev, want);
+        if (rc) return rc;
+        slot.bytes = want;
+    }
+    if (!slot.ev) HIP_TRY(ctx, hipEventCreateWithFlags(&slot.ev, hipEventDisableTiming));
+    (void)s;
+    *out = &slot;
+    return ORBFE_OK;
+}
+
+int commit_table(orbfe_ctx* ctx, StageSlot* slot, size_t bytes, hipStream_t s) {
+    HIP_TRY(ctx, hipMemcpyAsync(slot->dev, slot->host, bytes, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, hipEventRecord(slot->ev, s));
+    slot->pending = true;
+    return ORBFE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orbfe_config_default(orbfe_config* cfg, uint32_t width, uint32_t height) {
+    if (!cfg) return ORBFE_EINVAL;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->frame_width = width;
+    cfg->frame_height = height;
+    cfg->levels = 6;
+    cfg->scale_factor = 1.2;
+    cfg->fast_threshold = 0.15f;
+    cfg->fast_min_run = 9;
+    cfg->orientation_radius = 3;
+    cfg->features_per_level = 0;
+    cfg->border_zero = 0;
+    cfg->lowe_ratio = 0.85f;
+    return ORBFE_OK;
+}
+
+const char* orbfe_last_error(const orbfe_ctx* ctx) {
+    return ctx ? ctx->err.c_str() : "null context";
+}
+
+void orbfe_destroy(orbfe_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    (void)hipDeviceSynchronize();
+    for (auto& kv : ctx->plans) {
+        dfree(kv.second->mask);
+        dfree(kv.second->dev);
+    }
+    dfree(ctx->surv);
+    dfree(ctx->kp);
+    dfree(ctx->pyr_scratch);
+    dfree(ctx->b_to_a);
+    dfree(ctx->d_img);
+    dfree(ctx->d_pyr);
+    dfree(ctx->d_feat[0]);
+    dfree(ctx->d_feat[1]);
+    dfree(ctx->d_match);
+    dfree(ctx->bf_part);
+    dfree(ctx->arena);
+    for (auto& slot : ctx->ring) {
+        if (slot.host) (void)hipHostFree(slot.host);
+        dfree(slot.dev);
+        if (slot.ev) (void)hipEventDestroy(slot.ev);
+    }
+    for (auto& e : ctx->pending) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    for (auto e : ctx->pool) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int orbfe_create(const orbfe_config* cfg, int device, orbfe_ctx** out) {
+    if (!cfg || !out) return ORBFE_EINVAL;
+    *out = nullptr;
+    if (cfg->levels == 0 || cfg->levels > ORBFE_MAX_LEVELS) return ORBFE_ERANGE;
+    if (!(cfg->scale_factor > 1.0)) return ORBFE_ERANGE;
+    if (cfg->orientation_radius < 0 || cfg->orientation_radius > 15) return ORBFE_ERANGE;
+    if (cfg->fast_min_run > 16) return ORBFE_ERANGE;
+    auto ctx = std::make_unique<orbfe_ctx>();
+    ctx->device = device;
+    ctx->cfg = *cfg;
+    ctx->prm.threshold = cfg->fast_threshold;
+    ctx->prm.min_run = cfg->fast_min_run;
+    ctx->prm.orient_r = cfg->orientation_radius;
+    ctx->prm.border_zero = cfg->border_zero ? 1u : 0u;
+    ctx->prm.ratio = cfg->lowe_ratio;
+    orbfe_ctx* c = ctx.get();
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        // Report through the return code only: there is no context to own text.
+        return ORBFE_EHIP;
+    }
+    if (device < 0 || device >= ndev) return ORBFE_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return ORBFE_EHIP;
+    if (orbfe::upload_pattern() != hipSuccess) return ORBFE_EHIP;
+    if (hipStreamCreate(&c->stream) != hipSuccess) return ORBFE_EHIP;
+    PlanEntry* pe = nullptr;
+    if (get_plan(c, cfg->frame_width, cfg->frame_height, cfg->features_per_level, &pe))
+        return ORBFE_EINVAL;
+    c->main = pe;
+    if (dalloc(c, &c->surv, pe->surv_bytes) || dalloc(c, &c->kp, pe->kp_bytes) ||
+        dalloc(c, &c->pyr_scratch, pe->pyr_bytes) || dalloc(c, &c->b_to_a, pe->match_bytes) ||
+        dalloc(c, &c->d_img, align_up(static_cast<size_t>(cfg->frame_width) * cfg->frame_height)) ||
+        dalloc(c, &c->d_pyr, pe->pyr_bytes) || dalloc(c, &c->d_feat[0], pe->feat_bytes) ||
+        dalloc(c, &c->d_feat[1], pe->feat_bytes) || dalloc(c, &c->d_match, pe->match_bytes)) {
+        orbfe_destroy(ctx.release());
+        return ORBFE_ENOMEM;
+    }
+    *out = ctx.release();
+    return ORBFE_OK;
+}
+
+int orbfe_level_dims(const orbfe_ctx* ctx, uint32_t* widths, uint32_t* heights) {
+    if (!ctx || !widths || !heights) return ORBFE_EINVAL;
+    for (uint32_t l = 0; l < ctx->cfg.levels; ++l) {
+        widths[l] = ctx->main->host.lv[l].w;
+        heights[l] = ctx->main->host.lv[l].h;
+    }
+    return ORBFE_OK;
+}
+
+size_t orbfe_feature_array_bytes(const orbfe_ctx* ctx) {
+    return ctx ? static_cast<size_t>(ctx->cfg.levels) * ctx->main->host.feat_stride : 0;
+}
+
+size_t orbfe_pyramid_floats(const orbfe_ctx* ctx) {
+    return ctx ? ctx->main->host.pyr_floats : 0;
+}
+
+uint32_t orbfe_cap(const orbfe_ctx* ctx) { return ctx ? ctx->main->host.cap : 0; }
+
+int orbfe_detect_async(orbfe_ctx* ctx, const uint8_t* image, size_t pitch, float* pyramid,
+                       void* features, hipStream_t stream) {
+    if (!ctx) return ORBFE_EINVAL;
+    if (!image || !features) return fail(ctx, ORBFE_EINVAL, "null image or features");
+    const Plan& P = ctx->main->host;
+    if (pitch < P.W0) return fail(ctx, ORBFE_EINVAL, "pitch %zu < width %u", pitch, P.W0);
+    Job job0{};
+    job0.plan = ctx->main->dev;
+    job0.img = image;
+    job0.pitch = pitch;
+    job0.pyr = pyramid ? pyramid : ctx->pyr_scratch;
+    job0.feat = static_cast<uint8_t*>(features);
+    job0.surv = ctx->surv;
+    job0.kp = ctx->kp;
+    const BatchView bv{nullptr, nullptr, nullptr, 0};
+    return launch_detect_stage(ctx, bv, job0, P.tiles, P.L, P.L * P.desc_groups, stream);
+}
+
+int orbfe_detect(orbfe_ctx* ctx, const uint8_t* image, size_t pitch, float* pyramid,
+                 void* features) {
+    if (!ctx) return ORBFE_EINVAL;
+    if (!image || !features) return fail(ctx, ORBFE_EINVAL, "null image or features");
+    const Plan& P = ctx->main->host;
+    if (pitch < P.W0) return fail(ctx, ORBFE_EINVAL, "pitch %zu < width %u", pitch, P.W0);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    HIP_TRY(ctx, hipMemcpy2DAsync(ctx->d_img, P.W0, image, pitch, P.W0, P.H0,
+                                  hipMemcpyHostToDevice, s));
+    const int rc = orbfe_detect_async(ctx, ctx->d_img, P.W0, pyramid ? ctx->d_pyr : nullptr,
+                                      ctx->d_feat[0], s);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipMemcpyAsync(features, ctx->d_feat[0],
+                                static_cast<size_t>(P.L) * P.feat_stride,
+                                hipMemcpyDeviceToHost, s));
+    if (pyramid)
+        HIP_TRY(ctx, hipMemcpyAsync(pyramid, ctx->d_pyr, static_cast<size_t>(P.pyr_floats) * 4,
+                                    hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    return ORBFE_OK;
+}
+
+int orbfe_detect_batch_async(orbfe_ctx* ctx, const orbfe_frame* frames, uint32_t n,
+                             hipStream_t stream) {
+    if (!ctx) return ORBFE_EINVAL;
+    if (n == 0) return ORBFE_OK;
+    if (!frames) return fail(ctx, ORBFE_EINVAL, "null frames");
+    DetectLayout lay;
+    int rc = layout_detect(ctx, frames, n, lay, nullptr);
+    if (rc) return rc;
+    rc = ensure_arena(ctx, lay.arena_bytes, stream);
+    if (rc) return rc;
+    rc = layout_detect(ctx, frames, n, lay, ctx->arena);
+    if (rc) return rc;
+    StageSlot* slot = nullptr;
+    rc = stage_table(ctx, lay.table_bytes, stream, &slot);
+    if (rc) return rc;
+    pack_detect_table(lay, static_cast<uint8_t*>(slot->host));
+    rc = commit_table(ctx, slot, lay.table_bytes, stream);
+    if (rc) return rc;
+    const BatchView bv = detect_view(lay, static_cast<uint8_t*>(slot->dev));
+    return launch_detect_stage(ctx, bv, Job{}, lay.tile_prefix[n], n * ctx->cfg.levels,
+                               lay.desc_prefix[n], stream);
+}
+
+int orbfe_batch_create(orbfe_ctx* ctx, const orbfe_frame* frames, uint32_t n,
+                       orbfe_batch** out) {
+    if (!ctx || !out) return ORBFE_EINVAL;
+    *out = nullptr;
+    if (n == 0 || !frames) return fail(ctx, ORBFE_EINVAL, "empty batch");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    DetectLayout lay;
+    int rc = layout_detect(ctx, frames, n, lay, nullptr);
+    if (rc) return rc;
+    auto b = std::make_unique<orbfe_batch>();
+    rc = dalloc(ctx, &b->arena, lay.arena_bytes);
+    if (rc) return rc;
+    rc = layout_detect(ctx, frames, n, lay, b->arena);
+    if (rc) { dfree(b->arena); return rc; }
+    std::vector<uint8_t> host(lay.table_bytes);
+    pack_detect_table(lay, host.data());
+    uint8_t* table = nullptr;
+    rc = dalloc(ctx, &table, lay.table_bytes);
+    if (rc) { dfree(b->arena); return rc; }
+    b->table = table;
+    if (hipMemcpy(table, host.data(), lay.table_bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        dfree(b->arena);
+        dfree(table);
+        return fail(ctx, ORBFE_EHIP, "batch table upload failed");
+    }
+    b->ctx = ctx;
+    b->n = n;
+    b->tiles = lay.tile_prefix[n];
+    b->desc_blocks = lay.desc_prefix[n];
+    b->view = detect_view(lay, table);
+    *out = b.release();
+    return ORBFE_OK;
+}
+
+int orbfe_batch_detect_async(orbfe_ctx* ctx, orbfe_batch* batch, hipStream_t stream) {
+    if (!ctx || !batch || batch->ctx != ctx) return fail(ctx, ORBFE_EINVAL, "bad batch");
+    return launch_detect_stage(ctx, batch->view, Job{}, batch->tiles, batch->n * ctx->cfg.levels,
+                               batch->desc_blocks, stream);
+}
+
+void orbfe_batch_destroy(orbfe_batch* batch) {
+    if (!batch) return;
+    (void)hipDeviceSynchronize();
+    dfree(batch->arena);
+    if (batch->table) (void)hipFree(batch->table);
+    delete batch;
+}
+
+int orbfe_load_features(const void* features, uint32_t levels, uint32_t cap, orbfe_feature* out,
+                        uint32_t* counts) {
+    if (!features || !out || !counts || levels == 0 || levels > ORBFE_MAX_LEVELS)
+        return ORBFE_EINVAL;
+    const size_t stride = 8 + 48ull * cap;
+    const uint8_t* base = static_cast<const uint8_t*>(features);
+    size_t n = 0;
+    for (uint32_t l = 0; l < levels; ++l) {
+        const uint8_t* block = base + stride * l;
+        uint32_t count;
+        std::memcpy(&count, block, 4);
+        const uint32_t take = std::min(count, cap);
+        for (uint32_t i = 0; i < take; ++i) {
+            orbfe_gpu_feature g;
+            std::memcpy(&g, block + 8 + 48ull * i, sizeof(g));
+            orbfe_feature& f = out[n++];
+            f.x = static_cast<uint32_t>(g.x);
+            f.y = static_cast<uint32_t>(g.y);
+            f.lod = l;
+            f.strength = g.strength;
+            f.orientation = g.orientation;
+            std::memcpy(f.descriptor, g.descriptor, 32);
+        }
+        counts[l] = take;
+    }
+    return ORBFE_OK;
+}
+
+int orbfe_match_pipeline_async(orbfe_ctx* ctx, const void* prev_features,
+                               const void* curr_features, int32_t* prev_match,
+                               hipStream_t stream) {
+    if (!ctx) return ORBFE_EINVAL;
+    if (!prev_features || !curr_features || !prev_match)
+        return fail(ctx, ORBFE_EINVAL, "null pointer");
+    const Plan& P = ctx->main->host;
+    MatchJob job0{};
+    job0.plan = ctx->main->dev;
+    job0.prev = static_cast<const uint8_t*>(prev_features);
+    job0.curr = static_cast<const uint8_t*>(curr_features);
+    job0.prev_match = prev_match;
+    job0.b_to_a = ctx->b_to_a;
+    const MatchBatchView bv{nullptr, nullptr, 0};
+    return launch_match_stage(ctx, bv, job0, P.L * P.desc_groups, stream);
+}
+
+int orbfe_match_pipeline_batch_async(orbfe_ctx* ctx, const orbfe_match_pair* pairs, uint32_t n,
+                                     hipStream_t stream) {
+    if (!ctx) return ORBFE_EINVAL;
+    if (n == 0) return ORBFE_OK;
+    if (!pairs) return fail(ctx, ORBFE_EINVAL, "null pairs");
+    MatchLayout lay;
+    int rc = layout_match(ctx, pairs, n, lay, nullptr);
+    if (rc) return rc;
+    rc = ensure_arena(ctx, lay.arena_bytes, stream);
+    if (rc) return rc;
+    rc = layout_match(ctx, pairs, n, lay, ctx->arena);
+    if (rc) return rc;
+    StageSlot* slot = nullptr;
+    rc = stage_table(ctx, lay.table_bytes, stream, &slot);
+    if (rc) return rc;
+    pack_match_table(lay, static_cast<uint8_t*>(slot->host));
+    rc = commit_table(ctx, slot, lay.table_bytes, stream);
+    if (rc) return rc;
+    const MatchBatchView bv = match_view(lay, static_cast<uint8_t*>(slot->dev));
+    return launch_match_stage(ctx, bv, MatchJob{}, lay.prefix[n], stream);
+}
+
+int orbfe_match_batch_create(orbfe_ctx* ctx, const orbfe_match_pair* pairs, uint32_t n,
+                             orbfe_match_batch** out) {
+    if (!ctx || !out) return ORBFE_EINVAL;
+    *out = nullptr;
+    if (n == 0 || !pairs) return fail(ctx, ORBFE_EINVAL, "empty batch");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    MatchLayout lay;
+    int rc = layout_match(ctx, pairs, n, lay, nullptr);
+    if (rc) return rc;
+    auto b = std::make_unique<orbfe_match_batch>();
+    rc = dalloc(ctx, &b->arena, lay.arena_bytes);
+    if (rc) return rc;
+    rc = layout_match(ctx, pairs, n, lay, b->arena);
+    if (rc) { dfree(b->arena); return rc; }
+    std::vector<uint8_t> host(lay.table_bytes);
+    pack_match_table(lay, host.data());
+    uint8_t* table = nullptr;
+    rc = dalloc(ctx, &table, lay.table_bytes);
+    if (rc) { dfree(b->arena); return rc; }
+    b->table = table;
+    if (hipMemcpy(table, host.data(), lay.table_bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        dfree(b->arena);
+        dfree(table);
+        return fail(ctx, ORBFE_EHIP, "match table upload failed");
+    }
+    b->ctx = ctx;
+    b->n = n;
+    b->blocks = lay.prefix[n];
+    b->view = match_view(lay, table);
+    *out = b.release();
+    return ORBFE_OK;
+}
+
+int orbfe_match_batch_async(orbfe_ctx* ctx, orbfe_match_batch* batch, hipStream_t stream) {
+    if (!ctx || !batch || batch->ctx != ctx) return fail(ctx, ORBFE_EINVAL, "bad batch");
+    return launch_match_stage(ctx, batch->view, MatchJob{}, batch->blocks, stream);
+}
+
+void orbfe_match_batch_destroy(orbfe_match_batch* batch) {
+    if (!batch) return;
+    (void)hipDeviceSynchronize();
+    dfree(batch->arena);
+    if (batch->table) (void)hipFree(batch->table);
+    delete batch;
+}
+
+int orbfe_match_pipeline(orbfe_ctx* ctx, const void* prev_features, const void* curr_features,
+                         int32_t* prev_match, uint32_t* n_pairs) {
+    if (!ctx) return ORBFE_EINVAL;
+    if (!prev_features || !curr_features || !prev_match)
+        return fail(ctx, ORBFE_EINVAL, "null pointer");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const Plan& P = ctx->main->host;
+    const size_t fb = static_cast<size_t>(P.L) * P.feat_stride;
+    const size_t mb = static_cast<size_t>(P.L) * P.cap * 4;
+    hipStream_t s = ctx->stream;
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->d_feat[0], prev_features, fb, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->d_feat[1], curr_features, fb, hipMemcpyHostToDevice, s));
+    const int rc = orbfe_match_pipeline_async(ctx, ctx->d_feat[0], ctx->d_feat[1], ctx->d_match, s);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipMemcpyAsync(prev_match, ctx->d_match, mb, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    if (n_pairs) {
+        uint32_t c = 0;
+        for (size_t i = 0; i < mb / 4; ++i) c += prev_match[i] >= 0 ? 1u : 0u;
+        *n_pairs = c;
+    }
+    return ORBFE_OK;
+}
+
+int orbfe_match_bf_async(orbfe_ctx* ctx, const uint32_t* query, uint32_t nq,
+                         const uint32_t* train, uint32_t nt, uint32_t* idx, uint32_t* d1,
+                         uint32_t* d2, hipStream_t stream) {
+    if (!ctx) return ORBFE_EINVAL;
+    if (nq == 0) return ORBFE_OK;
+    if (!query || !idx || !d1 || !d2 || (nt > 0 && !train))
+        return fail(ctx, ORBFE_EINVAL, "null pointer");
+    if (nt >= (1u << orbfe::kBfKeyShift))
+        return fail(ctx, ORBFE_ERANGE, "train count %u >= 2^%u", nt, orbfe::kBfKeyShift);
+    // Enough workgroups to cover the 256 CUs several times over: split the
+    // train set when the query blocks alone are too few.
+    const uint32_t qblocks = ceil_div(nq, orbfe::bf_queries_per_block());
+    uint32_t splits = 1;
+    if (nt > 0) {
+        const uint32_t want = std::max<uint32_t>(1, 2048 / qblocks);
+        const uint32_t max_splits = std::max<uint32_t>(1, nt / 256);
+        splits = std::min(want, max_splits);
+    }
+    const uint32_t split_len = nt > 0 ? ceil_div(nt, splits) : 0;
+    const size_t need = static_cast<size_t>(splits) * nq * sizeof(uint2);
+    if (need > ctx->bf_part_bytes) {
+        HIP_TRY(ctx, hipStreamSynchronize(stream));
+        dfree(ctx->bf_part);
+        const int rc = dalloc(ctx, &ctx->bf_part, need);
+        if (rc) {
+            ctx->bf_part_bytes = 0;
+            return rc;
+        }
+        ctx->bf_part_bytes = need;
+    }
+    HIP_TRY(ctx, timed(ctx, "k_bf_partial", stream, [&] {
+                return orbfe::launch_bf(query, nq, train, nt, splits, split_len, ctx->bf_part,
+                                        idx, d1, d2, stream, 0);
+            }));
+    HIP_TRY(ctx, timed(ctx, "k_bf_merge", stream, [&] {
+                return orbfe::launch_bf(query, nq, train, nt, splits, split_len, ctx->bf_part,
+                                        idx, d1, d2, stream, 1);
+            }));
+    return ORBFE_OK;
+}
+
+int orbfe_timing_enable(orbfe_ctx* ctx, int enable) {
+    if (!ctx) return ORBFE_EINVAL;
+    ctx->timing = enable != 0;
+    return ORBFE_OK;
+}
+
+int orbfe_timing_reset(orbfe_ctx* ctx) {
+    if (!ctx) return ORBFE_EINVAL;
+    for (auto& e : ctx->pending) {
+        HIP_TRY(ctx, hipEventSynchronize(e.b));
+        ctx->pool.push_back(e.a);
+        ctx->pool.push_back(e.b);
+    }
+    ctx->pending.clear();
+    ctx->totals.clear();
+    return ORBFE_OK;
+}
+
+int orbfe_timing_read(orbfe_ctx* ctx, const char* name, uint32_t* launches, double* total_ms) {
+    if (!ctx || !name) return ORBFE_EINVAL;
+    for (auto& e : ctx->pending) {
+        HIP_TRY(ctx, hipEventSynchronize(e.b));
+        float ms = 0.0f;
+        HIP_TRY(ctx, hipEventElapsedTime(&ms, e.a, e.b));
+        auto& t = ctx->totals[e.name];
+        t.first += 1;
+        t.second += ms;
+        ctx->pool.push_back(e.a);
+        ctx->pool.push_back(e.b);
+    }
+    ctx->pending.clear();
+    const auto it = ctx->totals.find(name);
+    if (launches) *launches = it == ctx->totals.end() ? 0 : it->second.first;
+    if (total_ms) *total_ms = it == ctx->totals.end() ? 0.0 : it->second.second;
+    return ORBFE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,221 @@
+// Shared definitions of the orbfe HIP kernels and their host launchers.
+//
+// Arithmetic contract (the HIP path and the CPU restatement used as its
+// checker agree on every bit of it):
+//   * bilinear sampling (textureSampleLevel with a linear clamp-to-edge
+//     sampler): u = uv*dim, u' = u - 0.5, i0 = floor(u'), a = u' - i0, both
+//     taps clamped to [0, dim-1], lerp a + t*(b - a) as separate multiply and
+//     add, x then y;
+//   * f32 division correctly rounded; no FMA contraction (the library is
+//     compiled with -ffp-contract=off) except the explicit fmaf calls of the
+//     trigonometric routines below;
+//   * textureLoad out of range: Tint's robustness clamp min(u32(c), dim-1)
+//     (vendor/dawn/src/tint/lang/core/ir/transform/robustness.cc:344-355), so
+//     a negative coordinate reads the far edge, or zero when border_zero.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace orbfe {
+
+constexpr uint32_t kMaxLevels = 8;
+
+// Detect tile: 64 columns (one wave per row segment) x 16 rows, 256 threads,
+// each thread scoring 4 rows.  The halo of 3 covers the radius-3 FAST ring.
+constexpr int kTileW = 64;
+constexpr int kTileH = 16;
+constexpr int kHalo = 3;
+constexpr int kLdsW = kTileW + 2 * kHalo;  // 70
+constexpr int kLdsH = kTileH + 2 * kHalo;  // 22
+constexpr int kDetectThreads = 256;
+
+// Key packing of the pipeline matcher: distance (<= 256) above a 22-bit
+// candidate id (2 bits pool level slot, 20 bits index).
+constexpr uint32_t kMatchKeyShift = 22;
+constexpr uint32_t kMatchIdxBits = 20;
+// Brute-force matcher keys: distance above a 23-bit train index.
+constexpr uint32_t kBfKeyShift = 23;
+
+struct LevelGeom {
+    uint32_t w, h;
+    uint32_t wpr;        // 64-pixel words per row, ceil(w / 64)
+    uint32_t tiles_x, tiles_y;
+    uint32_t tile_base;  // first detect tile of this level within the frame
+    uint32_t pyr_off;    // float offset in the pyramid buffer (levels >= 1)
+    uint32_t word_off;   // 64-bit word offset in the cull mask / survivor map
+    uint32_t feat_off;   // byte offset of the level block in the feature array
+    uint32_t kp_off;     // entry offset of the level's keypoint list
+    float lod0_scale;    // product of l factors 1.2f (toLod0)
+    uint32_t pad;
+};
+
+struct Plan {
+    uint32_t W0, H0, L, cap;
+    uint32_t tiles;       // detect tiles of one frame
+    uint32_t words;       // 64-bit survivor words of one frame
+    uint32_t pyr_floats;  // pyramid floats (levels >= 1)
+    uint32_t feat_stride; // 8 + 48*cap
+    uint32_t desc_groups; // describe blocks per level = ceil(cap / 4)
+    float max_dx, max_dy; // 0.1f*W0, 0.1f*H0 (match_features_cpu.cpp:76-77)
+    uint32_t pad;
+    const uint64_t* mask; // static cull mask, `words` words
+    LevelGeom lv[kMaxLevels];
+};
+
+struct Job {
+    const Plan* plan;
+    const uint8_t* img;
+    uint64_t pitch;
+    float* pyr;        // nullable
+    uint8_t* feat;     // feature array
+    uint64_t* surv;    // survivor bitmap scratch, plan->words
+    uint32_t* kp;      // keypoint list scratch, L*cap (x | y << 16)
+};
+
+struct MatchJob {
+    const Plan* plan;
+    const uint8_t* prev;  // feature arrays
+    const uint8_t* curr;
+    int32_t* prev_match;  // L*cap
+    int32_t* b_to_a;      // scratch L*cap
+};
+
+struct Params {
+    float threshold;
+    uint32_t min_run;
+    int32_t orient_r;
+    uint32_t border_zero;
+    float ratio;
+    uint32_t pad[3];
+};
+
+// Batch tables in device memory: jobs and exclusive prefix sums of the
+// per-job block counts of each kernel.
+struct BatchView {
+    const Job* jobs;
+    const uint32_t* tile_prefix;  // n+1
+    const uint32_t* desc_prefix;  // n+1
+    uint32_t n;
+};
+
+struct MatchBatchView {
+    const MatchJob* jobs;
+    const uint32_t* block_prefix;  // n+1
+    uint32_t n;
+};
+
+// ---------------------------------------------------------------------------
+// Own f32 math: Cody & Waite (1980) reduction by pi/2 with a three-part
+// split of pi/2 and Taylor polynomials of sin and cos on [-pi/4, pi/4];
+// atan2 through atan on [0, 1], using atan(t) = pi/4 + atan((t-1)/(t+1))
+// above tan(pi/8) and the Taylor series of atan below it.  Each constant is
+// the f32 rounding of the real number named beside it.
+// ---------------------------------------------------------------------------
+namespace fm {
+constexpr float kTwoOverPi = 0x1.45f306p-1f;  // 2/pi
+constexpr float kPio2Hi = 0x1.921fb6p+0f;     // pi/2
+constexpr float kPio2Mid = -0x1.777a5cp-25f;  // pi/2 - kPio2Hi
+constexpr float kPio2Lo = -0x1.ee59dap-50f;   // pi/2 - kPio2Hi - kPio2Mid
+constexpr float kPio4 = 0x1.921fb6p-1f;       // pi/4
+constexpr float kPi = 0x1.921fb6p+1f;         // pi
+constexpr float kTanPi8 = 0x1.a8279ap-2f;     // tan(pi/8) = sqrt(2) - 1
+}  // namespace fm
+
+__host__ __device__ inline float atan_taylor(float t) {
+    // sum_{k=0..10} (-1)^k t^(2k+1) / (2k+1), Horner in t^2
+    const float t2 = t * t;
+    float p = 0x1.861862p-5f;               //  1/21
+    p = __builtin_fmaf(p, t2, -0x1.af286cp-5f);  // -1/19
+    p = __builtin_fmaf(p, t2, 0x1.e1e1e2p-5f);   //  1/17
+    p = __builtin_fmaf(p, t2, -0x1.111112p-4f);  // -1/15
+    p = __builtin_fmaf(p, t2, 0x1.3b13b2p-4f);   //  1/13
+    p = __builtin_fmaf(p, t2, -0x1.745d18p-4f);  // -1/11
+    p = __builtin_fmaf(p, t2, 0x1.c71c72p-4f);   //  1/9
+    p = __builtin_fmaf(p, t2, -0x1.24924ap-3f);  // -1/7
+    p = __builtin_fmaf(p, t2, 0x1.99999ap-3f);   //  1/5
+    p = __builtin_fmaf(p, t2, -0x1.555556p-2f);  // -1/3
+    const float t3 = t2 * t;
+    return __builtin_fmaf(p, t3, t);
+}
+
+__host__ __device__ inline float atan2_own(float y, float x) {
+    const float ax = __builtin_fabsf(x);
+    const float ay = __builtin_fabsf(y);
+    const bool swap = ay > ax;
+    const float mx = swap ? ay : ax;
+    const float mn = swap ? ax : ay;
+    const float t = mx == 0.0f ? 0.0f : mn / mx;
+    float a;
+    if (t > fm::kTanPi8) {
+        const float u = (t - 1.0f) / (t + 1.0f);
+        a = fm::kPio4 + atan_taylor(u);
+    } else {
+        a = atan_taylor(t);
+    }
+    if (swap) a = fm::kPio2Hi - a;
+    if (__builtin_signbit(x)) a = fm::kPi - a;
+    if (__builtin_signbit(y)) a = -a;
+    return a;
+}
+
+__host__ __device__ inline void sincos_own(float x, float* s, float* c) {
+    const float k = __builtin_rintf(x * fm::kTwoOverPi);
+    float r = __builtin_fmaf(-k, fm::kPio2Hi, x);
+    r = __builtin_fmaf(-k, fm::kPio2Mid, r);
+    r = __builtin_fmaf(-k, fm::kPio2Lo, r);
+    const float r2 = r * r;
+    // sin: r - r^3/3! + r^5/5! - r^7/7! + r^9/9!
+    float ps = __builtin_fmaf(0x1.71de3ap-19f, r2, -0x1.a01a02p-13f);
+    ps = __builtin_fmaf(ps, r2, 0x1.111112p-7f);
+    ps = __builtin_fmaf(ps, r2, -0x1.555556p-3f);
+    const float r3 = r2 * r;
+    const float sn = __builtin_fmaf(ps, r3, r);
+    // cos: 1 - r^2/2! + r^4/4! - r^6/6! + r^8/8! - r^10/10!
+    float pc = __builtin_fmaf(-0x1.27e4fcp-22f, r2, 0x1.a01a02p-16f);
+    pc = __builtin_fmaf(pc, r2, -0x1.6c16c2p-10f);
+    pc = __builtin_fmaf(pc, r2, 0x1.555556p-5f);
+    pc = __builtin_fmaf(pc, r2, -0x1p-1f);
+    const float cs = __builtin_fmaf(pc, r2, 1.0f);
+    const int q = static_cast<int>(k) & 3;
+    const float s0 = (q & 1) ? cs : sn;
+    const float c0 = (q & 1) ? sn : cs;
+    *s = (q & 2) ? -s0 : s0;
+    *c = ((q + 1) & 2) ? -c0 : c0;
+}
+
+// One axis of the linear clamp-to-edge sampler.
+struct AxisTaps {
+    uint32_t i0, i1;
+    float frac;
+};
+
+__host__ __device__ inline AxisTaps axis_taps(float coord, uint32_t n) {
+    const float u = coord * static_cast<float>(n);
+    const float uu = u - 0.5f;
+    const float f = __builtin_floorf(uu);
+    AxisTaps t;
+    t.frac = uu - f;
+    const float hi = static_cast<float>(n - 1);
+    const float f1 = f + 1.0f;
+    const float c0 = f < 0.0f ? 0.0f : (f > hi ? hi : f);
+    const float c1 = f1 < 0.0f ? 0.0f : (f1 > hi ? hi : f1);
+    t.i0 = static_cast<uint32_t>(c0);
+    t.i1 = static_cast<uint32_t>(c1);
+    return t;
+}
+
+__host__ __device__ inline float lerp2(float t00, float t10, float t01, float t11,
+                                       float a, float b) {
+    const float top = t00 + a * (t10 - t00);
+    const float bot = t01 + a * (t11 - t01);
+    return top + b * (bot - top);
+}
+
+// Tint's textureLoad clamp of one coordinate (int in, index out).
+__host__ __device__ inline uint32_t tint_clamp(int32_t c, uint32_t n) {
+    const uint32_t u = static_cast<uint32_t>(c);
+    return u < n - 1 ? u : n - 1;
+}
+
+}  // namespace orbfe

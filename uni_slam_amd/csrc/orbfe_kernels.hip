@@ -1,0 +1,740 @@
+// HIP kernels of the ORB front-end for gfx950 (CDNA4, wave64).
+//
+//   k_detect    pyramid levels + FAST-style score + cull mask, one 64x16 tile
+//               per workgroup, LDS-staged halo; survivors leave as a bitmap
+//               built with wave ballots (one 64-bit word per 64-pixel row
+//               segment).
+//   k_compact   per (frame, level): raster-order prefix sum over the survivor
+//               bitmap, first `cap` survivors to a keypoint list, count.
+//   k_describe  one wave per keypoint: corner strength, intensity-centroid
+//               angle, 256 rotated 9-tap BRIEF tests (4 per lane, bits from
+//               ballots), 48-byte feature record.
+//   k_match_dir pipeline matcher, one wave per query: LOD+-1 pool, 64-ary
+//               wave search of the y window, XOR/popcount over the x window,
+//               best/second-best keys reduced across the wave.
+//   k_bf_partial / k_bf_merge  brute-force best/second-best Hamming search,
+//               queries in registers, train tiles in LDS.
+#include <hip/hip_runtime.h>
+
+#include "brief_pattern.h"
+#include "orbfe_device.h"
+
+namespace orbfe {
+
+namespace {
+
+__constant__ int8_t c_pattern[256][4];
+
+// detect_corners.wgsl:31-32
+constexpr int8_t c_circle_x[16] = {0, 1, 2, 3, 3, 3, 2, 1, 0, -1, -2, -3, -3, -3, -2, -1};
+constexpr int8_t c_circle_y[16] = {-3, -3, -2, -1, 0, 1, 2, 3, 3, 3, 2, 1, 0, -1, -2, -3};
+
+__device__ inline uint32_t find_job(const uint32_t* prefix, uint32_t n, uint32_t b) {
+    // largest j with prefix[j] <= b (prefix[0] = 0, prefix[n] = total)
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (prefix[mid] <= b) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Level-l texel (x, y) computed from the u8 frame: level 0 is float(u8)/255
+// (data/src/provider_base.cpp:121-124); level l >= 1 is the bilinear sample of
+// level 0 at the texel centre (create_pyramid.wgsl:5-14).  x, y in range.
+__device__ inline float level_texel_from_frame(const Job& J, const Plan& P,
+                                               const LevelGeom& G, uint32_t l,
+                                               const float* lut, uint32_t x,
+                                               uint32_t y) {
+    if (l == 0) return lut[J.img[y * J.pitch + x]];
+    const float u = (static_cast<float>(x) + 0.5f) / static_cast<float>(G.w);
+    const float v = (static_cast<float>(y) + 0.5f) / static_cast<float>(G.h);
+    const AxisTaps tx = axis_taps(u, P.W0);
+    const AxisTaps ty = axis_taps(v, P.H0);
+    const uint8_t* r0 = J.img + ty.i0 * J.pitch;
+    const uint8_t* r1 = J.img + ty.i1 * J.pitch;
+    return lerp2(lut[r0[tx.i0]], lut[r0[tx.i1]], lut[r1[tx.i0]], lut[r1[tx.i1]],
+                 tx.frac, ty.frac);
+}
+
+// textureLoad(level l, (gx, gy)) with the robustness rule, from the frame.
+__device__ inline float load_from_frame(const Job& J, const Plan& P, const LevelGeom& G,
+                                        uint32_t l, const float* lut, int32_t gx,
+                                        int32_t gy, uint32_t border_zero) {
+    if (border_zero) {
+        if (gx < 0 || gy < 0 || gx >= static_cast<int32_t>(G.w) ||
+            gy >= static_cast<int32_t>(G.h))
+            return 0.0f;
+        return level_texel_from_frame(J, P, G, l, lut, gx, gy);
+    }
+    return level_texel_from_frame(J, P, G, l, lut, tint_clamp(gx, G.w),
+                                  tint_clamp(gy, G.h));
+}
+
+// detect_corners.wgsl:58-81 evaluated straight from the frame (used for the
+// few row-0 neighbours outside a tile).
+__device__ uint32_t fast_score_from_frame(const Job& J, const Plan& P, const LevelGeom& G,
+                                          uint32_t l, const float* lut, uint32_t x,
+                                          uint32_t y, const Params& prm) {
+    const float center = level_texel_from_frame(J, P, G, l, lut, x, y);
+    uint32_t curr = 1, streak = 1;
+    for (int i = 0; i < 16; ++i) {
+        const float counter =
+            load_from_frame(J, P, G, l, lut, static_cast<int32_t>(x) + c_circle_x[i],
+                            static_cast<int32_t>(y) + c_circle_y[i], prm.border_zero);
+        const bool diff = __builtin_fabsf(center - counter) > prm.threshold;
+        curr = diff ? curr + 1 : 1;
+        streak = curr > streak ? curr : streak;
+    }
+    return streak > prm.min_run ? streak : 0u;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// k_detect
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job0,
+                                                            Params prm) {
+    __shared__ float lut[256];
+    __shared__ float tile[kLdsH][kLdsW];
+    __shared__ uint32_t col_i0[kLdsW], col_i1[kLdsW];
+    __shared__ float col_a[kLdsW];
+    __shared__ uint32_t row_i0[kLdsH], row_i1[kLdsH];
+    __shared__ float row_b[kLdsH];
+    __shared__ uint32_t col_ok[kLdsW], row_ok[kLdsH];
+    __shared__ uint32_t row0_score[kTileW + 4];
+
+    const uint32_t tid = threadIdx.x;
+    uint32_t b = blockIdx.x;
+    uint32_t j = 0;
+    if (bv.n) {
+        j = find_job(bv.tile_prefix, bv.n, b);
+        b -= bv.tile_prefix[j];
+    }
+    const Job J = bv.n ? bv.jobs[j] : job0;
+    const Plan& P = *J.plan;
+    uint32_t l = 0;
+    while (l + 1 < P.L && b >= P.lv[l + 1].tile_base) ++l;
+    const LevelGeom& G = P.lv[l];
+    const uint32_t t = b - G.tile_base;
+    const uint32_t tile_y = t / G.tiles_x;
+    const uint32_t tile_x = t - tile_y * G.tiles_x;
+    const int32_t x0 = static_cast<int32_t>(tile_x) * kTileW;
+    const int32_t y0 = static_cast<int32_t>(tile_y) * kTileH;
+    const uint32_t w = G.w, h = G.h;
+
+    lut[tid] = static_cast<float>(tid) / 255.0f;
+
+    // Per-column / per-row source taps of the staged region.
+    if (tid < kLdsW + kLdsH) {
+        const bool is_col = tid < kLdsW;
+        const uint32_t c = is_col ? tid : tid - kLdsW;
+        const int32_t g = (is_col ? x0 : y0) - kHalo + static_cast<int32_t>(c);
+        const uint32_t n = is_col ? w : h;
+        const bool inside = g >= 0 && g < static_cast<int32_t>(n);
+        const uint32_t cl = prm.border_zero ? (inside ? static_cast<uint32_t>(g) : 0u)
+                                            : tint_clamp(g, n);
+        uint32_t i0 = cl, i1 = cl;
+        float frac = 0.0f;
+        if (l > 0) {
+            const float uv = (static_cast<float>(cl) + 0.5f) / static_cast<float>(n);
+            const AxisTaps tp = axis_taps(uv, is_col ? P.W0 : P.H0);
+            i0 = tp.i0;
+            i1 = tp.i1;
+            frac = tp.frac;
+        }
+        const uint32_t ok = (!prm.border_zero || inside) ? 1u : 0u;
+        if (is_col) {
+            col_i0[c] = i0; col_i1[c] = i1; col_a[c] = frac; col_ok[c] = ok;
+        } else {
+            row_i0[c] = i0; row_i1[c] = i1; row_b[c] = frac; row_ok[c] = ok;
+        }
+    }
+    __syncthreads();
+
+    for (uint32_t e = tid; e < kLdsW * kLdsH; e += kDetectThreads) {
+        const uint32_t r = e / kLdsW;
+        const uint32_t c = e - r * kLdsW;
+        float v = 0.0f;
+        if (col_ok[c] & row_ok[r]) {
+            const uint8_t* s0 = J.img + row_i0[r] * J.pitch;
+            if (l == 0) {
+                v = lut[s0[col_i0[c]]];
+            } else {
+                const uint8_t* s1 = J.img + row_i1[r] * J.pitch;
+                v = lerp2(lut[s0[col_i0[c]]], lut[s0[col_i1[c]]], lut[s1[col_i0[c]]],
+                          lut[s1[col_i1[c]]], col_a[c], row_b[r]);
+            }
+        }
+        tile[r][c] = v;
+    }
+    __syncthreads();
+
+    const uint32_t lane = tid & 63;
+    const uint32_t wave = tid >> 6;
+    const uint32_t x = static_cast<uint32_t>(x0) + lane;
+    uint32_t score[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t r = wave * 4 + k;
+        const uint32_t y = static_cast<uint32_t>(y0) + r;
+        score[k] = 0;
+        if (x < w && y < h) {
+            const float center = tile[r + kHalo][lane + kHalo];
+            if (l > 0 && J.pyr) J.pyr[G.pyr_off + y * w + x] = center;
+            uint32_t curr = 1, streak = 1;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const float counter =
+                    tile[r + kHalo + c_circle_y[i]][lane + kHalo + c_circle_x[i]];
+                const bool diff = __builtin_fabsf(center - counter) > prm.threshold;
+                curr = diff ? curr + 1 : 1;
+                streak = curr > streak ? curr : streak;
+            }
+            score[k] = streak > prm.min_run ? streak : 0u;
+        }
+    }
+
+    // Row 0 of the level keeps only the first maximum of each 3-wide window
+    // starting at a multiple of 9 (cull_corners.wgsl:36-56 with x = 9*gid.x).
+    const bool has_row0 = y0 == 0;
+    if (has_row0) {
+        if (wave == 0) row0_score[2 + lane] = score[0];
+        if (tid >= 64 && tid < 68) {
+            const int32_t k = static_cast<int32_t>(tid) - 64;  // 0..3
+            const int32_t xe = k < 2 ? x0 - 2 + k : x0 + kTileW + (k - 2);
+            uint32_t s = 0;
+            if (xe >= 0 && xe < static_cast<int32_t>(w))
+                s = fast_score_from_frame(J, P, G, l, lut, static_cast<uint32_t>(xe), 0, prm);
+            row0_score[k < 2 ? k : kTileW + k] = s;
+        }
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t r = wave * 4 + k;
+        const uint32_t y = static_cast<uint32_t>(y0) + r;
+        if (y >= h) continue;  // wave-uniform
+        const uint64_t mword = P.mask[G.word_off + y * G.wpr + tile_x];
+        bool keep = x < w && ((mword >> lane) & 1ull);
+        if (keep) {
+            if (y == 0) {
+                const uint32_t s = x - x % 9;
+                const uint32_t e = s + 3 < w ? s + 3 : w;
+                uint32_t m = s;
+                uint32_t best = row0_score[s - x0 + 2];
+                for (uint32_t q = s + 1; q < e; ++q) {
+                    const uint32_t v = row0_score[q - x0 + 2];
+                    if (v > best) { best = v; m = q; }
+                }
+                keep = (m == x) && best > 0;
+            } else {
+                keep = score[k] > 0;
+            }
+        }
+        const uint64_t bits = __ballot(keep);
+        if (lane == 0) J.surv[G.word_off + y * G.wpr + tile_x] = bits;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_compact: one workgroup per (frame, level).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kCompactThreads = 256;
+
+__global__ __launch_bounds__(kCompactThreads) void k_compact(BatchView bv, Job job0,
+                                                              uint32_t levels) {
+    __shared__ uint32_t scan[kCompactThreads];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t j = blockIdx.x / levels;
+    const uint32_t l = blockIdx.x - j * levels;
+    const Job J = bv.n ? bv.jobs[j] : job0;
+    const Plan& P = *J.plan;
+    const LevelGeom& G = P.lv[l];
+    const uint32_t words = G.h * G.wpr;
+    const uint64_t* surv = J.surv + G.word_off;
+    const uint32_t chunk = (words + kCompactThreads - 1) / kCompactThreads;
+    const uint32_t begin = tid * chunk < words ? tid * chunk : words;
+    const uint32_t end = begin + chunk < words ? begin + chunk : words;
+
+    uint32_t cnt = 0;
+    for (uint32_t k = begin; k < end; ++k) cnt += __popcll(surv[k]);
+    scan[tid] = cnt;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over 256 entries.
+    for (uint32_t off = 1; off < kCompactThreads; off <<= 1) {
+        const uint32_t v = tid >= off ? scan[tid - off] : 0u;
+        __syncthreads();
+        scan[tid] += v;
+        __syncthreads();
+    }
+    const uint32_t total = scan[kCompactThreads - 1];
+    uint32_t rank = scan[tid] - cnt;
+    uint32_t* kp = J.kp + G.kp_off;
+    for (uint32_t k = begin; k < end && rank < P.cap; ++k) {
+        uint64_t v = surv[k];
+        const uint32_t y = k / G.wpr;
+        const uint32_t xb = (k - y * G.wpr) * 64;
+        while (v && rank < P.cap) {
+            const uint32_t bit = __ffsll(static_cast<unsigned long long>(v)) - 1;
+            v &= v - 1;
+            kp[rank++] = (xb + bit) | (y << 16);
+        }
+    }
+    if (tid == 0) {
+        uint32_t* hdr = reinterpret_cast<uint32_t*>(J.feat + G.feat_off);
+        hdr[0] = total;
+        hdr[1] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_describe: one wave per keypoint slot.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct LevelSrc {
+    const uint8_t* img;  // level 0
+    uint64_t pitch;
+    const float* lvl;    // levels >= 1
+    const float* lut;
+    uint32_t w, h;
+    bool base;
+    __device__ float at(uint32_t x, uint32_t y) const {
+        return base ? lut[img[y * pitch + x]] : lvl[y * w + x];
+    }
+    __device__ float load(int32_t x, int32_t y, uint32_t border_zero) const {
+        if (border_zero) {
+            if (x < 0 || y < 0 || x >= static_cast<int32_t>(w) || y >= static_cast<int32_t>(h))
+                return 0.0f;
+            return at(x, y);
+        }
+        return at(tint_clamp(x, w), tint_clamp(y, h));
+    }
+};
+
+// generate_features.wgsl:149-172
+__device__ inline float gaussian_tap9(const LevelSrc& S, uint32_t px, uint32_t py) {
+    const float offs[3] = {-1.2f, 0.0f, 1.2f};
+    const float wts[3] = {0.3125f, 0.375f, 0.3125f};
+    const float texel_x = 1.0f / static_cast<float>(S.w);
+    const float texel_y = 1.0f / static_cast<float>(S.h);
+    const float uvx = (static_cast<float>(px) + 0.5f) * texel_x;
+    const float uvy = (static_cast<float>(py) + 0.5f) * texel_y;
+    AxisTaps ax[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ax[i] = axis_taps(uvx + offs[i] * texel_x, S.w);
+    float sum = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const AxisTaps ay = axis_taps(uvy + offs[j] * texel_y, S.h);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const float s = lerp2(S.at(ax[i].i0, ay.i0), S.at(ax[i].i1, ay.i0),
+                                  S.at(ax[i].i0, ay.i1), S.at(ax[i].i1, ay.i1),
+                                  ax[i].frac, ay.frac);
+            sum = sum + s * wts[i] * wts[j];
+        }
+    }
+    return sum;
+}
+
+}  // namespace
+
+constexpr uint32_t kDescribeThreads = 256;  // 4 waves = 4 keypoints
+
+__global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job job0,
+                                                                Params prm) {
+    __shared__ float lut[256];
+    const uint32_t tid = threadIdx.x;
+    lut[tid] = static_cast<float>(tid) / 255.0f;
+    __syncthreads();
+
+    uint32_t b = blockIdx.x;
+    uint32_t j = 0;
+    if (bv.n) {
+        j = find_job(bv.desc_prefix, bv.n, b);
+        b -= bv.desc_prefix[j];
+    }
+    const Job J = bv.n ? bv.jobs[j] : job0;
+    const Plan& P = *J.plan;
+    const uint32_t l = b / P.desc_groups;
+    const uint32_t g = b - l * P.desc_groups;
+    const uint32_t lane = tid & 63;
+    const uint32_t slot = g * 4 + (tid >> 6);
+    if (slot >= P.cap) return;
+    const LevelGeom& G = P.lv[l];
+    uint8_t* block = J.feat + G.feat_off;
+    const uint32_t count = reinterpret_cast<const uint32_t*>(block)[0];
+    uint32_t* out = reinterpret_cast<uint32_t*>(block + 8 + 48ull * slot);
+    if (slot >= (count < P.cap ? count : P.cap)) {
+        if (lane < 12) out[lane] = 0u;
+        return;
+    }
+    const uint32_t kpv = J.kp[G.kp_off + slot];
+    const uint32_t x = kpv & 0xFFFFu;
+    const uint32_t y = kpv >> 16;
+
+    LevelSrc S;
+    S.img = J.img;
+    S.pitch = J.pitch;
+    S.lvl = J.pyr + G.pyr_off;
+    S.lut = lut;
+    S.w = G.w;
+    S.h = G.h;
+    S.base = l == 0;
+
+    // Corner strength (detect_corners.wgsl:58-81): lanes 0..15 hold the ring.
+    const float center = S.at(x, y);
+    bool diff = false;
+    if (lane < 16) {
+        const float counter = S.load(static_cast<int32_t>(x) + c_circle_x[lane],
+                                     static_cast<int32_t>(y) + c_circle_y[lane],
+                                     prm.border_zero);
+        diff = __builtin_fabsf(center - counter) > prm.threshold;
+    }
+    const uint64_t ring = __ballot(diff);
+    uint32_t curr = 1, streak = 1;
+    for (int i = 0; i < 16; ++i) {
+        curr = ((ring >> i) & 1ull) ? curr + 1 : 1;
+        streak = curr > streak ? curr : streak;
+    }
+    const uint32_t strength = streak > prm.min_run ? streak : 0u;
+
+    // Orientation (generate_features.wgsl:80-99): sequential moments.
+    const int32_t r = prm.orient_r;
+    float m10 = 0.0f, m01 = 0.0f;
+    for (int32_t dy = -r; dy <= r; ++dy) {
+        for (int32_t dx = -r; dx <= r; ++dx) {
+            if (dx * dx + dy * dy > r * r) continue;
+            const float pixel = S.load(static_cast<int32_t>(x) + dx,
+                                       static_cast<int32_t>(y) + dy, prm.border_zero);
+            m10 += static_cast<float>(dx) * pixel;
+            m01 += static_cast<float>(dy) * pixel;
+        }
+    }
+    const float theta = atan2_own(m01, m10);
+    float sn, cs;
+    sincos_own(theta, &sn, &cs);
+
+    // Descriptor (generate_features.wgsl:101-144): test i = lane + 64k.
+    uint32_t desc[8];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t i = lane + 64 * k;
+        const float pax = c_pattern[i][0], pay = c_pattern[i][1];
+        const float pbx = c_pattern[i][2], pby = c_pattern[i][3];
+        const int32_t ax = static_cast<int32_t>(__builtin_rintf(pax * cs - pay * sn));
+        const int32_t ay = static_cast<int32_t>(__builtin_rintf(pax * sn + pay * cs));
+        const int32_t bx = static_cast<int32_t>(__builtin_rintf(pbx * cs - pby * sn));
+        const int32_t by = static_cast<int32_t>(__builtin_rintf(pbx * sn + pby * cs));
+        const float sa = gaussian_tap9(S, static_cast<uint32_t>(static_cast<int32_t>(x) + ax),
+                                       static_cast<uint32_t>(static_cast<int32_t>(y) + ay));
+        const float sb = gaussian_tap9(S, static_cast<uint32_t>(static_cast<int32_t>(x) + bx),
+                                       static_cast<uint32_t>(static_cast<int32_t>(y) + by));
+        const uint64_t bits = __ballot(sa > sb);
+        desc[2 * k] = static_cast<uint32_t>(bits);
+        desc[2 * k + 1] = static_cast<uint32_t>(bits >> 32);
+    }
+
+    if (lane < 12) {
+        uint32_t v;
+        switch (lane) {
+            case 0: v = __float_as_uint(static_cast<float>(x)); break;
+            case 1: v = __float_as_uint(static_cast<float>(y)); break;
+            case 2: v = strength; break;
+            case 3: v = __float_as_uint(theta); break;
+            default: v = desc[lane - 4]; break;
+        }
+        out[lane] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pipeline matcher (FindMatchesInLod / FindMatches).
+// ---------------------------------------------------------------------------
+namespace {
+
+struct FeatView {
+    const uint8_t* base;
+    uint32_t stride;  // feature array block stride
+    __device__ const uint32_t* rec(uint32_t lod, uint32_t i) const {
+        return reinterpret_cast<const uint32_t*>(base + stride * lod + 8 + 48ull * i);
+    }
+    __device__ uint32_t count(uint32_t lod, uint32_t cap) const {
+        const uint32_t c = *reinterpret_cast<const uint32_t*>(base + stride * lod);
+        return c < cap ? c : cap;
+    }
+};
+
+// First index in [0, n) at which the monotone predicate (false..true) holds,
+// or n; 64 probes per round.
+template <typename Pred>
+__device__ inline uint32_t wave_lower_bound(uint32_t n, uint32_t lane, Pred pred) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t stride = (hi - lo + 63) / 64;
+        const uint32_t pos = lo + lane * stride;
+        const bool p = pos < hi ? pred(pos) : true;
+        const uint64_t m = __ballot(p);
+        if (m == 0) {
+            lo = lo + 63 * stride + 1;
+            continue;
+        }
+        const uint32_t f = __ffsll(static_cast<unsigned long long>(m)) - 1;
+        if (f == 0) {
+            hi = lo;
+            break;
+        }
+        const uint32_t nlo = lo + (f - 1) * stride + 1;
+        const uint32_t cand = lo + f * stride;
+        hi = cand < hi ? cand : hi;
+        lo = nlo;
+    }
+    return lo;
+}
+
+__device__ inline uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
+__device__ inline uint32_t umax(uint32_t a, uint32_t b) { return a > b ? a : b; }
+
+}  // namespace
+
+constexpr uint32_t kMatchThreads = 256;  // 4 waves = 4 queries
+
+// kMutual = false: queries = curr, pool from prev -> b_to_a.
+// kMutual = true:  queries = prev, pool from curr, mutual check -> prev_match.
+template <bool kMutual>
+__global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, MatchJob job0,
+                                                              float ratio) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63;
+    uint32_t b = blockIdx.x;
+    uint32_t j = 0;
+    if (bv.n) {
+        j = find_job(bv.block_prefix, bv.n, b);
+        b -= bv.block_prefix[j];
+    }
+    const MatchJob M = bv.n ? bv.jobs[j] : job0;
+    const Plan& P = *M.plan;
+    const uint32_t l = b / P.desc_groups;
+    const uint32_t g = b - l * P.desc_groups;
+    const uint32_t slot = g * 4 + (tid >> 6);
+    if (slot >= P.cap) return;
+    const FeatView Q{kMutual ? M.prev : M.curr, P.feat_stride};
+    const FeatView C{kMutual ? M.curr : M.prev, P.feat_stride};
+    int32_t* out = kMutual ? M.prev_match : M.b_to_a;
+    const uint32_t nq = Q.count(l, P.cap);
+    if (slot >= nq) {
+        if (lane == 0) out[l * P.cap + slot] = -1;
+        return;
+    }
+
+    // Pool levels in the reference's concatenation order: l, l-1, l+1.
+    uint32_t pl[3], pn[3];
+    uint32_t npool = 0, total = 0;
+    pl[npool] = l; pn[npool] = C.count(l, P.cap); total += pn[npool]; ++npool;
+    if (l > 0) { pl[npool] = l - 1; pn[npool] = C.count(l - 1, P.cap); total += pn[npool]; ++npool; }
+    if (l + 1 < P.L) { pl[npool] = l + 1; pn[npool] = C.count(l + 1, P.cap); total += pn[npool]; ++npool; }
+
+    int32_t result = -1;
+    if (total >= 2) {
+        const uint32_t* qr = Q.rec(l, slot);
+        const float qs = P.lv[l].lod0_scale;
+        const float qx = static_cast<float>(static_cast<uint32_t>(__uint_as_float(qr[0]))) * qs;
+        const float qy = static_cast<float>(static_cast<uint32_t>(__uint_as_float(qr[1]))) * qs;
+        uint32_t qd[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) qd[k] = qr[4 + k];
+        const float lo_x = qx - P.max_dx;
+        const float hi_x = qx + P.max_dx;
+        const float dy = P.max_dy;
+
+        uint32_t min1 = 0xFFFFFFFFu, min2 = 0xFFFFFFFFu;
+        for (uint32_t s = 0; s < npool; ++s) {
+            const uint32_t cl = pl[s];
+            const float cs = P.lv[cl].lod0_scale;
+            auto cy_at = [&](uint32_t i) {
+                return static_cast<float>(static_cast<uint32_t>(__uint_as_float(C.rec(cl, i)[1]))) * cs;
+            };
+            // y window: |qy - cy| <= dy, a contiguous range because records
+            // are in raster order.
+            const uint32_t lo = wave_lower_bound(pn[s], lane, [&](uint32_t i) { return qy - cy_at(i) <= dy; });
+            const uint32_t hi = wave_lower_bound(pn[s], lane, [&](uint32_t i) { return qy - cy_at(i) < -dy; });
+            for (uint32_t i = lo + lane; i < hi; i += 64) {
+                const uint32_t* cr = C.rec(cl, i);
+                const float cx = static_cast<float>(static_cast<uint32_t>(__uint_as_float(cr[0]))) * cs;
+                const float cy = static_cast<float>(static_cast<uint32_t>(__uint_as_float(cr[1]))) * cs;
+                if (cx < lo_x || cx > hi_x || __builtin_fabsf(qy - cy) > dy) continue;
+                uint32_t d = 0;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) d += __popc(qd[k] ^ cr[4 + k]);
+                const uint32_t key = (d << kMatchKeyShift) | (s << kMatchIdxBits) | i;
+                min2 = umax(min1, umin(min2, key));
+                min1 = umin(min1, key);
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint32_t o1 = __shfl_xor(min1, off);
+            const uint32_t o2 = __shfl_xor(min2, off);
+            const uint32_t n1 = umin(min1, o1);
+            const uint32_t n2 = umin(umax(min1, o1), umin(min2, o2));
+            min1 = n1;
+            min2 = n2;
+        }
+        if (min1 != 0xFFFFFFFFu) {
+            const uint32_t d1 = min1 >> kMatchKeyShift;
+            const uint32_t d2 = min2 == 0xFFFFFFFFu ? 0xFFFFFFFFu : (min2 >> kMatchKeyShift);
+            if (static_cast<float>(d1) < ratio * static_cast<float>(d2)) {
+                const uint32_t s = (min1 >> kMatchIdxBits) & 3u;
+                const uint32_t idx = min1 & ((1u << kMatchIdxBits) - 1u);
+                result = static_cast<int32_t>((pl[s] << 24) | idx);
+            }
+        }
+        if (kMutual && result >= 0) {
+            const uint32_t bl = static_cast<uint32_t>(result) >> 24;
+            const uint32_t bi = static_cast<uint32_t>(result) & 0xFFFFFFu;
+            if (M.b_to_a[bl * P.cap + bi] != static_cast<int32_t>((l << 24) | slot)) result = -1;
+        }
+    }
+    if (lane == 0) out[l * P.cap + slot] = result;
+}
+
+template __global__ void k_match_dir<false>(MatchBatchView, MatchJob, float);
+template __global__ void k_match_dir<true>(MatchBatchView, MatchJob, float);
+
+// ---------------------------------------------------------------------------
+// Brute-force matcher.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kBfThreads = 256;
+constexpr uint32_t kBfQPT = 2;          // queries per thread
+constexpr uint32_t kBfTile = 256;       // train descriptors per LDS tile
+
+__global__ __launch_bounds__(kBfThreads) void k_bf_partial(const uint32_t* __restrict__ query,
+                                                            uint32_t nq,
+                                                            const uint32_t* __restrict__ train,
+                                                            uint32_t nt, uint32_t split_len,
+                                                            uint2* __restrict__ part) {
+    __shared__ uint4 tr[kBfTile][2];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t split = blockIdx.y;
+    const uint32_t t_begin = split * split_len;
+    const uint32_t t_end = umin(t_begin + split_len, nt);
+    uint32_t q[kBfQPT][8];
+    uint32_t min1[kBfQPT], min2[kBfQPT];
+#pragma unroll
+    for (uint32_t k = 0; k < kBfQPT; ++k) {
+        const uint32_t qi = (blockIdx.x * kBfQPT + k) * kBfThreads + tid;
+        const uint4* src = reinterpret_cast<const uint4*>(query + 8ull * (qi < nq ? qi : 0));
+        const uint4 a = src[0], bq = src[1];
+        q[k][0] = a.x; q[k][1] = a.y; q[k][2] = a.z; q[k][3] = a.w;
+        q[k][4] = bq.x; q[k][5] = bq.y; q[k][6] = bq.z; q[k][7] = bq.w;
+        min1[k] = 0xFFFFFFFFu;
+        min2[k] = 0xFFFFFFFFu;
+    }
+    for (uint32_t base = t_begin; base < t_end; base += kBfTile) {
+        const uint32_t n = umin(kBfTile, t_end - base);
+        __syncthreads();
+        for (uint32_t e = tid; e < 2 * n; e += kBfThreads) {
+            const uint4* src = reinterpret_cast<const uint4*>(train + 8ull * (base + (e >> 1)));
+            tr[e >> 1][e & 1] = src[e & 1];
+        }
+        __syncthreads();
+        for (uint32_t t = 0; t < n; ++t) {
+            const uint4 a = tr[t][0], c = tr[t][1];
+            const uint32_t jt = base + t;
+#pragma unroll
+            for (uint32_t k = 0; k < kBfQPT; ++k) {
+                uint32_t d = __popc(q[k][0] ^ a.x);
+                d += __popc(q[k][1] ^ a.y);
+                d += __popc(q[k][2] ^ a.z);
+                d += __popc(q[k][3] ^ a.w);
+                d += __popc(q[k][4] ^ c.x);
+                d += __popc(q[k][5] ^ c.y);
+                d += __popc(q[k][6] ^ c.z);
+                d += __popc(q[k][7] ^ c.w);
+                const uint32_t key = (d << kBfKeyShift) | jt;
+                min2[k] = umax(min1[k], umin(min2[k], key));
+                min1[k] = umin(min1[k], key);
+            }
+        }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kBfQPT; ++k) {
+        const uint32_t qi = (blockIdx.x * kBfQPT + k) * kBfThreads + tid;
+        if (qi < nq) part[static_cast<size_t>(split) * nq + qi] = make_uint2(min1[k], min2[k]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bf_merge(const uint2* __restrict__ part, uint32_t nq,
+                                                   uint32_t splits, uint32_t* __restrict__ idx,
+                                                   uint32_t* __restrict__ d1,
+                                                   uint32_t* __restrict__ d2) {
+    const uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (qi >= nq) return;
+    uint32_t m1 = 0xFFFFFFFFu, m2 = 0xFFFFFFFFu;
+    for (uint32_t s = 0; s < splits; ++s) {
+        const uint2 p = part[static_cast<size_t>(s) * nq + qi];
+        const uint32_t n1 = umin(m1, p.x);
+        const uint32_t n2 = umin(umax(m1, p.x), umin(m2, p.y));
+        m1 = n1;
+        m2 = n2;
+    }
+    const uint32_t mask = (1u << kBfKeyShift) - 1u;
+    idx[qi] = m1 == 0xFFFFFFFFu ? 0xFFFFFFFFu : (m1 & mask);
+    d1[qi] = m1 == 0xFFFFFFFFu ? 0xFFFFFFFFu : (m1 >> kBfKeyShift);
+    d2[qi] = m2 == 0xFFFFFFFFu ? 0xFFFFFFFFu : (m2 >> kBfKeyShift);
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers (called from orbfe_api.cpp).
+// ---------------------------------------------------------------------------
+hipError_t upload_pattern() {
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_pattern), kOrbfeBriefPattern, sizeof(kOrbfeBriefPattern));
+}
+
+hipError_t launch_detect(uint32_t blocks, BatchView bv, const Job& job0, const Params& prm,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(k_detect, dim3(blocks), dim3(kDetectThreads), 0, s, bv, job0, prm);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact(uint32_t blocks, BatchView bv, const Job& job0, uint32_t levels,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_compact, dim3(blocks), dim3(kCompactThreads), 0, s, bv, job0, levels);
+    return hipGetLastError();
+}
+
+hipError_t launch_describe(uint32_t blocks, BatchView bv, const Job& job0, const Params& prm,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_describe, dim3(blocks), dim3(kDescribeThreads), 0, s, bv, job0, prm);
+    return hipGetLastError();
+}
+
+hipError_t launch_match_dir(bool mutual, uint32_t blocks, MatchBatchView bv, const MatchJob& job0,
+                            float ratio, hipStream_t s) {
+    if (mutual)
+        hipLaunchKernelGGL(k_match_dir<true>, dim3(blocks), dim3(kMatchThreads), 0, s, bv, job0, ratio);
+    else
+        hipLaunchKernelGGL(k_match_dir<false>, dim3(blocks), dim3(kMatchThreads), 0, s, bv, job0, ratio);
+    return hipGetLastError();
+}
+
+uint32_t bf_queries_per_block() { return kBfQPT * kBfThreads; }
+
+hipError_t launch_bf(const uint32_t* q, uint32_t nq, const uint32_t* t, uint32_t nt,
+                     uint32_t splits, uint32_t split_len, uint2* part, uint32_t* idx,
+                     uint32_t* d1, uint32_t* d2, hipStream_t s, int phase) {
+    if (phase == 0) {
+        const uint32_t qb = (nq + kBfQPT * kBfThreads - 1) / (kBfQPT * kBfThreads);
+        hipLaunchKernelGGL(k_bf_partial, dim3(qb, splits), dim3(kBfThreads), 0, s, q, nq, t, nt,
+                           split_len, part);
+    } else {
+        hipLaunchKernelGGL(k_bf_merge, dim3((nq + 255) / 256), dim3(256), 0, s, part, nq, splits,
+                           idx, d1, d2);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace orbfe
