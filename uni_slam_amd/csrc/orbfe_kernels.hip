@@ -21,9 +21,10 @@
 
 namespace orbfe {
 
-namespace {
-
+// BRIEF test pattern (brief_pattern.h), uploaded once per device.
 __constant__ int8_t c_pattern[256][4];
+
+namespace {
 
 // detect_corners.wgsl:31-32
 constexpr int8_t c_circle_x[16] = {0, 1, 2, 3, 3, 3, 2, 1, 0, -1, -2, -3, -3, -3, -2, -1};
