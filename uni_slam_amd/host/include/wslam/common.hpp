@@ -1,0 +1,34 @@
+#pragma once
+
+// Store keys of the feature path (wslam/include/wslam/common.hpp:114-159 of
+// the reference) and the frame type handed to the feature-detect stage.
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace wslam {
+
+namespace ResourceIdentifier {
+inline std::string GetFrameName(uint32_t frame_idx, uint32_t lod = 0) {
+    return "res:frame:" + std::to_string(frame_idx) + ":lod:" + std::to_string(lod);
+}
+inline std::string GetFeatureSetName(size_t keyframes_ago) {
+    return "gen:feat:-" + std::to_string(keyframes_ago);
+}
+inline const std::string MatchedFeaturesName = "gen:feat:match";
+// Consume-once marker asking the loader to advance the reference feature set.
+inline const std::string FeatureReferenceAdvanceName = "gen:feat:refadv";
+}  // namespace ResourceIdentifier
+
+// A grayscale frame kept as u8 end to end (the reference converts to f32 on
+// the host, data/src/provider_base.cpp:121-124; here the conversion happens
+// in the detect kernel).
+struct FrameU8 {
+    uint32_t width = 0;
+    uint32_t height = 0;
+    uint64_t timestamp = 0;
+    std::vector<uint8_t> pixels;  // row-major, pitch = width
+};
+
+}  // namespace wslam
