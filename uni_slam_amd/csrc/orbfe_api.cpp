@@ -114,6 +114,7 @@ struct PlanEntry {
     uint64_t* mask = nullptr;
     // per-frame scratch sizes (bytes), 256-aligned
     size_t surv_bytes = 0, kp_bytes = 0, pyr_bytes = 0, feat_bytes = 0, match_bytes = 0;
+    size_t sm_bytes = 0;
 };
 
 struct Event2 {
@@ -143,6 +144,7 @@ struct orbfe_ctx {
     uint64_t* surv = nullptr;
     uint32_t* kp = nullptr;
     float* pyr_scratch = nullptr;
+    float* smooth = nullptr;
     int32_t* b_to_a = nullptr;
     // host-memory entry points: device staging
     uint8_t* d_img = nullptr;
@@ -251,7 +253,7 @@ int get_plan(orbfe_ctx* ctx, uint32_t W0, uint32_t H0, uint32_t cap, PlanEntry**
     P.desc_groups = ceil_div(cap, 4);
     P.max_dx = 0.1f * static_cast<float>(W0);
     P.max_dy = 0.1f * static_cast<float>(H0);
-    uint32_t tiles = 0, words = 0, pyr = 0;
+    uint32_t tiles = 0, words = 0, pyr = 0, sm = 0;
     float scale = 1.0f;
     for (uint32_t l = 0; l < L; ++l) {
         if (dw[l] == 0 || dh[l] == 0)
@@ -260,8 +262,8 @@ int get_plan(orbfe_ctx* ctx, uint32_t W0, uint32_t H0, uint32_t cap, PlanEntry**
         G.w = dw[l];
         G.h = dh[l];
         G.wpr = ceil_div(G.w, 64);
-        G.tiles_x = ceil_div(G.w, orbfe::kTileW);
-        G.tiles_y = ceil_div(G.h, orbfe::kTileH);
+        G.tiles_x = ceil_div(G.w + orbfe::kSmoothExt, orbfe::kTileW);
+        G.tiles_y = ceil_div(G.h + orbfe::kSmoothExt, orbfe::kTileH);
         G.tile_base = tiles;
         tiles += G.tiles_x * G.tiles_y;
         G.pyr_off = l == 0 ? 0 : pyr;
@@ -270,12 +272,15 @@ int get_plan(orbfe_ctx* ctx, uint32_t W0, uint32_t H0, uint32_t cap, PlanEntry**
         words += G.h * G.wpr;
         G.feat_off = l * P.feat_stride;
         G.kp_off = l * cap;
+        G.sm_off = sm;
+        sm += (G.w + orbfe::kSmoothExt) * (G.h + orbfe::kSmoothExt);
         G.lod0_scale = scale;  // toLod0 (match_features_cpu.cpp:58-64)
         scale *= static_cast<float>(1.2);
     }
     P.tiles = tiles;
     P.words = words;
     P.pyr_floats = pyr;
+    P.sm_floats = sm;
     std::vector<uint64_t> mask(words ? words : 1, 0ull);
     for (uint32_t l = 0; l < L; ++l)
         build_level_mask(W0, H0, P.lv[l].w, P.lv[l].h, P.lv[l].wpr,
@@ -292,6 +297,7 @@ int get_plan(orbfe_ctx* ctx, uint32_t W0, uint32_t H0, uint32_t cap, PlanEntry**
     pe->pyr_bytes = align_up(static_cast<size_t>(pyr) * 4);
     pe->feat_bytes = align_up(static_cast<size_t>(L) * P.feat_stride);
     pe->match_bytes = align_up(static_cast<size_t>(L) * cap * 4);
+    pe->sm_bytes = align_up(static_cast<size_t>(sm) * 4);
     *out = pe.get();
     ctx->plans.emplace(key, std::move(pe));
     return ORBFE_OK;
@@ -379,6 +385,8 @@ int layout_detect(orbfe_ctx* ctx, const orbfe_frame* frames, uint32_t n, DetectL
         off += pe->surv_bytes;
         J.kp = reinterpret_cast<uint32_t*>(arena + off);
         off += pe->kp_bytes;
+        J.smooth = reinterpret_cast<float*>(arena + off);
+        off += pe->sm_bytes;
         if (f.pyramid) {
             J.pyr = f.pyramid;
         } else {
@@ -539,6 +547,7 @@ void orbfe_destroy(orbfe_ctx* ctx) {
     dfree(ctx->surv);
     dfree(ctx->kp);
     dfree(ctx->pyr_scratch);
+    dfree(ctx->smooth);
     dfree(ctx->b_to_a);
     dfree(ctx->d_img);
     dfree(ctx->d_pyr);
@@ -591,7 +600,8 @@ int orbfe_create(const orbfe_config* cfg, int device, orbfe_ctx** out) {
         return ORBFE_EINVAL;
     c->main = pe;
     if (dalloc(c, &c->surv, pe->surv_bytes) || dalloc(c, &c->kp, pe->kp_bytes) ||
-        dalloc(c, &c->pyr_scratch, pe->pyr_bytes) || dalloc(c, &c->b_to_a, pe->match_bytes) ||
+        dalloc(c, &c->pyr_scratch, pe->pyr_bytes) || dalloc(c, &c->smooth, pe->sm_bytes) ||
+        dalloc(c, &c->b_to_a, pe->match_bytes) ||
         dalloc(c, &c->d_img, align_up(static_cast<size_t>(cfg->frame_width) * cfg->frame_height)) ||
         dalloc(c, &c->d_pyr, pe->pyr_bytes) || dalloc(c, &c->d_feat[0], pe->feat_bytes) ||
         dalloc(c, &c->d_feat[1], pe->feat_bytes) || dalloc(c, &c->d_match, pe->match_bytes)) {
@@ -635,6 +645,7 @@ int orbfe_detect_async(orbfe_ctx* ctx, const uint8_t* image, size_t pitch, float
     job0.feat = static_cast<uint8_t*>(features);
     job0.surv = ctx->surv;
     job0.kp = ctx->kp;
+    job0.smooth = ctx->smooth;
     const BatchView bv{nullptr, nullptr, nullptr, 0};
     return launch_detect_stage(ctx, bv, job0, P.tiles, P.L, P.L * P.desc_groups, stream);
 }

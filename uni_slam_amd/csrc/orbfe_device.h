@@ -37,17 +37,24 @@ constexpr uint32_t kMatchIdxBits = 20;
 // Brute-force matcher keys: distance above a 23-bit train index.
 constexpr uint32_t kBfKeyShift = 23;
 
+// The smoothed image of a level holds G(P), the 9-tap Gaussian of
+// generate_features.wgsl:149-172, for every integer point P of an extended
+// grid [0, w+kSmoothExt) x [0, h+kSmoothExt).  For P >= w+3 (and for the
+// huge values negative coordinates wrap to) every sampler tap clamps to the
+// last column, so G(P) = G(w+3); the same holds for rows.
+constexpr uint32_t kSmoothExt = 4;
+
 struct LevelGeom {
     uint32_t w, h;
     uint32_t wpr;        // 64-pixel words per row, ceil(w / 64)
-    uint32_t tiles_x, tiles_y;
+    uint32_t tiles_x, tiles_y;  // detect tiles over the extended smooth grid
     uint32_t tile_base;  // first detect tile of this level within the frame
     uint32_t pyr_off;    // float offset in the pyramid buffer (levels >= 1)
     uint32_t word_off;   // 64-bit word offset in the cull mask / survivor map
     uint32_t feat_off;   // byte offset of the level block in the feature array
     uint32_t kp_off;     // entry offset of the level's keypoint list
+    uint32_t sm_off;     // float offset of the level's smoothed image
     float lod0_scale;    // product of l factors 1.2f (toLod0)
-    uint32_t pad;
 };
 
 struct Plan {
@@ -58,7 +65,7 @@ struct Plan {
     uint32_t feat_stride; // 8 + 48*cap
     uint32_t desc_groups; // describe blocks per level = ceil(cap / 4)
     float max_dx, max_dy; // 0.1f*W0, 0.1f*H0 (match_features_cpu.cpp:76-77)
-    uint32_t pad;
+    uint32_t sm_floats;   // smoothed-image floats of one frame
     const uint64_t* mask; // static cull mask, `words` words
     LevelGeom lv[kMaxLevels];
 };
@@ -71,6 +78,7 @@ struct Job {
     uint8_t* feat;     // feature array
     uint64_t* surv;    // survivor bitmap scratch, plan->words
     uint32_t* kp;      // keypoint list scratch, L*cap (x | y << 16)
+    float* smooth;     // smoothed-image scratch, plan->sm_floats
 };
 
 struct MatchJob {

@@ -30,6 +30,9 @@ namespace {
 constexpr int8_t c_circle_x[16] = {0, 1, 2, 3, 3, 3, 2, 1, 0, -1, -2, -3, -3, -3, -2, -1};
 constexpr int8_t c_circle_y[16] = {-3, -3, -2, -1, 0, 1, 2, 3, 3, 3, 2, 1, 0, -1, -2, -3};
 
+__device__ inline uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
+__device__ inline uint32_t umax(uint32_t a, uint32_t b) { return a > b ? a : b; }
+
 __device__ inline uint32_t find_job(const uint32_t* prefix, uint32_t n, uint32_t b) {
     // largest j with prefix[j] <= b (prefix[0] = 0, prefix[n] = total)
     uint32_t lo = 0, hi = n;
@@ -175,6 +178,50 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     const uint32_t lane = tid & 63;
     const uint32_t wave = tid >> 6;
     const uint32_t x = static_cast<uint32_t>(x0) + lane;
+
+    // Smoothed image over the extended grid (generate_features.wgsl:149-172
+    // evaluated at every integer point).  Sampler taps are clamped to the
+    // level, so they read true texels: from the staged tile when they fall
+    // inside it, else (only near the extension) from the frame.
+    {
+        const float offs[3] = {-1.2f, 0.0f, 1.2f};
+        const float wts[3] = {0.3125f, 0.375f, 0.3125f};
+        const float texel_x = 1.0f / static_cast<float>(w);
+        const float texel_y = 1.0f / static_cast<float>(h);
+        const uint32_t sw = w + kSmoothExt, sh = h + kSmoothExt;
+        auto texel = [&](uint32_t cx, uint32_t cy) -> float {
+            const int32_t lx = static_cast<int32_t>(cx) - x0 + kHalo;
+            const int32_t ly = static_cast<int32_t>(cy) - y0 + kHalo;
+            if (lx >= 0 && lx < kLdsW && ly >= 0 && ly < kLdsH) return tile[ly][lx];
+            return level_texel_from_frame(J, P, G, l, lut, cx, cy);
+        };
+        if (x < sw) {
+            const float uvx = (static_cast<float>(x) + 0.5f) * texel_x;
+            AxisTaps ax[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) ax[i] = axis_taps(uvx + offs[i] * texel_x, w);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t y = static_cast<uint32_t>(y0) + wave * 4 + k;
+                if (y >= sh) break;
+                const float uvy = (static_cast<float>(y) + 0.5f) * texel_y;
+                float sum = 0.0f;
+#pragma unroll
+                for (int jj = 0; jj < 3; ++jj) {
+                    const AxisTaps ay = axis_taps(uvy + offs[jj] * texel_y, h);
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) {
+                        const float sv = lerp2(texel(ax[i].i0, ay.i0), texel(ax[i].i1, ay.i0),
+                                               texel(ax[i].i0, ay.i1), texel(ax[i].i1, ay.i1),
+                                               ax[i].frac, ay.frac);
+                        sum = sum + sv * wts[i] * wts[jj];
+                    }
+                }
+                J.smooth[G.sm_off + y * sw + x] = sum;
+            }
+        }
+    }
+
     uint32_t score[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -199,7 +246,7 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
 
     // Row 0 of the level keeps only the first maximum of each 3-wide window
     // starting at a multiple of 9 (cull_corners.wgsl:36-56 with x = 9*gid.x).
-    const bool has_row0 = y0 == 0;
+    const bool has_row0 = y0 == 0 && tile_x < G.wpr;
     if (has_row0) {
         if (wave == 0) row0_score[2 + lane] = score[0];
         if (tid >= 64 && tid < 68) {
@@ -213,6 +260,7 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     }
     __syncthreads();
 
+    if (tile_x >= G.wpr) return;  // extension-only tile: no corners
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const uint32_t r = wave * 4 + k;
@@ -241,14 +289,30 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
 }
 
 // ---------------------------------------------------------------------------
-// k_compact: one workgroup per (frame, level).
+// k_compact: one workgroup of 16 waves per (frame, level).  Each wave owns a
+// contiguous run of survivor words, read 64 at a time (coalesced): pass 1
+// counts, a block scan of the 16 wave totals gives each wave its base rank,
+// pass 2 re-reads its words with a wave prefix sum and writes keypoints
+// until the cap is reached.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kCompactThreads = 256;
+constexpr uint32_t kCompactThreads = 1024;
+constexpr uint32_t kCompactWaves = kCompactThreads / 64;
+
+__device__ inline uint32_t wave_inclusive_sum(uint32_t v, uint32_t lane) {
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t o = __shfl_up(v, off);
+        if (lane >= static_cast<uint32_t>(off)) v += o;
+    }
+    return v;
+}
 
 __global__ __launch_bounds__(kCompactThreads) void k_compact(BatchView bv, Job job0,
                                                               uint32_t levels) {
-    __shared__ uint32_t scan[kCompactThreads];
+    __shared__ uint32_t wave_total[kCompactWaves];
     const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63;
+    const uint32_t wave = tid >> 6;
     const uint32_t j = blockIdx.x / levels;
     const uint32_t l = blockIdx.x - j * levels;
     const Job J = bv.n ? bv.jobs[j] : job0;
@@ -256,33 +320,39 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(BatchView bv, Job j
     const LevelGeom& G = P.lv[l];
     const uint32_t words = G.h * G.wpr;
     const uint64_t* surv = J.surv + G.word_off;
-    const uint32_t chunk = (words + kCompactThreads - 1) / kCompactThreads;
-    const uint32_t begin = tid * chunk < words ? tid * chunk : words;
-    const uint32_t end = begin + chunk < words ? begin + chunk : words;
+    const uint32_t seg = (words + kCompactWaves - 1) / kCompactWaves;
+    const uint32_t begin = wave * seg < words ? wave * seg : words;
+    const uint32_t end = begin + seg < words ? begin + seg : words;
 
     uint32_t cnt = 0;
-    for (uint32_t k = begin; k < end; ++k) cnt += __popcll(surv[k]);
-    scan[tid] = cnt;
+    for (uint32_t k = begin + lane; k < end; k += 64) cnt += __popcll(surv[k]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
+    if (lane == 0) wave_total[wave] = cnt;
     __syncthreads();
-    // Hillis-Steele inclusive scan over 256 entries.
-    for (uint32_t off = 1; off < kCompactThreads; off <<= 1) {
-        const uint32_t v = tid >= off ? scan[tid - off] : 0u;
-        __syncthreads();
-        scan[tid] += v;
-        __syncthreads();
+    uint32_t base = 0, total = 0;
+    for (uint32_t v = 0; v < kCompactWaves; ++v) {
+        const uint32_t tv = wave_total[v];
+        base += v < wave ? tv : 0u;
+        total += tv;
     }
-    const uint32_t total = scan[kCompactThreads - 1];
-    uint32_t rank = scan[tid] - cnt;
     uint32_t* kp = J.kp + G.kp_off;
-    for (uint32_t k = begin; k < end && rank < P.cap; ++k) {
-        uint64_t v = surv[k];
-        const uint32_t y = k / G.wpr;
-        const uint32_t xb = (k - y * G.wpr) * 64;
-        while (v && rank < P.cap) {
-            const uint32_t bit = __ffsll(static_cast<unsigned long long>(v)) - 1;
-            v &= v - 1;
-            kp[rank++] = (xb + bit) | (y << 16);
+    for (uint32_t k0 = begin; k0 < end && base < P.cap; k0 += 64) {
+        const uint32_t k = k0 + lane;
+        uint64_t v = k < end ? surv[k] : 0ull;
+        const uint32_t c = __popcll(v);
+        const uint32_t incl = wave_inclusive_sum(c, lane);
+        uint32_t rank = base + incl - c;
+        if (v && rank < P.cap) {
+            const uint32_t y = k / G.wpr;
+            const uint32_t xb = (k - y * G.wpr) * 64;
+            while (v && rank < P.cap) {
+                const uint32_t bit = __ffsll(static_cast<unsigned long long>(v)) - 1;
+                v &= v - 1;
+                kp[rank++] = (xb + bit) | (y << 16);
+            }
         }
+        base += __shfl(incl, 63);
     }
     if (tid == 0) {
         uint32_t* hdr = reinterpret_cast<uint32_t*>(J.feat + G.feat_off);
@@ -315,32 +385,6 @@ struct LevelSrc {
         return at(tint_clamp(x, w), tint_clamp(y, h));
     }
 };
-
-// generate_features.wgsl:149-172
-__device__ inline float gaussian_tap9(const LevelSrc& S, uint32_t px, uint32_t py) {
-    const float offs[3] = {-1.2f, 0.0f, 1.2f};
-    const float wts[3] = {0.3125f, 0.375f, 0.3125f};
-    const float texel_x = 1.0f / static_cast<float>(S.w);
-    const float texel_y = 1.0f / static_cast<float>(S.h);
-    const float uvx = (static_cast<float>(px) + 0.5f) * texel_x;
-    const float uvy = (static_cast<float>(py) + 0.5f) * texel_y;
-    AxisTaps ax[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) ax[i] = axis_taps(uvx + offs[i] * texel_x, S.w);
-    float sum = 0.0f;
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-        const AxisTaps ay = axis_taps(uvy + offs[j] * texel_y, S.h);
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            const float s = lerp2(S.at(ax[i].i0, ay.i0), S.at(ax[i].i1, ay.i0),
-                                  S.at(ax[i].i0, ay.i1), S.at(ax[i].i1, ay.i1),
-                                  ax[i].frac, ay.frac);
-            sum = sum + s * wts[i] * wts[j];
-        }
-    }
-    return sum;
-}
 
 }  // namespace
 
@@ -421,6 +465,8 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
     sincos_own(theta, &sn, &cs);
 
     // Descriptor (generate_features.wgsl:101-144): test i = lane + 64k.
+    const uint32_t sw = G.w + kSmoothExt, sh = G.h + kSmoothExt;
+    const float* sm = J.smooth + G.sm_off;
     uint32_t desc[8];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -431,10 +477,15 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
         const int32_t ay = static_cast<int32_t>(__builtin_rintf(pax * sn + pay * cs));
         const int32_t bx = static_cast<int32_t>(__builtin_rintf(pbx * cs - pby * sn));
         const int32_t by = static_cast<int32_t>(__builtin_rintf(pbx * sn + pby * cs));
-        const float sa = gaussian_tap9(S, static_cast<uint32_t>(static_cast<int32_t>(x) + ax),
-                                       static_cast<uint32_t>(static_cast<int32_t>(y) + ay));
-        const float sb = gaussian_tap9(S, static_cast<uint32_t>(static_cast<int32_t>(x) + bx),
-                                       static_cast<uint32_t>(static_cast<int32_t>(y) + by));
+        // G at the wrapped u32 coordinates (generate_features.wgsl:116-129),
+        // looked up in the smoothed image; coordinates past the extension
+        // (including wrapped negatives) share the value at w+3 / h+3.
+        const uint32_t pax_u = umin(static_cast<uint32_t>(static_cast<int32_t>(x) + ax), sw - 1);
+        const uint32_t pay_u = umin(static_cast<uint32_t>(static_cast<int32_t>(y) + ay), sh - 1);
+        const uint32_t pbx_u = umin(static_cast<uint32_t>(static_cast<int32_t>(x) + bx), sw - 1);
+        const uint32_t pby_u = umin(static_cast<uint32_t>(static_cast<int32_t>(y) + by), sh - 1);
+        const float sa = sm[pay_u * sw + pax_u];
+        const float sb = sm[pby_u * sw + pbx_u];
         const uint64_t bits = __ballot(sa > sb);
         desc[2 * k] = static_cast<uint32_t>(bits);
         desc[2 * k + 1] = static_cast<uint32_t>(bits >> 32);
@@ -496,9 +547,6 @@ __device__ inline uint32_t wave_lower_bound(uint32_t n, uint32_t lane, Pred pred
     }
     return lo;
 }
-
-__device__ inline uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
-__device__ inline uint32_t umax(uint32_t a, uint32_t b) { return a > b ? a : b; }
 
 }  // namespace
 
