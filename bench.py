@@ -72,6 +72,167 @@ def cpu_baseline(frames, n_sample):
     return n_sample / dt, dt
 
 
+INT_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 int32 lanes/clk x 2.4 GHz
+
+
+def capture_graph(torch, launch):
+    """Capture launch(stream) into a HIP graph (torch's graph API drives
+    hipStreamBeginCapture on the stream the kernels are enqueued on)."""
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        launch(side)  # warm-up outside capture
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        launch(torch.cuda.current_stream())
+    return g
+
+
+def timed(torch, fn, reps):
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def phase_batched_sequence(torch, orbfe, ctx, d_frames, feats, matches, reps):
+    """The config-1 sequence as one prepared batch (32 detects) + one
+    prepared match batch (31 pairs), captured in a HIP graph."""
+    frames = [{"image": d_frames[t], "width": W, "height": H, "pitch": W, "cap": CAP,
+               "features": feats[t]} for t in range(N_FRAMES)]
+    pairs = [{"prev": feats[t - 1], "curr": feats[t], "prev_match": matches[t - 1],
+              "width": W, "height": H, "cap": CAP} for t in range(1, N_FRAMES)]
+    b = ctx.batch_create(frames)
+    mb = ctx.match_batch_create(pairs)
+
+    def launch(stream):
+        b.detect_async(stream=stream)
+        mb.match_async(stream=stream)
+    g = capture_graph(torch, launch)
+    dt = timed(torch, g.replay, reps)
+    return {"frames_per_s": round(N_FRAMES / dt, 1), "ms_per_step": round(dt * 1e3, 4),
+            "launches_per_step": 5, "graph": True}, (b, mb, g)
+
+
+def phase_mixed(torch, orbfe, synth, reps):
+    """Config 2: 64 frames of random resolution and cap, each matched against
+    its translated twin; plain per-frame launches vs one prepared batch per
+    stage in a HIP graph."""
+    items = synth.mixed_batch(2, n_frames=64)
+    dev = torch.device("cuda:0")
+    ctx = orbfe.Context(752, 480, fast_threshold=thr20(), lowe_ratio=RATIO)
+    per_res = {}
+    frames, pairs, plain_calls = [], [], []
+    for it in items:
+        h, w = it["a"].shape
+        cap = it["cap"]
+        key = (w, h, cap)
+        if key not in per_res:
+            per_res[key] = orbfe.Context(w, h, features_per_level=cap, fast_threshold=thr20(),
+                                         lowe_ratio=RATIO)
+        c = per_res[key]
+        fa = torch.zeros(orbfe.feature_array_bytes(6, cap), dtype=torch.uint8, device=dev)
+        fb = torch.zeros_like(fa)
+        pm = torch.zeros(6 * cap, dtype=torch.int32, device=dev)
+        da = torch.from_numpy(it["a"]).to(dev)
+        db = torch.from_numpy(it["b"]).to(dev)
+        frames += [{"image": da, "width": w, "height": h, "pitch": w, "cap": cap, "features": fa},
+                   {"image": db, "width": w, "height": h, "pitch": w, "cap": cap, "features": fb}]
+        pairs.append({"prev": fa, "curr": fb, "prev_match": pm, "width": w, "height": h,
+                      "cap": cap})
+        plain_calls.append((c, da, db, w, fa, fb, pm))
+    stream = torch.cuda.current_stream()
+
+    def plain():
+        for c, da, db, w, fa, fb, pm in plain_calls:
+            c.detect_async(da, w, fa, stream=stream)
+            c.detect_async(db, w, fb, stream=stream)
+            c.match_pipeline_async(fa, fb, pm, stream=stream)
+    t_plain = timed(torch, plain, reps)
+    b = ctx.batch_create(frames)
+    mb = ctx.match_batch_create(pairs)
+
+    def launch(st):
+        b.detect_async(stream=st)
+        mb.match_async(stream=st)
+    g = capture_graph(torch, launch)
+    t_batch = timed(torch, g.replay, reps)
+    n = len(frames)
+    return {"frames": n, "pairs": len(pairs),
+            "plain_frames_per_s": round(n / t_plain, 1),
+            "batched_frames_per_s": round(n / t_batch, 1),
+            "speedup": round(t_plain / t_batch, 2),
+            "plain_ms": round(t_plain * 1e3, 3), "batched_ms": round(t_batch * 1e3, 3)}
+
+
+def phase_matcher(torch, orbfe, synth, n, reps):
+    """Config 3: brute-force best/second-best over n x n 256-bit descriptors."""
+    q, t, _, _ = synth.descriptors(n, n, n)
+    dev = torch.device("cuda:0")
+    dq = torch.from_numpy(q).to(dev)
+    dt_ = torch.from_numpy(t).to(dev)
+    idx = torch.empty(n, dtype=torch.int32, device=dev)
+    d1 = torch.empty_like(idx)
+    d2 = torch.empty_like(idx)
+    ctx = orbfe.Context(64, 48)
+    stream = torch.cuda.current_stream()
+    dt = timed(torch, lambda: ctx.match_bf_async(dq, n, dt_, n, idx, d1, d2, stream=stream), reps)
+    ctx.timing(True)
+    ctx.timing_reset()
+    for _ in range(reps):
+        ctx.match_bf_async(dq, n, dt_, n, idx, d1, d2, stream=stream)
+    torch.cuda.synchronize()
+    launches, ms = ctx.timing_read("k_bf_partial")
+    ctx.timing(False)
+    kern = ms / launches * 1e-3
+    pairs = n * n
+    return {"n_query": n, "n_train": n, "ms": round(dt * 1e3, 4),
+            "pairs_per_s": round(pairs / dt, 1),
+            "effective_GBps": round(32 * pairs / dt / 1e9, 1),
+            "int_Tops": round(16 * pairs / kern / 1e12, 2),
+            "int_peak_Tops": round(INT_PEAK_TOPS, 2),
+            "int_frac": round(16 * pairs / kern / 1e12 / INT_PEAK_TOPS, 4),
+            "k_bf_partial_us": round(kern * 1e6, 2)}
+
+
+def phase_serving(torch, orbfe, synth, seconds, seed=4):
+    """Config 4: 32 streams at mixed resolutions, Poisson arrivals at
+    20-60 fps each, continuous batching on two in-flight slots."""
+    from uni_slam_amd import serve
+    rng = np.random.Generator(np.random.PCG64(seed))
+    specs, rates, pool = [], [], []
+    for s in range(32):
+        w, h = synth.RESOLUTIONS[int(rng.integers(0, 4))]
+        specs.append(serve.StreamSpec(w, h, 2000))
+        rates.append(float(rng.uniform(20, 60)))
+        fr, _ = synth.sequence(1000 + s, w, h, n_frames=4)
+        pool.append([torch.from_numpy(f).pin_memory() for f in fr])
+    arrivals = serve.poisson_arrivals(rates, seconds, seed)
+    srv = serve.Server(specs, fast_threshold=thr20(), lowe_ratio=RATIO, hbm_budget=2 << 30)
+    wall, done = serve.run_schedule(srv, arrivals, pool)
+    stats = serve.latency_stats(done)
+    vga = [c for c in done if (specs[c.stream].width, specs[c.stream].height) == (752, 480)]
+    out = {"streams": 32, "seconds": seconds, "frames": len(done),
+           "frames_per_s": round(len(done) / wall, 1), "batches": srv.batches,
+           "max_batch": srv.max_batch_seen,
+           "latency_ms": {k: round(v, 3) for k, v in stats.items() if k.endswith("_ms")}}
+    if vga:
+        out["latency_752x480_ms"] = {k: round(v, 3) for k, v in serve.latency_stats(vga).items()
+                                     if k.endswith("_ms")}
+    # capacity: the same streams with every frame of a 2 s schedule already
+    # waiting (arrivals compressed to zero), frames/s of the drained backlog
+    srv2 = serve.Server(specs, fast_threshold=thr20(), lowe_ratio=RATIO, hbm_budget=2 << 30)
+    arr2 = serve.poisson_arrivals(rates, 2.0, seed + 1)
+    wall2, done2 = serve.run_schedule(srv2, arr2, pool, time_scale=0.0)
+    out["saturated_frames_per_s"] = round(len(done2) / wall2, 1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +241,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=8,
                     help="frames of the sequence the CPU baseline processes")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the batched / mixed / matcher / serving phases")
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -143,6 +307,20 @@ def main():
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None,
                 "avg_us": per_kernel[dom]["avg_us"], "bytes_per_launch": kb[dom]}
 
+    extra = {}
+    if not args.no_extra:
+        bseq, _keep = phase_batched_sequence(torch, orbfe, ctx, d_frames, feats, matches,
+                                             max(5, args.steps))
+        bseq["speedup_vs_plain"] = round(bseq["frames_per_s"] / fps, 2)
+        extra["batched_sequence"] = bseq
+        log("batched sequence:", json.dumps(bseq))
+        extra["mixed_batch"] = phase_mixed(torch, orbfe, synth, max(3, args.steps // 4))
+        log("mixed batch:", json.dumps(extra["mixed_batch"]))
+        extra["matcher"] = phase_matcher(torch, orbfe, synth, 65536, 3)
+        log("matcher:", json.dumps(extra["matcher"]))
+        extra["serving"] = phase_serving(torch, orbfe, synth, args.serve_seconds)
+        log("serving:", json.dumps(extra["serving"]))
+
     cpu = None
     if not args.no_cpu:
         cfps, cdt = cpu_baseline(frames, args.cpu_sample)
@@ -166,6 +344,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    out.update(extra)
     print(json.dumps(out), flush=True)
 
 

@@ -250,7 +250,8 @@ int get_plan(orbfe_ctx* ctx, uint32_t W0, uint32_t H0, uint32_t cap, PlanEntry**
     P.L = L;
     P.cap = cap;
     P.feat_stride = 8 + 48 * cap;
-    P.desc_groups = ceil_div(cap, 4);
+    P.desc_groups = ceil_div(cap, orbfe::kDescPerBlock);
+    P.query_groups = ceil_div(cap, orbfe::kQueriesPerBlock);
     P.max_dx = 0.1f * static_cast<float>(W0);
     P.max_dy = 0.1f * static_cast<float>(H0);
     uint32_t tiles = 0, words = 0, pyr = 0, sm = 0;
@@ -443,7 +444,7 @@ int layout_match(orbfe_ctx* ctx, const orbfe_match_pair* pairs, uint32_t n, Matc
         M.prev_match = p.prev_match;
         M.b_to_a = reinterpret_cast<int32_t*>(arena + off);
         off += pe->match_bytes;
-        lay.prefix[i + 1] = lay.prefix[i] + pe->host.L * pe->host.desc_groups;
+        lay.prefix[i + 1] = lay.prefix[i] + pe->host.L * pe->host.query_groups;
     }
     lay.arena_bytes = off;
     lay.off_prefix = align_up(sizeof(MatchJob) * n);
@@ -786,7 +787,7 @@ int orbfe_match_pipeline_async(orbfe_ctx* ctx, const void* prev_features,
     job0.prev_match = prev_match;
     job0.b_to_a = ctx->b_to_a;
     const MatchBatchView bv{nullptr, nullptr, 0};
-    return launch_match_stage(ctx, bv, job0, P.L * P.desc_groups, stream);
+    return launch_match_stage(ctx, bv, job0, P.L * P.query_groups, stream);
 }
 
 int orbfe_match_pipeline_batch_async(orbfe_ctx* ctx, const orbfe_match_pair* pairs, uint32_t n,

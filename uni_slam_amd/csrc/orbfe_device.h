@@ -30,6 +30,11 @@ constexpr int kLdsW = kTileW + 2 * kHalo;  // 70
 constexpr int kLdsH = kTileH + 2 * kHalo;  // 22
 constexpr int kDetectThreads = 256;
 
+// k_describe: keypoints per workgroup (4 per wave, 16 lanes each).
+constexpr uint32_t kDescPerBlock = 16;
+// k_match_dir: queries per workgroup (one per wave).
+constexpr uint32_t kQueriesPerBlock = 4;
+
 // Key packing of the pipeline matcher: distance (<= 256) above a 22-bit
 // candidate id (2 bits pool level slot, 20 bits index).
 constexpr uint32_t kMatchKeyShift = 22;
@@ -63,7 +68,8 @@ struct Plan {
     uint32_t words;       // 64-bit survivor words of one frame
     uint32_t pyr_floats;  // pyramid floats (levels >= 1)
     uint32_t feat_stride; // 8 + 48*cap
-    uint32_t desc_groups; // describe blocks per level = ceil(cap / 4)
+    uint32_t desc_groups; // describe blocks per level = ceil(cap / kDescPerBlock)
+    uint32_t query_groups; // matcher blocks per level = ceil(cap / kQueriesPerBlock)
     float max_dx, max_dy; // 0.1f*W0, 0.1f*H0 (match_features_cpu.cpp:76-77)
     uint32_t sm_floats;   // smoothed-image floats of one frame
     const uint64_t* mask; // static cull mask, `words` words

@@ -9,8 +9,9 @@
 //   k_describe  one wave per keypoint: corner strength, intensity-centroid
 //               angle, 256 rotated 9-tap BRIEF tests (4 per lane, bits from
 //               ballots), 48-byte feature record.
-//   k_match_dir pipeline matcher, one wave per query: LOD+-1 pool, 64-ary
-//               wave search of the y window, XOR/popcount over the x window,
+//   k_match_dir pipeline matcher, one wave per query: LOD+-1 pool, six
+//               concurrent lane-group searches of the y windows, XOR/popcount
+//               over the x window,
 //               best/second-best keys reduced across the wave.
 //   k_bf_partial / k_bf_merge  brute-force best/second-best Hamming search,
 //               queries in registers, train tiles in LDS.
@@ -307,6 +308,23 @@ __device__ inline uint32_t wave_inclusive_sum(uint32_t v, uint32_t lane) {
     return v;
 }
 
+// Survivor words a wave keeps in registers (64 lanes x kCompactChunks):
+// enough for a 1920x1080 level 0 (32,400 words over 16 waves); larger
+// levels take the re-reading path.
+constexpr uint32_t kCompactChunks = 32;
+
+__device__ inline void emit_word(uint64_t v, uint32_t k, uint32_t rank, uint32_t cap,
+                                 uint32_t wpr, uint32_t* kp) {
+    if (!v || rank >= cap) return;
+    const uint32_t y = k / wpr;
+    const uint32_t xb = (k - y * wpr) * 64;
+    while (v && rank < cap) {
+        const uint32_t bit = __ffsll(static_cast<unsigned long long>(v)) - 1;
+        v &= v - 1;
+        kp[rank++] = (xb + bit) | (y << 16);
+    }
+}
+
 __global__ __launch_bounds__(kCompactThreads) void k_compact(BatchView bv, Job job0,
                                                               uint32_t levels) {
     __shared__ uint32_t wave_total[kCompactWaves];
@@ -323,9 +341,21 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(BatchView bv, Job j
     const uint32_t seg = (words + kCompactWaves - 1) / kCompactWaves;
     const uint32_t begin = wave * seg < words ? wave * seg : words;
     const uint32_t end = begin + seg < words ? begin + seg : words;
+    const uint32_t nch = (end - begin + 63) / 64;
+    const bool in_regs = nch <= kCompactChunks;  // block-uniform (seg is)
 
+    uint64_t wv[kCompactChunks];
     uint32_t cnt = 0;
-    for (uint32_t k = begin + lane; k < end; k += 64) cnt += __popcll(surv[k]);
+    if (in_regs) {
+#pragma unroll
+        for (uint32_t c = 0; c < kCompactChunks; ++c) {
+            const uint32_t k = begin + c * 64 + lane;
+            wv[c] = (c < nch && k < end) ? surv[k] : 0ull;
+            cnt += __popcll(wv[c]);
+        }
+    } else {
+        for (uint32_t k = begin + lane; k < end; k += 64) cnt += __popcll(surv[k]);
+    }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
     if (lane == 0) wave_total[wave] = cnt;
@@ -337,22 +367,25 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(BatchView bv, Job j
         total += tv;
     }
     uint32_t* kp = J.kp + G.kp_off;
-    for (uint32_t k0 = begin; k0 < end && base < P.cap; k0 += 64) {
-        const uint32_t k = k0 + lane;
-        uint64_t v = k < end ? surv[k] : 0ull;
-        const uint32_t c = __popcll(v);
-        const uint32_t incl = wave_inclusive_sum(c, lane);
-        uint32_t rank = base + incl - c;
-        if (v && rank < P.cap) {
-            const uint32_t y = k / G.wpr;
-            const uint32_t xb = (k - y * G.wpr) * 64;
-            while (v && rank < P.cap) {
-                const uint32_t bit = __ffsll(static_cast<unsigned long long>(v)) - 1;
-                v &= v - 1;
-                kp[rank++] = (xb + bit) | (y << 16);
+    if (in_regs) {
+#pragma unroll
+        for (uint32_t c = 0; c < kCompactChunks; ++c) {
+            if (c < nch && base < P.cap) {  // wave-uniform
+                const uint32_t c1 = __popcll(wv[c]);
+                const uint32_t incl = wave_inclusive_sum(c1, lane);
+                emit_word(wv[c], begin + c * 64 + lane, base + incl - c1, P.cap, G.wpr, kp);
+                base += __shfl(incl, 63);
             }
         }
-        base += __shfl(incl, 63);
+    } else {
+        for (uint32_t k0 = begin; k0 < end && base < P.cap; k0 += 64) {
+            const uint32_t k = k0 + lane;
+            const uint64_t v = k < end ? surv[k] : 0ull;
+            const uint32_t c1 = __popcll(v);
+            const uint32_t incl = wave_inclusive_sum(c1, lane);
+            emit_word(v, k, base + incl - c1, P.cap, G.wpr, kp);
+            base += __shfl(incl, 63);
+        }
     }
     if (tid == 0) {
         uint32_t* hdr = reinterpret_cast<uint32_t*>(J.feat + G.feat_off);
@@ -388,7 +421,9 @@ struct LevelSrc {
 
 }  // namespace
 
-constexpr uint32_t kDescribeThreads = 256;  // 4 waves = 4 keypoints
+// Four keypoints per wave, 16 lanes each (16 per workgroup): lane s of a
+// group scores ring pixel s and evaluates tests 16*s .. 16*s+15.
+constexpr uint32_t kDescribeThreads = 256;
 
 __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job job0,
                                                                 Params prm) {
@@ -408,17 +443,21 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
     const uint32_t l = b / P.desc_groups;
     const uint32_t g = b - l * P.desc_groups;
     const uint32_t lane = tid & 63;
-    const uint32_t slot = g * 4 + (tid >> 6);
-    if (slot >= P.cap) return;
+    const uint32_t sub = lane & 15;        // lane within the keypoint group
+    const uint32_t gbase = lane & ~15u;    // first lane of the group
+    const uint32_t slot = g * kDescPerBlock + (tid >> 4);
     const LevelGeom& G = P.lv[l];
     uint8_t* block = J.feat + G.feat_off;
     const uint32_t count = reinterpret_cast<const uint32_t*>(block)[0];
+    const uint32_t kept = count < P.cap ? count : P.cap;
+    const bool in_cap = slot < P.cap;
+    const bool active = slot < kept;
     uint32_t* out = reinterpret_cast<uint32_t*>(block + 8 + 48ull * slot);
-    if (slot >= (count < P.cap ? count : P.cap)) {
-        if (lane < 12) out[lane] = 0u;
+    if (__ballot(active) == 0) {  // whole wave empty: zero its slots
+        if (in_cap && sub < 12) out[sub] = 0u;
         return;
     }
-    const uint32_t kpv = J.kp[G.kp_off + slot];
+    const uint32_t kpv = active ? J.kp[G.kp_off + slot] : 0u;
     const uint32_t x = kpv & 0xFFFFu;
     const uint32_t y = kpv >> 16;
 
@@ -431,19 +470,15 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
     S.h = G.h;
     S.base = l == 0;
 
-    // Corner strength (detect_corners.wgsl:58-81): lanes 0..15 hold the ring.
+    // Corner strength (detect_corners.wgsl:58-81): lane s holds ring pixel s.
     const float center = S.at(x, y);
-    bool diff = false;
-    if (lane < 16) {
-        const float counter = S.load(static_cast<int32_t>(x) + c_circle_x[lane],
-                                     static_cast<int32_t>(y) + c_circle_y[lane],
-                                     prm.border_zero);
-        diff = __builtin_fabsf(center - counter) > prm.threshold;
-    }
-    const uint64_t ring = __ballot(diff);
+    const float counter = S.load(static_cast<int32_t>(x) + c_circle_x[sub],
+                                 static_cast<int32_t>(y) + c_circle_y[sub], prm.border_zero);
+    const bool diff = __builtin_fabsf(center - counter) > prm.threshold;
+    const uint32_t ring = static_cast<uint32_t>(__ballot(diff) >> gbase) & 0xFFFFu;
     uint32_t curr = 1, streak = 1;
     for (int i = 0; i < 16; ++i) {
-        curr = ((ring >> i) & 1ull) ? curr + 1 : 1;
+        curr = ((ring >> i) & 1u) ? curr + 1 : 1;
         streak = curr > streak ? curr : streak;
     }
     const uint32_t strength = streak > prm.min_run ? streak : 0u;
@@ -464,43 +499,48 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
     float sn, cs;
     sincos_own(theta, &sn, &cs);
 
-    // Descriptor (generate_features.wgsl:101-144): test i = lane + 64k.
+    // Descriptor (generate_features.wgsl:101-144): lane s runs tests
+    // 16s .. 16s+15; G at the wrapped u32 coordinates (:116-129) is looked
+    // up in the smoothed image, where coordinates past the extension
+    // (including wrapped negatives) share the value at w+3 / h+3.
     const uint32_t sw = G.w + kSmoothExt, sh = G.h + kSmoothExt;
     const float* sm = J.smooth + G.sm_off;
-    uint32_t desc[8];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t i = lane + 64 * k;
+    uint32_t bits = 0;
+#pragma unroll 4
+    for (uint32_t k = 0; k < 16; ++k) {
+        const uint32_t i = 16 * sub + k;
         const float pax = c_pattern[i][0], pay = c_pattern[i][1];
         const float pbx = c_pattern[i][2], pby = c_pattern[i][3];
         const int32_t ax = static_cast<int32_t>(__builtin_rintf(pax * cs - pay * sn));
         const int32_t ay = static_cast<int32_t>(__builtin_rintf(pax * sn + pay * cs));
         const int32_t bx = static_cast<int32_t>(__builtin_rintf(pbx * cs - pby * sn));
         const int32_t by = static_cast<int32_t>(__builtin_rintf(pbx * sn + pby * cs));
-        // G at the wrapped u32 coordinates (generate_features.wgsl:116-129),
-        // looked up in the smoothed image; coordinates past the extension
-        // (including wrapped negatives) share the value at w+3 / h+3.
         const uint32_t pax_u = umin(static_cast<uint32_t>(static_cast<int32_t>(x) + ax), sw - 1);
         const uint32_t pay_u = umin(static_cast<uint32_t>(static_cast<int32_t>(y) + ay), sh - 1);
         const uint32_t pbx_u = umin(static_cast<uint32_t>(static_cast<int32_t>(x) + bx), sw - 1);
         const uint32_t pby_u = umin(static_cast<uint32_t>(static_cast<int32_t>(y) + by), sh - 1);
         const float sa = sm[pay_u * sw + pax_u];
         const float sb = sm[pby_u * sw + pbx_u];
-        const uint64_t bits = __ballot(sa > sb);
-        desc[2 * k] = static_cast<uint32_t>(bits);
-        desc[2 * k + 1] = static_cast<uint32_t>(bits >> 32);
+        bits |= (sa > sb ? 1u : 0u) << k;
     }
-
-    if (lane < 12) {
+    // Word w of the descriptor = bits of lanes 2w (low half) and 2w+1.
+    const uint32_t wsel = sub >= 4 ? sub - 4 : 0;
+    const uint32_t lo = __shfl(bits, static_cast<int>(gbase + 2 * wsel));
+    const uint32_t hi = __shfl(bits, static_cast<int>(gbase + 2 * wsel + 1));
+    if (in_cap && sub < 12) {
         uint32_t v;
-        switch (lane) {
-            case 0: v = __float_as_uint(static_cast<float>(x)); break;
-            case 1: v = __float_as_uint(static_cast<float>(y)); break;
-            case 2: v = strength; break;
-            case 3: v = __float_as_uint(theta); break;
-            default: v = desc[lane - 4]; break;
+        if (!active) {
+            v = 0u;
+        } else {
+            switch (sub) {
+                case 0: v = __float_as_uint(static_cast<float>(x)); break;
+                case 1: v = __float_as_uint(static_cast<float>(y)); break;
+                case 2: v = strength; break;
+                case 3: v = __float_as_uint(theta); break;
+                default: v = lo | (hi << 16); break;
+            }
         }
-        out[lane] = v;
+        out[sub] = v;
     }
 }
 
@@ -521,33 +561,6 @@ struct FeatView {
     }
 };
 
-// First index in [0, n) at which the monotone predicate (false..true) holds,
-// or n; 64 probes per round.
-template <typename Pred>
-__device__ inline uint32_t wave_lower_bound(uint32_t n, uint32_t lane, Pred pred) {
-    uint32_t lo = 0, hi = n;
-    while (lo < hi) {
-        const uint32_t stride = (hi - lo + 63) / 64;
-        const uint32_t pos = lo + lane * stride;
-        const bool p = pos < hi ? pred(pos) : true;
-        const uint64_t m = __ballot(p);
-        if (m == 0) {
-            lo = lo + 63 * stride + 1;
-            continue;
-        }
-        const uint32_t f = __ffsll(static_cast<unsigned long long>(m)) - 1;
-        if (f == 0) {
-            hi = lo;
-            break;
-        }
-        const uint32_t nlo = lo + (f - 1) * stride + 1;
-        const uint32_t cand = lo + f * stride;
-        hi = cand < hi ? cand : hi;
-        lo = nlo;
-    }
-    return lo;
-}
-
 }  // namespace
 
 constexpr uint32_t kMatchThreads = 256;  // 4 waves = 4 queries
@@ -567,9 +580,9 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
     }
     const MatchJob M = bv.n ? bv.jobs[j] : job0;
     const Plan& P = *M.plan;
-    const uint32_t l = b / P.desc_groups;
-    const uint32_t g = b - l * P.desc_groups;
-    const uint32_t slot = g * 4 + (tid >> 6);
+    const uint32_t l = b / P.query_groups;
+    const uint32_t g = b - l * P.query_groups;
+    const uint32_t slot = g * kQueriesPerBlock + (tid >> 6);
     if (slot >= P.cap) return;
     const FeatView Q{kMutual ? M.prev : M.curr, P.feat_stride};
     const FeatView C{kMutual ? M.curr : M.prev, P.feat_stride};
@@ -600,17 +613,60 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
         const float hi_x = qx + P.max_dx;
         const float dy = P.max_dy;
 
+        // y windows |qy - cy| <= dy of the pool levels are contiguous ranges
+        // (records are in raster order).  Six searches run at once: lane
+        // group q = lane / 10 (q < 6) looks for bound q & 1 of pool slot
+        // q >> 1 with 10 probes per round.
+        const uint32_t grp = lane / 10;
+        const uint32_t gl = lane - grp * 10;
+        const uint32_t gs = grp >> 1;
+        const bool gvalid = grp < 6 && gs < npool;
+        const uint32_t gcl = gs == 0 ? pl[0] : (gs == 1 ? pl[1] : pl[2]);
+        const uint32_t gn = gvalid ? (gs == 0 ? pn[0] : (gs == 1 ? pn[1] : pn[2])) : 0u;
+        const float gsc = P.lv[gvalid ? gcl : 0].lod0_scale;
+        uint32_t glo = 0, ghi = gn;
+        while (__ballot(glo < ghi) != 0) {
+            bool p = true;
+            uint32_t stride = 1;
+            if (glo < ghi) {
+                stride = (ghi - glo + 9) / 10;
+                const uint32_t pos = glo + gl * stride;
+                if (pos < ghi) {
+                    const float cy = static_cast<float>(static_cast<uint32_t>(
+                                         __uint_as_float(C.rec(gcl, pos)[1]))) * gsc;
+                    const float d = qy - cy;
+                    p = (grp & 1) ? (d < -dy) : (d <= dy);
+                }
+            }
+            const uint32_t m = grp < 6 ? static_cast<uint32_t>(__ballot(p) >> (grp * 10)) & 0x3FFu
+                                       : 0x3FFu;
+            if (glo < ghi) {
+                if (m == 0) {
+                    glo = glo + 9 * stride + 1;
+                } else {
+                    const uint32_t f = __ffs(m) - 1;
+                    if (f == 0) {
+                        ghi = glo;
+                    } else {
+                        const uint32_t cand = glo + f * stride;
+                        glo = glo + (f - 1) * stride + 1;
+                        ghi = cand < ghi ? cand : ghi;
+                    }
+                }
+            }
+        }
+        uint32_t wlo[3], whi[3];
+#pragma unroll
+        for (uint32_t s = 0; s < 3; ++s) {
+            wlo[s] = __shfl(glo, static_cast<int>(20 * s));
+            whi[s] = __shfl(glo, static_cast<int>(20 * s + 10));
+        }
+
         uint32_t min1 = 0xFFFFFFFFu, min2 = 0xFFFFFFFFu;
         for (uint32_t s = 0; s < npool; ++s) {
             const uint32_t cl = pl[s];
             const float cs = P.lv[cl].lod0_scale;
-            auto cy_at = [&](uint32_t i) {
-                return static_cast<float>(static_cast<uint32_t>(__uint_as_float(C.rec(cl, i)[1]))) * cs;
-            };
-            // y window: |qy - cy| <= dy, a contiguous range because records
-            // are in raster order.
-            const uint32_t lo = wave_lower_bound(pn[s], lane, [&](uint32_t i) { return qy - cy_at(i) <= dy; });
-            const uint32_t hi = wave_lower_bound(pn[s], lane, [&](uint32_t i) { return qy - cy_at(i) < -dy; });
+            const uint32_t lo = wlo[s], hi = whi[s];
             for (uint32_t i = lo + lane; i < hi; i += 64) {
                 const uint32_t* cr = C.rec(cl, i);
                 const float cx = static_cast<float>(static_cast<uint32_t>(__uint_as_float(cr[0]))) * cs;

@@ -1,0 +1,253 @@
+"""Continuous-batching front-end server for many camera streams on one GPU.
+
+Frames arrive from many streams at their own times.  The server keeps, per
+stream, the feature arrays of the previous and current frame, a pyramid
+buffer and two upload slots resident in HBM.  Each scheduling tick takes the
+frames that have arrived, admits as many as the HBM budget and the in-flight
+slots allow, uploads them and launches one batched detect (all admitted
+frames, any mix of sizes) and one batched match (each frame against its
+stream's previous frame) on one of two HIP streams, so a new batch overlaps
+the one already on the device.  A stream's next frame waits (on the device)
+for the batch holding its previous frame.  Latency = completion - arrival.
+
+Results are bit-identical to the plain per-frame path: the batched kernels
+compute each frame exactly as the per-frame launch does.
+"""
+from __future__ import annotations
+
+import collections
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import orbfe
+
+LEVELS = 6
+
+
+def frame_scratch_bytes(width: int, height: int, cap: int) -> int:
+    """Device scratch one in-flight frame uses in a batch (survivor bitmap,
+    keypoint list, smoothed image), as laid out by liborbfe."""
+    dims = orbfe.level_dims(width, height)
+    words = sum(h * ((w + 63) // 64) for w, h in dims)
+    smooth = sum((w + 4) * (h + 4) for w, h in dims)
+
+    def al(v):
+        return (v + 255) // 256 * 256
+    return al(8 * words) + al(4 * LEVELS * cap) + al(4 * smooth) + al(4 * LEVELS * cap)
+
+
+@dataclass
+class StreamSpec:
+    width: int
+    height: int
+    cap: int
+
+
+@dataclass
+class _Stream:
+    spec: StreamSpec
+    feats: List[torch.Tensor]
+    pyramid: torch.Tensor
+    match: torch.Tensor
+    upload: List[torch.Tensor]
+    frames_done: int = 0
+    last_event: Optional[torch.cuda.Event] = None
+    scratch: int = 0
+
+
+@dataclass
+class _Request:
+    stream: int
+    seq: int            # frame number within its stream
+    t_arrival: float
+    host: torch.Tensor  # pinned u8 (H, W)
+
+
+@dataclass
+class Completed:
+    stream: int
+    seq: int
+    t_arrival: float
+    t_done: float
+
+
+class Server:
+    def __init__(self, specs: Sequence[StreamSpec], device: int = 0,
+                 hbm_budget: int = 1 << 30, max_batch: int = 64, inflight: int = 2,
+                 fast_threshold: float = 0.15, lowe_ratio: float = 0.85,
+                 keep_outputs: bool = False):
+        self.dev = torch.device(f"cuda:{device}")
+        self.specs = list(specs)
+        self.budget = hbm_budget
+        self.max_batch = max_batch
+        self.keep_outputs = keep_outputs
+        self.outputs: Dict[Tuple[int, int], Tuple[np.ndarray, Optional[np.ndarray]]] = {}
+        self._snap: Dict[Tuple[int, int], Tuple[torch.Tensor, Optional[torch.Tensor]]] = {}
+        w0, h0 = self.specs[0].width, self.specs[0].height
+        self.ctxs = [orbfe.Context(w0, h0, device=device, fast_threshold=fast_threshold,
+                                   lowe_ratio=lowe_ratio) for _ in range(inflight)]
+        self.hip_streams = [torch.cuda.Stream(device=self.dev) for _ in range(inflight)]
+        self.busy: List[Optional[Tuple[torch.cuda.Event, List[_Request], int]]] = [None] * inflight
+        self.streams: List[_Stream] = []
+        for sp in self.specs:
+            fb = orbfe.feature_array_bytes(LEVELS, sp.cap)
+            npyr = sum(w * h for w, h in orbfe.level_dims(sp.width, sp.height)[1:])
+            self.streams.append(_Stream(
+                spec=sp,
+                feats=[torch.zeros(fb, dtype=torch.uint8, device=self.dev) for _ in range(2)],
+                pyramid=torch.zeros(npyr, dtype=torch.float32, device=self.dev),
+                match=torch.full((LEVELS * sp.cap,), -1, dtype=torch.int32, device=self.dev),
+                upload=[torch.empty((sp.height, sp.width), dtype=torch.uint8, device=self.dev)
+                        for _ in range(2)],
+                scratch=frame_scratch_bytes(sp.width, sp.height, sp.cap)))
+        self.pending: collections.deque = collections.deque()
+        self.completed: List[Completed] = []
+        self.in_flight_bytes = 0
+        self.max_batch_seen = 0
+        self.batches = 0
+
+    # -- requests --------------------------------------------------------------
+    def submit(self, stream: int, seq: int, host_frame: torch.Tensor, t_arrival: float):
+        self.pending.append(_Request(stream, seq, t_arrival, host_frame))
+
+    def idle(self) -> bool:
+        return not self.pending and all(b is None for b in self.busy)
+
+    # -- scheduling -------------------------------------------------------------
+    def poll(self, now_fn=time.perf_counter):
+        """Retire finished batches; record their completion time."""
+        for i, b in enumerate(self.busy):
+            if b is None:
+                continue
+            ev, reqs, nbytes = b
+            if ev.query():
+                t = now_fn()
+                for r in reqs:
+                    self.completed.append(Completed(r.stream, r.seq, r.t_arrival, t))
+                    if self.keep_outputs:
+                        f, m = self._snap.pop((r.stream, r.seq))
+                        self.outputs[(r.stream, r.seq)] = (
+                            f.cpu().numpy(), None if m is None else m.cpu().numpy())
+                self.in_flight_bytes -= nbytes
+                self.busy[i] = None
+
+    def launch(self) -> bool:
+        """Admit pending frames into a free slot; returns True if launched."""
+        if not self.pending:
+            return False
+        try:
+            slot = self.busy.index(None)
+        except ValueError:
+            return False
+        # FIFO admission: one frame per stream per batch (a stream's frames
+        # stay in order), up to max_batch, within the HBM budget (a frame is
+        # always admitted when nothing else is in flight).
+        chosen: List[_Request] = []
+        seen = set()
+        nbytes = 0
+        full = False
+        keep: collections.deque = collections.deque()
+        for r in self.pending:
+            st = self.streams[r.stream]
+            if full or len(chosen) >= self.max_batch or r.stream in seen:
+                keep.append(r)
+                continue
+            if (self.in_flight_bytes + nbytes + st.scratch > self.budget
+                    and (chosen or self.in_flight_bytes > 0)):
+                full = True
+                keep.append(r)
+                continue
+            seen.add(r.stream)
+            chosen.append(r)
+            nbytes += st.scratch
+        self.pending = keep
+        if not chosen:
+            return False
+        ctx, hs = self.ctxs[slot], self.hip_streams[slot]
+        frames, pairs = [], []
+        with torch.cuda.stream(hs):
+            for r in chosen:
+                st = self.streams[r.stream]
+                if st.last_event is not None:
+                    hs.wait_event(st.last_event)  # previous frame of this stream
+                up = st.upload[r.seq % 2]
+                up.copy_(r.host, non_blocking=True)
+                sp = st.spec
+                frames.append({"image": up, "width": sp.width, "height": sp.height,
+                               "pitch": sp.width, "cap": sp.cap, "pyramid": st.pyramid,
+                               "features": st.feats[r.seq % 2]})
+                if r.seq > 0:
+                    pairs.append({"prev": st.feats[(r.seq - 1) % 2],
+                                  "curr": st.feats[r.seq % 2], "prev_match": st.match,
+                                  "width": sp.width, "height": sp.height, "cap": sp.cap})
+            ctx.detect_batch_async(frames, stream=hs)
+            if pairs:
+                ctx.match_batch_async(pairs, stream=hs)
+            if self.keep_outputs:  # stream-ordered snapshots of the results
+                for r in chosen:
+                    st = self.streams[r.stream]
+                    self._snap[(r.stream, r.seq)] = (
+                        st.feats[r.seq % 2].clone(),
+                        st.match.clone() if r.seq > 0 else None)
+            ev = torch.cuda.Event()
+            ev.record(hs)
+        for r in chosen:
+            self.streams[r.stream].last_event = ev
+        self.busy[slot] = (ev, chosen, nbytes)
+        self.in_flight_bytes += nbytes
+        self.max_batch_seen = max(self.max_batch_seen, len(chosen))
+        self.batches += 1
+        return True
+
+
+def poisson_arrivals(rates: Sequence[float], duration: float, seed: int):
+    """Per-stream Poisson arrival times over [0, duration): list of
+    (t, stream, seq) sorted by t."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for s, lam in enumerate(rates):
+        t, k = 0.0, 0
+        while True:
+            t += rng.exponential(1.0 / lam)
+            if t >= duration:
+                break
+            out.append((t, s, k))
+            k += 1
+    out.sort()
+    return out
+
+
+def run_schedule(server: Server, arrivals, frame_pool, time_scale: float = 1.0,
+                 spin_s: float = 20e-6):
+    """Replays arrivals in real time (scaled by time_scale) through server.
+    frame_pool[stream] is a list of pinned host frames, used cyclically.
+    Returns (wall seconds, completed list)."""
+    t0 = time.perf_counter()
+    i = 0
+    n = len(arrivals)
+    while i < n or not server.idle():
+        now = time.perf_counter() - t0
+        while i < n and arrivals[i][0] * time_scale <= now:
+            ta, s, k = arrivals[i]
+            pool = frame_pool[s]
+            server.submit(s, k, pool[k % len(pool)], t0 + ta * time_scale)
+            i += 1
+        server.poll()
+        launched = server.launch()
+        if not launched and (i < n or not server.idle()):
+            time.sleep(spin_s)
+    wall = time.perf_counter() - t0
+    return wall, server.completed
+
+
+def latency_stats(completed: List[Completed]):
+    lat = np.array([c.t_done - c.t_arrival for c in completed]) * 1e3
+    if len(lat) == 0:
+        return {}
+    return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "p999_ms": float(np.percentile(lat, 99.9)), "max_ms": float(lat.max()),
+            "mean_ms": float(lat.mean()), "frames": int(len(lat))}
