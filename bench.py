@@ -49,6 +49,8 @@ def kernel_bytes(dims, cap, counts):
         "k_compact": 8 * words + 4 * kept + 8 * LEVELS,
         # levels in (u8 level 0, f32 others), keypoints in, feature array out
         "k_describe": frame + 4 * pyr + 4 * kept + feat_array,
+        # two feature arrays' positions in, two cell indices out
+        "k_match_index": 2 * 8 * kept + 2 * 4 * LEVELS * (577 + cap),
         # two feature arrays in, one int32 per slot out
         "k_match_dir": 2 * feat_array + 4 * LEVELS * cap,
         "k_match_mutual": 2 * feat_array + 8 * LEVELS * cap,
@@ -289,7 +291,8 @@ def main():
     for _ in range(max(2, args.steps // 4)):
         step()
     torch.cuda.synchronize()
-    names = ["k_detect", "k_compact", "k_describe", "k_match_dir", "k_match_mutual"]
+    names = ["k_detect", "k_compact", "k_describe", "k_match_index", "k_match_dir",
+             "k_match_mutual"]
     times = {n: ctx.timing_read(n) for n in names}
     ctx.timing(False)
     counts = [c for c, _ in orbfe.parse_feature_array(feats[0].cpu().numpy(), LEVELS, CAP)]

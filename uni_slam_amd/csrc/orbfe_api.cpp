@@ -25,6 +25,8 @@ hipError_t launch_compact(uint32_t blocks, BatchView bv, const Job& job0, uint32
                           hipStream_t s);
 hipError_t launch_describe(uint32_t blocks, BatchView bv, const Job& job0, const Params& prm,
                            hipStream_t s);
+hipError_t launch_match_index(uint32_t blocks, MatchBatchView bv, const MatchJob& job0,
+                              hipStream_t s);
 hipError_t launch_match_dir(bool mutual, uint32_t blocks, MatchBatchView bv,
                             const MatchJob& job0, float ratio, hipStream_t s);
 uint32_t bf_queries_per_block();
@@ -114,7 +116,7 @@ struct PlanEntry {
     uint64_t* mask = nullptr;
     // per-frame scratch sizes (bytes), 256-aligned
     size_t surv_bytes = 0, kp_bytes = 0, pyr_bytes = 0, feat_bytes = 0, match_bytes = 0;
-    size_t sm_bytes = 0;
+    size_t sm_bytes = 0, index_bytes = 0;
 };
 
 struct Event2 {
@@ -146,6 +148,7 @@ struct orbfe_ctx {
     float* pyr_scratch = nullptr;
     float* smooth = nullptr;
     int32_t* b_to_a = nullptr;
+    uint32_t* index[2] = {nullptr, nullptr};
     // host-memory entry points: device staging
     uint8_t* d_img = nullptr;
     float* d_pyr = nullptr;
@@ -254,6 +257,9 @@ int get_plan(orbfe_ctx* ctx, uint32_t W0, uint32_t H0, uint32_t cap, PlanEntry**
     P.query_groups = ceil_div(cap, orbfe::kQueriesPerBlock);
     P.max_dx = 0.1f * static_cast<float>(W0);
     P.max_dy = 0.1f * static_cast<float>(H0);
+    P.cell_inv_x = 2.0f / P.max_dx;
+    P.cell_inv_y = 2.0f / P.max_dy;
+    P.idx_stride = orbfe::kCells + 1 + cap;
     uint32_t tiles = 0, words = 0, pyr = 0, sm = 0;
     float scale = 1.0f;
     for (uint32_t l = 0; l < L; ++l) {
@@ -299,6 +305,7 @@ int get_plan(orbfe_ctx* ctx, uint32_t W0, uint32_t H0, uint32_t cap, PlanEntry**
     pe->feat_bytes = align_up(static_cast<size_t>(L) * P.feat_stride);
     pe->match_bytes = align_up(static_cast<size_t>(L) * cap * 4);
     pe->sm_bytes = align_up(static_cast<size_t>(sm) * 4);
+    pe->index_bytes = align_up(static_cast<size_t>(L) * P.idx_stride * 4);
     *out = pe.get();
     ctx->plans.emplace(key, std::move(pe));
     return ORBFE_OK;
@@ -344,6 +351,10 @@ int launch_detect_stage(orbfe_ctx* ctx, const BatchView& bv, const Job& job0, ui
 int launch_match_stage(orbfe_ctx* ctx, const MatchBatchView& bv, const MatchJob& job0,
                        uint32_t blocks, hipStream_t s) {
     const float ratio = ctx->prm.ratio;
+    const uint32_t pairs = bv.n ? bv.n : 1;
+    HIP_TRY(ctx, timed(ctx, "k_match_index", s, [&] {
+                return orbfe::launch_match_index(pairs * 2 * ctx->cfg.levels, bv, job0, s);
+            }));
     HIP_TRY(ctx, timed(ctx, "k_match_dir", s, [&] {
                 return orbfe::launch_match_dir(false, blocks, bv, job0, ratio, s);
             }));
@@ -444,6 +455,10 @@ int layout_match(orbfe_ctx* ctx, const orbfe_match_pair* pairs, uint32_t n, Matc
         M.prev_match = p.prev_match;
         M.b_to_a = reinterpret_cast<int32_t*>(arena + off);
         off += pe->match_bytes;
+        M.index_prev = reinterpret_cast<uint32_t*>(arena + off);
+        off += pe->index_bytes;
+        M.index_curr = reinterpret_cast<uint32_t*>(arena + off);
+        off += pe->index_bytes;
         lay.prefix[i + 1] = lay.prefix[i] + pe->host.L * pe->host.query_groups;
     }
     lay.arena_bytes = off;
@@ -550,6 +565,8 @@ void orbfe_destroy(orbfe_ctx* ctx) {
     dfree(ctx->pyr_scratch);
     dfree(ctx->smooth);
     dfree(ctx->b_to_a);
+    dfree(ctx->index[0]);
+    dfree(ctx->index[1]);
     dfree(ctx->d_img);
     dfree(ctx->d_pyr);
     dfree(ctx->d_feat[0]);
@@ -602,7 +619,8 @@ int orbfe_create(const orbfe_config* cfg, int device, orbfe_ctx** out) {
     c->main = pe;
     if (dalloc(c, &c->surv, pe->surv_bytes) || dalloc(c, &c->kp, pe->kp_bytes) ||
         dalloc(c, &c->pyr_scratch, pe->pyr_bytes) || dalloc(c, &c->smooth, pe->sm_bytes) ||
-        dalloc(c, &c->b_to_a, pe->match_bytes) ||
+        dalloc(c, &c->b_to_a, pe->match_bytes) || dalloc(c, &c->index[0], pe->index_bytes) ||
+        dalloc(c, &c->index[1], pe->index_bytes) ||
         dalloc(c, &c->d_img, align_up(static_cast<size_t>(cfg->frame_width) * cfg->frame_height)) ||
         dalloc(c, &c->d_pyr, pe->pyr_bytes) || dalloc(c, &c->d_feat[0], pe->feat_bytes) ||
         dalloc(c, &c->d_feat[1], pe->feat_bytes) || dalloc(c, &c->d_match, pe->match_bytes)) {
@@ -786,6 +804,8 @@ int orbfe_match_pipeline_async(orbfe_ctx* ctx, const void* prev_features,
     job0.curr = static_cast<const uint8_t*>(curr_features);
     job0.prev_match = prev_match;
     job0.b_to_a = ctx->b_to_a;
+    job0.index_prev = ctx->index[0];
+    job0.index_curr = ctx->index[1];
     const MatchBatchView bv{nullptr, nullptr, 0};
     return launch_match_stage(ctx, bv, job0, P.L * P.query_groups, stream);
 }

@@ -34,6 +34,12 @@ constexpr int kDetectThreads = 256;
 constexpr uint32_t kDescPerBlock = 16;
 // k_match_dir: queries per workgroup (one per wave).
 constexpr uint32_t kQueriesPerBlock = 4;
+// Matcher cell index: LOD0 plane cut into kCellsPerAxis^2 cells of half the
+// search window (0.05*W0 x 0.05*H0); 24 cells cover 1.2x the frame.
+constexpr uint32_t kCellsPerAxis = 24;
+constexpr uint32_t kCells = kCellsPerAxis * kCellsPerAxis;
+// Per-wave LDS words of the matcher (range offsets and sources, 64 each).
+constexpr uint32_t kCandCap = 128;
 
 // Key packing of the pipeline matcher: distance (<= 256) above a 22-bit
 // candidate id (2 bits pool level slot, 20 bits index).
@@ -72,6 +78,8 @@ struct Plan {
     uint32_t query_groups; // matcher blocks per level = ceil(cap / kQueriesPerBlock)
     float max_dx, max_dy; // 0.1f*W0, 0.1f*H0 (match_features_cpu.cpp:76-77)
     uint32_t sm_floats;   // smoothed-image floats of one frame
+    float cell_inv_x, cell_inv_y;  // 1 / cell size (cells of max_dx/2 x max_dy/2)
+    uint32_t idx_stride;  // u32 per level of a cell index: kCells + 1 + cap
     const uint64_t* mask; // static cull mask, `words` words
     LevelGeom lv[kMaxLevels];
 };
@@ -93,6 +101,8 @@ struct MatchJob {
     const uint8_t* curr;
     int32_t* prev_match;  // L*cap
     int32_t* b_to_a;      // scratch L*cap
+    uint32_t* index_prev; // scratch: cell index of prev, L * plan->idx_stride
+    uint32_t* index_curr; // scratch: cell index of curr
 };
 
 struct Params {

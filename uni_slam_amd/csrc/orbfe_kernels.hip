@@ -34,6 +34,13 @@ constexpr int8_t c_circle_y[16] = {-3, -3, -2, -1, 0, 1, 2, 3, 3, 3, 2, 1, 0, -1
 __device__ inline uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
 __device__ inline uint32_t umax(uint32_t a, uint32_t b) { return a > b ? a : b; }
 
+// Orders the LDS accesses of the lanes of one wave around this point.
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ inline uint32_t find_job(const uint32_t* prefix, uint32_t n, uint32_t b) {
     // largest j with prefix[j] <= b (prefix[0] = 0, prefix[n] = total)
     uint32_t lo = 0, hi = n;
@@ -99,15 +106,51 @@ __device__ uint32_t fast_score_from_frame(const Job& J, const Plan& P, const Lev
 // ---------------------------------------------------------------------------
 // k_detect
 // ---------------------------------------------------------------------------
+// LDS bytes for the u8 source region of a level >= 1 tile: (70*s + 3) x
+// (22*s + 3) for level scale s, 10.4 KB at s = 1.2^5.  Larger regions (more
+// levels) fall back to per-texel gathers.
+constexpr uint32_t kSrcWords = 3072;
+
+// Smoothed value at (x, y) evaluated directly from the frame (tiles whose
+// sampler taps fall outside the staged region: only at the far corner of
+// the extended grid).
+__device__ float smooth_direct(const Job& J, const Plan& P, const LevelGeom& G, uint32_t l,
+                               const float* lut, uint32_t x, uint32_t y) {
+    const float offs[3] = {-1.2f, 0.0f, 1.2f};
+    const float wts[3] = {0.3125f, 0.375f, 0.3125f};
+    const float texel_x = 1.0f / static_cast<float>(G.w);
+    const float texel_y = 1.0f / static_cast<float>(G.h);
+    const float uvx = (static_cast<float>(x) + 0.5f) * texel_x;
+    const float uvy = (static_cast<float>(y) + 0.5f) * texel_y;
+    float sum = 0.0f;
+    for (int jj = 0; jj < 3; ++jj) {
+        const AxisTaps ay = axis_taps(uvy + offs[jj] * texel_y, G.h);
+        for (int i = 0; i < 3; ++i) {
+            const AxisTaps ax = axis_taps(uvx + offs[i] * texel_x, G.w);
+            const float sv = lerp2(level_texel_from_frame(J, P, G, l, lut, ax.i0, ay.i0),
+                                   level_texel_from_frame(J, P, G, l, lut, ax.i1, ay.i0),
+                                   level_texel_from_frame(J, P, G, l, lut, ax.i0, ay.i1),
+                                   level_texel_from_frame(J, P, G, l, lut, ax.i1, ay.i1),
+                                   ax.frac, ay.frac);
+            sum = sum + sv * wts[i] * wts[jj];
+        }
+    }
+    return sum;
+}
+
 __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job0,
                                                             Params prm) {
     __shared__ float lut[256];
     __shared__ float tile[kLdsH][kLdsW];
+    __shared__ float hbuf[3][kLdsH][kTileW];  // horizontal lerps of the 9-tap Gaussian
+    __shared__ uint32_t src[kSrcWords];       // u8 source region (levels >= 1)
     __shared__ uint32_t col_i0[kLdsW], col_i1[kLdsW];
     __shared__ float col_a[kLdsW];
     __shared__ uint32_t row_i0[kLdsH], row_i1[kLdsH];
     __shared__ float row_b[kLdsH];
     __shared__ uint32_t col_ok[kLdsW], row_ok[kLdsH];
+    __shared__ uint32_t srow_i0[3][kTileH], srow_i1[3][kTileH];
+    __shared__ float srow_b[3][kTileH];
     __shared__ uint32_t row0_score[kTileW + 4];
 
     const uint32_t tid = threadIdx.x;
@@ -128,6 +171,11 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     const int32_t x0 = static_cast<int32_t>(tile_x) * kTileW;
     const int32_t y0 = static_cast<int32_t>(tile_y) * kTileH;
     const uint32_t w = G.w, h = G.h;
+    const uint32_t sw = w + kSmoothExt, sh = h + kSmoothExt;
+    const float texel_x = 1.0f / static_cast<float>(w);
+    const float texel_y = 1.0f / static_cast<float>(h);
+    const float offs[3] = {-1.2f, 0.0f, 1.2f};
+    const float wts[3] = {0.3125f, 0.375f, 0.3125f};
 
     lut[tid] = static_cast<float>(tid) / 255.0f;
 
@@ -155,18 +203,63 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
         } else {
             row_i0[c] = i0; row_i1[c] = i1; row_b[c] = frac; row_ok[c] = ok;
         }
+    } else if (tid < kLdsW + kLdsH + 3 * kTileH) {
+        // Row taps of the 9-tap Gaussian for the tile's 16 output rows.
+        const uint32_t e = tid - (kLdsW + kLdsH);
+        const uint32_t jj = e / kTileH, rr = e - jj * kTileH;
+        const float uvy = (static_cast<float>(static_cast<uint32_t>(y0) + rr) + 0.5f) * texel_y;
+        const AxisTaps ay = axis_taps(uvy + offs[jj] * texel_y, h);
+        srow_i0[jj][rr] = ay.i0;
+        srow_i1[jj][rr] = ay.i1;
+        srow_b[jj][rr] = ay.frac;
     }
     __syncthreads();
+
+    // Staged columns/rows with g >= 0 map monotonically onto the frame, so
+    // their source texels form one rectangle; negative g (Tint sends those
+    // to the far edge) and levels whose rectangle does not fit use gathers.
+    const uint32_t c_first = x0 >= kHalo ? 0u : static_cast<uint32_t>(kHalo - x0);
+    const uint32_t r_first = y0 >= kHalo ? 0u : static_cast<uint32_t>(kHalo - y0);
+    uint32_t c_last = kLdsW - 1, r_last = kLdsH - 1;
+    if (prm.border_zero) {
+        const int32_t cl_ = static_cast<int32_t>(w) - 1 - x0 + kHalo;
+        const int32_t rl_ = static_cast<int32_t>(h) - 1 - y0 + kHalo;
+        c_last = cl_ < kLdsW - 1 ? static_cast<uint32_t>(cl_ < 0 ? 0 : cl_) : kLdsW - 1;
+        r_last = rl_ < kLdsH - 1 ? static_cast<uint32_t>(rl_ < 0 ? 0 : rl_) : kLdsH - 1;
+    }
+    const uint32_t colmin = col_i0[c_first], colmax = col_i1[c_last];
+    const uint32_t rowmin = row_i0[r_first], rowmax = row_i1[r_last];
+    const uint32_t cs = colmin & ~3u;
+    const uint32_t nd = (colmax - cs) / 4 + 1;  // dwords per source row
+    const uint32_t nrows = rowmax - rowmin + 1;
+    const bool use_src = l > 0 && colmax >= colmin && rowmax >= rowmin &&
+                         (J.pitch & 3u) == 0 && (reinterpret_cast<uintptr_t>(J.img) & 3u) == 0 &&
+                         nd * nrows <= kSrcWords;
+    if (use_src) {
+        for (uint32_t e = tid; e < nd * nrows; e += kDetectThreads) {
+            const uint32_t rr = e / nd;
+            const uint32_t d = e - rr * nd;
+            src[e] = reinterpret_cast<const uint32_t*>(J.img + (rowmin + rr) * J.pitch)[cs / 4 + d];
+        }
+        __syncthreads();
+    }
+    const uint8_t* srcb = reinterpret_cast<const uint8_t*>(src);
+    const uint32_t spitch = nd * 4;
 
     for (uint32_t e = tid; e < kLdsW * kLdsH; e += kDetectThreads) {
         const uint32_t r = e / kLdsW;
         const uint32_t c = e - r * kLdsW;
         float v = 0.0f;
         if (col_ok[c] & row_ok[r]) {
-            const uint8_t* s0 = J.img + row_i0[r] * J.pitch;
             if (l == 0) {
-                v = lut[s0[col_i0[c]]];
+                v = lut[J.img[row_i0[r] * J.pitch + col_i0[c]]];
+            } else if (use_src && c >= c_first && r >= r_first && c <= c_last && r <= r_last) {
+                const uint8_t* s0 = srcb + (row_i0[r] - rowmin) * spitch - cs;
+                const uint8_t* s1 = srcb + (row_i1[r] - rowmin) * spitch - cs;
+                v = lerp2(lut[s0[col_i0[c]]], lut[s0[col_i1[c]]], lut[s1[col_i0[c]]],
+                          lut[s1[col_i1[c]]], col_a[c], row_b[r]);
             } else {
+                const uint8_t* s0 = J.img + row_i0[r] * J.pitch;
                 const uint8_t* s1 = J.img + row_i1[r] * J.pitch;
                 v = lerp2(lut[s0[col_i0[c]]], lut[s0[col_i1[c]]], lut[s1[col_i0[c]]],
                           lut[s1[col_i1[c]]], col_a[c], row_b[r]);
@@ -181,45 +274,51 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     const uint32_t x = static_cast<uint32_t>(x0) + lane;
 
     // Smoothed image over the extended grid (generate_features.wgsl:149-172
-    // evaluated at every integer point).  Sampler taps are clamped to the
-    // level, so they read true texels: from the staged tile when they fall
-    // inside it, else (only near the extension) from the frame.
-    {
-        const float offs[3] = {-1.2f, 0.0f, 1.2f};
-        const float wts[3] = {0.3125f, 0.375f, 0.3125f};
-        const float texel_x = 1.0f / static_cast<float>(w);
-        const float texel_y = 1.0f / static_cast<float>(h);
-        const uint32_t sw = w + kSmoothExt, sh = h + kSmoothExt;
-        auto texel = [&](uint32_t cx, uint32_t cy) -> float {
-            const int32_t lx = static_cast<int32_t>(cx) - x0 + kHalo;
-            const int32_t ly = static_cast<int32_t>(cy) - y0 + kHalo;
-            if (lx >= 0 && lx < kLdsW && ly >= 0 && ly < kLdsH) return tile[ly][lx];
-            return level_texel_from_frame(J, P, G, l, lut, cx, cy);
-        };
+    // at every integer point), separably: the horizontal lerp of each
+    // (column tap, staged row) once, then 3x3 vertical lerps and weights per
+    // point, in the shader's order.  Sampler taps are clamped into the
+    // level, so on all but the far-corner tiles they lie in the staged tile.
+    const bool slow = x0 > static_cast<int32_t>(w) + 2 || y0 > static_cast<int32_t>(h) + 2;
+    if (!slow) {
         if (x < sw) {
             const float uvx = (static_cast<float>(x) + 0.5f) * texel_x;
-            AxisTaps ax[3];
 #pragma unroll
-            for (int i = 0; i < 3; ++i) ax[i] = axis_taps(uvx + offs[i] * texel_x, w);
+            for (int i = 0; i < 3; ++i) {
+                const AxisTaps ax = axis_taps(uvx + offs[i] * texel_x, w);
+                const uint32_t c0 = ax.i0 - x0 + kHalo, c1 = ax.i1 - x0 + kHalo;
+                for (uint32_t r = wave; r < kLdsH; r += kDetectThreads / 64) {
+                    const float t0 = tile[r][c0];
+                    hbuf[i][r][lane] = t0 + ax.frac * (tile[r][c1] - t0);
+                }
+            }
+        }
+        __syncthreads();
+        if (x < sw) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const uint32_t y = static_cast<uint32_t>(y0) + wave * 4 + k;
-                if (y >= sh) break;
-                const float uvy = (static_cast<float>(y) + 0.5f) * texel_y;
+                const uint32_t rr = wave * 4 + k;
+                const uint32_t y = static_cast<uint32_t>(y0) + rr;
+                if (y >= sh) continue;
                 float sum = 0.0f;
 #pragma unroll
                 for (int jj = 0; jj < 3; ++jj) {
-                    const AxisTaps ay = axis_taps(uvy + offs[jj] * texel_y, h);
+                    const uint32_t ra = srow_i0[jj][rr] - y0 + kHalo;
+                    const uint32_t rb = srow_i1[jj][rr] - y0 + kHalo;
+                    const float bb = srow_b[jj][rr];
 #pragma unroll
                     for (int i = 0; i < 3; ++i) {
-                        const float sv = lerp2(texel(ax[i].i0, ay.i0), texel(ax[i].i1, ay.i0),
-                                               texel(ax[i].i0, ay.i1), texel(ax[i].i1, ay.i1),
-                                               ax[i].frac, ay.frac);
+                        const float top = hbuf[i][ra][lane];
+                        const float sv = top + bb * (hbuf[i][rb][lane] - top);
                         sum = sum + sv * wts[i] * wts[jj];
                     }
                 }
                 J.smooth[G.sm_off + y * sw + x] = sum;
             }
+        }
+    } else if (x < sw) {
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t y = static_cast<uint32_t>(y0) + wave * 4 + k;
+            if (y < sh) J.smooth[G.sm_off + y * sw + x] = smooth_direct(J, P, G, l, lut, x, y);
         }
     }
 
@@ -565,13 +664,81 @@ struct FeatView {
 
 constexpr uint32_t kMatchThreads = 256;  // 4 waves = 4 queries
 
+// LOD0 position of a record (toLod0, match_features_cpu.cpp:58-64).
+__device__ inline void rec_lod0(const uint32_t* r, float scale, float* x, float* y) {
+    *x = static_cast<float>(static_cast<uint32_t>(__uint_as_float(r[0]))) * scale;
+    *y = static_cast<float>(static_cast<uint32_t>(__uint_as_float(r[1]))) * scale;
+}
+
+__device__ inline uint32_t cell_of(float v, float inv) {
+    const float c = __builtin_floorf((v > 0.0f ? v : 0.0f) * inv);
+    return c < static_cast<float>(kCellsPerAxis - 1) ? static_cast<uint32_t>(c)
+                                                     : kCellsPerAxis - 1;
+}
+
+// k_match_index: per (pair, side, level) a counting sort of the level's
+// records by LOD0 cell: cell_start[kCells + 1] then record indices.  Cell
+// order is monotone in the coordinates, so every record inside a search
+// window lies in the cells the window's (margin-widened) corners map to.
+__global__ __launch_bounds__(256) void k_match_index(MatchBatchView bv, MatchJob job0) {
+    __shared__ uint32_t hist[kCells + 1];
+    const uint32_t tid = threadIdx.x;
+    const MatchJob M = bv.n ? bv.jobs[0] : job0;  // layout constants only
+    const uint32_t L = M.plan->L;
+    const uint32_t j = blockIdx.x / (2 * L);
+    const uint32_t side = (blockIdx.x / L) & 1;
+    const uint32_t l = blockIdx.x % L;
+    const MatchJob J = bv.n ? bv.jobs[j] : job0;
+    const Plan& P = *J.plan;
+    const FeatView A{side ? J.curr : J.prev, P.feat_stride};
+    uint32_t* out = (side ? J.index_curr : J.index_prev) + l * P.idx_stride;
+    const uint32_t n = A.count(l, P.cap);
+    const float sc = P.lv[l].lod0_scale;
+    for (uint32_t c = tid; c <= kCells; c += 256) hist[c] = 0;
+    __syncthreads();
+    for (uint32_t i = tid; i < n; i += 256) {
+        float x, y;
+        rec_lod0(A.rec(l, i), sc, &x, &y);
+        atomicAdd(&hist[cell_of(y, P.cell_inv_y) * kCellsPerAxis + cell_of(x, P.cell_inv_x)], 1u);
+    }
+    __syncthreads();
+    if (tid < 64) {  // exclusive scan of kCells counts by one wave, 9 per lane
+        constexpr uint32_t kPer = kCells / 64;
+        uint32_t v[kPer];
+        uint32_t sum = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) {
+            v[k] = hist[tid * kPer + k];
+            sum += v[k];
+        }
+        const uint32_t incl = wave_inclusive_sum(sum, tid);
+        uint32_t run = incl - sum;
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) {
+            hist[tid * kPer + k] = run;
+            out[tid * kPer + k] = run;
+            run += v[k];
+        }
+        if (tid == 63) out[kCells] = run;
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < n; i += 256) {
+        float x, y;
+        rec_lod0(A.rec(l, i), sc, &x, &y);
+        const uint32_t c = cell_of(y, P.cell_inv_y) * kCellsPerAxis + cell_of(x, P.cell_inv_x);
+        out[kCells + 1 + atomicAdd(&hist[c], 1u)] = i;
+    }
+}
+
 // kMutual = false: queries = curr, pool from prev -> b_to_a.
 // kMutual = true:  queries = prev, pool from curr, mutual check -> prev_match.
 template <bool kMutual>
 __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, MatchJob job0,
                                                               float ratio) {
+    __shared__ uint32_t cand[kMatchThreads / 64][kCandCap];
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63;
+    const uint32_t wave = tid >> 6;
     uint32_t b = blockIdx.x;
     uint32_t j = 0;
     if (bv.n) {
@@ -582,10 +749,11 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
     const Plan& P = *M.plan;
     const uint32_t l = b / P.query_groups;
     const uint32_t g = b - l * P.query_groups;
-    const uint32_t slot = g * kQueriesPerBlock + (tid >> 6);
+    const uint32_t slot = g * kQueriesPerBlock + wave;
     if (slot >= P.cap) return;
     const FeatView Q{kMutual ? M.prev : M.curr, P.feat_stride};
     const FeatView C{kMutual ? M.curr : M.prev, P.feat_stride};
+    const uint32_t* cidx = kMutual ? M.index_curr : M.index_prev;
     int32_t* out = kMutual ? M.prev_match : M.b_to_a;
     const uint32_t nq = Q.count(l, P.cap);
     if (slot >= nq) {
@@ -603,82 +771,86 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
     int32_t result = -1;
     if (total >= 2) {
         const uint32_t* qr = Q.rec(l, slot);
-        const float qs = P.lv[l].lod0_scale;
-        const float qx = static_cast<float>(static_cast<uint32_t>(__uint_as_float(qr[0]))) * qs;
-        const float qy = static_cast<float>(static_cast<uint32_t>(__uint_as_float(qr[1]))) * qs;
+        float qx, qy;
+        rec_lod0(qr, P.lv[l].lod0_scale, &qx, &qy);
         uint32_t qd[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) qd[k] = qr[4 + k];
         const float lo_x = qx - P.max_dx;
         const float hi_x = qx + P.max_dx;
         const float dy = P.max_dy;
-
-        // y windows |qy - cy| <= dy of the pool levels are contiguous ranges
-        // (records are in raster order).  Six searches run at once: lane
-        // group q = lane / 10 (q < 6) looks for bound q & 1 of pool slot
-        // q >> 1 with 10 probes per round.
-        const uint32_t grp = lane / 10;
-        const uint32_t gl = lane - grp * 10;
-        const uint32_t gs = grp >> 1;
-        const bool gvalid = grp < 6 && gs < npool;
-        const uint32_t gcl = gs == 0 ? pl[0] : (gs == 1 ? pl[1] : pl[2]);
-        const uint32_t gn = gvalid ? (gs == 0 ? pn[0] : (gs == 1 ? pn[1] : pn[2])) : 0u;
-        const float gsc = P.lv[gvalid ? gcl : 0].lod0_scale;
-        uint32_t glo = 0, ghi = gn;
-        while (__ballot(glo < ghi) != 0) {
-            bool p = true;
-            uint32_t stride = 1;
-            if (glo < ghi) {
-                stride = (ghi - glo + 9) / 10;
-                const uint32_t pos = glo + gl * stride;
-                if (pos < ghi) {
-                    const float cy = static_cast<float>(static_cast<uint32_t>(
-                                         __uint_as_float(C.rec(gcl, pos)[1]))) * gsc;
-                    const float d = qy - cy;
-                    p = (grp & 1) ? (d < -dy) : (d <= dy);
-                }
-            }
-            const uint32_t m = grp < 6 ? static_cast<uint32_t>(__ballot(p) >> (grp * 10)) & 0x3FFu
-                                       : 0x3FFu;
-            if (glo < ghi) {
-                if (m == 0) {
-                    glo = glo + 9 * stride + 1;
-                } else {
-                    const uint32_t f = __ffs(m) - 1;
-                    if (f == 0) {
-                        ghi = glo;
-                    } else {
-                        const uint32_t cand = glo + f * stride;
-                        glo = glo + (f - 1) * stride + 1;
-                        ghi = cand < ghi ? cand : ghi;
-                    }
-                }
-            }
-        }
-        uint32_t wlo[3], whi[3];
-#pragma unroll
-        for (uint32_t s = 0; s < 3; ++s) {
-            wlo[s] = __shfl(glo, static_cast<int>(20 * s));
-            whi[s] = __shfl(glo, static_cast<int>(20 * s + 10));
-        }
+        // Cells of the window widened by one pixel (covers the rounding of
+        // the exact gates below).
+        const uint32_t cx0 = cell_of(lo_x - 1.0f, P.cell_inv_x);
+        const uint32_t cx1 = cell_of(hi_x + 1.0f, P.cell_inv_x);
+        const uint32_t cy0 = cell_of(qy - dy - 1.0f, P.cell_inv_y);
+        const uint32_t cy1 = cell_of(qy + dy + 1.0f, P.cell_inv_y);
 
         uint32_t min1 = 0xFFFFFFFFu, min2 = 0xFFFFFFFFu;
-        for (uint32_t s = 0; s < npool; ++s) {
-            const uint32_t cl = pl[s];
-            const float cs = P.lv[cl].lod0_scale;
-            const uint32_t lo = wlo[s], hi = whi[s];
-            for (uint32_t i = lo + lane; i < hi; i += 64) {
-                const uint32_t* cr = C.rec(cl, i);
-                const float cx = static_cast<float>(static_cast<uint32_t>(__uint_as_float(cr[0]))) * cs;
-                const float cy = static_cast<float>(static_cast<uint32_t>(__uint_as_float(cr[1]))) * cs;
-                if (cx < lo_x || cx > hi_x || __builtin_fabsf(qy - cy) > dy) continue;
-                uint32_t d = 0;
-#pragma unroll
-                for (int k = 0; k < 8; ++k) d += __popc(qd[k] ^ cr[4 + k]);
-                const uint32_t key = (d << kMatchKeyShift) | (s << kMatchIdxBits) | i;
-                min2 = umax(min1, umin(min2, key));
-                min1 = umin(min1, key);
+        // Candidate ranges: for each pool slot s and cell row, the cells
+        // cx0..cx1 are contiguous in the index.  Lane k of a round fetches
+        // the bounds of range k; a wave scan of the lengths flattens them.
+        const uint32_t rows = cy1 - cy0 + 1;
+        const uint32_t nranges = npool * rows;
+        uint32_t* r_off = cand[wave];                 // [64] flattened offsets
+        uint32_t* r_src = cand[wave] + 64;            // [64] index position | slot
+        for (uint32_t r0 = 0; r0 < nranges; r0 += 64) {
+            const uint32_t k = r0 + lane;
+            uint32_t a = 0, len = 0, sk = 0;
+            if (k < nranges) {
+                sk = k / rows;
+                const uint32_t row = cy0 + (k - sk * rows);
+                const uint32_t* base = cidx + (sk == 0 ? pl[0] : (sk == 1 ? pl[1] : pl[2])) * P.idx_stride;
+                a = base[row * kCellsPerAxis + cx0];
+                len = base[row * kCellsPerAxis + cx1 + 1] - a;
             }
+            const uint32_t incl = wave_inclusive_sum(len, lane);
+            const uint32_t total_c = __shfl(incl, 63);
+            const uint32_t nr = umin(64u, nranges - r0);
+            wave_sync();
+            r_off[lane] = incl - len;
+            r_src[lane] = a | (sk << 30);
+            wave_sync();
+            for (uint32_t t0 = 0; t0 < total_c; t0 += 256) {
+                uint32_t ent[4];
+                const uint32_t* crs[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t t = t0 + lane + 64 * u;
+                    ent[u] = 0xFFFFFFFFu;
+                    if (t < total_c) {
+                        // last range with offset <= t
+                        uint32_t lo = 0, hi = nr;
+                        while (hi - lo > 1) {
+                            const uint32_t mid = (lo + hi) >> 1;
+                            if (r_off[mid] <= t) lo = mid; else hi = mid;
+                        }
+                        const uint32_t src = r_src[lo];
+                        const uint32_t s2 = src >> 30;
+                        const uint32_t cl = s2 == 0 ? pl[0] : (s2 == 1 ? pl[1] : pl[2]);
+                        const uint32_t i = cidx[cl * P.idx_stride + kCells + 1 +
+                                                (src & 0x3FFFFFFFu) + (t - r_off[lo])];
+                        ent[u] = (s2 << kMatchIdxBits) | i;
+                        crs[u] = C.rec(cl, i);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (ent[u] == 0xFFFFFFFFu) continue;
+                    const uint32_t s2 = ent[u] >> kMatchIdxBits;
+                    const uint32_t cl = s2 == 0 ? pl[0] : (s2 == 1 ? pl[1] : pl[2]);
+                    float cx, cy;
+                    rec_lod0(crs[u], P.lv[cl].lod0_scale, &cx, &cy);
+                    if (cx < lo_x || cx > hi_x || __builtin_fabsf(qy - cy) > dy) continue;
+                    uint32_t d = 0;
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) d += __popc(qd[q] ^ crs[u][4 + q]);
+                    const uint32_t key = (d << kMatchKeyShift) | ent[u];
+                    min2 = umax(min1, umin(min2, key));
+                    min1 = umin(min1, key);
+                }
+            }
+            wave_sync();
         }
         for (int off = 32; off > 0; off >>= 1) {
             const uint32_t o1 = __shfl_xor(min1, off);
@@ -694,7 +866,7 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
             if (static_cast<float>(d1) < ratio * static_cast<float>(d2)) {
                 const uint32_t s = (min1 >> kMatchIdxBits) & 3u;
                 const uint32_t idx = min1 & ((1u << kMatchIdxBits) - 1u);
-                result = static_cast<int32_t>((pl[s] << 24) | idx);
+                result = static_cast<int32_t>(((s == 0 ? pl[0] : (s == 1 ? pl[1] : pl[2])) << 24) | idx);
             }
         }
         if (kMutual && result >= 0) {
@@ -814,6 +986,12 @@ hipError_t launch_compact(uint32_t blocks, BatchView bv, const Job& job0, uint32
 hipError_t launch_describe(uint32_t blocks, BatchView bv, const Job& job0, const Params& prm,
                            hipStream_t s) {
     hipLaunchKernelGGL(k_describe, dim3(blocks), dim3(kDescribeThreads), 0, s, bv, job0, prm);
+    return hipGetLastError();
+}
+
+hipError_t launch_match_index(uint32_t blocks, MatchBatchView bv, const MatchJob& job0,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_match_index, dim3(blocks), dim3(256), 0, s, bv, job0);
     return hipGetLastError();
 }
 
