@@ -282,6 +282,8 @@ int get_plan(orbfe_ctx* ctx, uint32_t W0, uint32_t H0, uint32_t cap, PlanEntry**
         G.sm_off = sm;
         sm += (G.w + orbfe::kSmoothExt) * (G.h + orbfe::kSmoothExt);
         G.lod0_scale = scale;  // toLod0 (match_features_cpu.cpp:58-64)
+        G.texel_x = 1.0f / static_cast<float>(G.w);
+        G.texel_y = 1.0f / static_cast<float>(G.h);
         scale *= static_cast<float>(1.2);
     }
     P.tiles = tiles;

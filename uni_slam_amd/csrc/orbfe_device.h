@@ -66,6 +66,7 @@ struct LevelGeom {
     uint32_t kp_off;     // entry offset of the level's keypoint list
     uint32_t sm_off;     // float offset of the level's smoothed image
     float lod0_scale;    // product of l factors 1.2f (toLod0)
+    float texel_x, texel_y;  // 1.0f / w, 1.0f / h (generate_features.wgsl:151)
 };
 
 struct Plan {

@@ -34,11 +34,13 @@ constexpr int8_t c_circle_y[16] = {-3, -3, -2, -1, 0, 1, 2, 3, 3, 3, 2, 1, 0, -1
 __device__ inline uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
 __device__ inline uint32_t umax(uint32_t a, uint32_t b) { return a > b ? a : b; }
 
-// Orders the LDS accesses of the lanes of one wave around this point.
+// Orders the LDS accesses of the lanes of one wave around this point.  The
+// LDS executes one wave's DS instructions in issue order, so only the
+// compiler must be kept from moving memory operations across; global
+// stores in flight are not waited for.
 __device__ inline void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    asm volatile("" ::: "memory");
 }
 
 __device__ inline uint32_t find_job(const uint32_t* prefix, uint32_t n, uint32_t b) {
@@ -152,6 +154,8 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     __shared__ uint32_t srow_i0[3][kTileH], srow_i1[3][kTileH];
     __shared__ float srow_b[3][kTileH];
     __shared__ uint32_t row0_score[kTileW + 4];
+    __shared__ uint8_t score_lds[kDetectThreads / 64][4 * 64];  // per wave: 4 rows x 64
+    __shared__ uint8_t list_lds[kDetectThreads / 64][4 * 64];
 
     const uint32_t tid = threadIdx.x;
     uint32_t b = blockIdx.x;
@@ -172,8 +176,8 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     const int32_t y0 = static_cast<int32_t>(tile_y) * kTileH;
     const uint32_t w = G.w, h = G.h;
     const uint32_t sw = w + kSmoothExt, sh = h + kSmoothExt;
-    const float texel_x = 1.0f / static_cast<float>(w);
-    const float texel_y = 1.0f / static_cast<float>(h);
+    const float texel_x = G.texel_x;
+    const float texel_y = G.texel_y;
     const float offs[3] = {-1.2f, 0.0f, 1.2f};
     const float wts[3] = {0.3125f, 0.375f, 0.3125f};
 
@@ -235,42 +239,46 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     const bool use_src = l > 0 && colmax >= colmin && rowmax >= rowmin &&
                          (J.pitch & 3u) == 0 && (reinterpret_cast<uintptr_t>(J.img) & 3u) == 0 &&
                          nd * nrows <= kSrcWords;
+    const uint32_t lane = tid & 63;
+    const uint32_t wave = tid >> 6;
+    constexpr uint32_t kWaves = kDetectThreads / 64;
     if (use_src) {
-        for (uint32_t e = tid; e < nd * nrows; e += kDetectThreads) {
-            const uint32_t rr = e / nd;
-            const uint32_t d = e - rr * nd;
-            src[e] = reinterpret_cast<const uint32_t*>(J.img + (rowmin + rr) * J.pitch)[cs / 4 + d];
+        for (uint32_t rr = wave; rr < nrows; rr += kWaves) {
+            const uint32_t* row = reinterpret_cast<const uint32_t*>(J.img + (rowmin + rr) * J.pitch) + cs / 4;
+            for (uint32_t d = lane; d < nd; d += 64) src[rr * nd + d] = row[d];
         }
         __syncthreads();
     }
     const uint8_t* srcb = reinterpret_cast<const uint8_t*>(src);
     const uint32_t spitch = nd * 4;
 
+    // Stage the level tile (22 x 70 texels, flat over the block).
     for (uint32_t e = tid; e < kLdsW * kLdsH; e += kDetectThreads) {
         const uint32_t r = e / kLdsW;
         const uint32_t c = e - r * kLdsW;
         float v = 0.0f;
         if (col_ok[c] & row_ok[r]) {
+            const uint32_t ci0 = col_i0[c];
             if (l == 0) {
-                v = lut[J.img[row_i0[r] * J.pitch + col_i0[c]]];
-            } else if (use_src && c >= c_first && r >= r_first && c <= c_last && r <= r_last) {
-                const uint8_t* s0 = srcb + (row_i0[r] - rowmin) * spitch - cs;
-                const uint8_t* s1 = srcb + (row_i1[r] - rowmin) * spitch - cs;
-                v = lerp2(lut[s0[col_i0[c]]], lut[s0[col_i1[c]]], lut[s1[col_i0[c]]],
-                          lut[s1[col_i1[c]]], col_a[c], row_b[r]);
+                v = lut[J.img[row_i0[r] * J.pitch + ci0]];
             } else {
-                const uint8_t* s0 = J.img + row_i0[r] * J.pitch;
-                const uint8_t* s1 = J.img + row_i1[r] * J.pitch;
-                v = lerp2(lut[s0[col_i0[c]]], lut[s0[col_i1[c]]], lut[s1[col_i0[c]]],
-                          lut[s1[col_i1[c]]], col_a[c], row_b[r]);
+                const uint32_t ci1 = col_i1[c];
+                const uint8_t* s0;
+                const uint8_t* s1;
+                if (use_src && c >= c_first && c <= c_last && r >= r_first && r <= r_last) {
+                    s0 = srcb + (row_i0[r] - rowmin) * spitch - cs;
+                    s1 = srcb + (row_i1[r] - rowmin) * spitch - cs;
+                } else {
+                    s0 = J.img + row_i0[r] * J.pitch;
+                    s1 = J.img + row_i1[r] * J.pitch;
+                }
+                v = lerp2(lut[s0[ci0]], lut[s0[ci1]], lut[s1[ci0]], lut[s1[ci1]], col_a[c], row_b[r]);
             }
         }
         tile[r][c] = v;
     }
     __syncthreads();
 
-    const uint32_t lane = tid & 63;
-    const uint32_t wave = tid >> 6;
     const uint32_t x = static_cast<uint32_t>(x0) + lane;
 
     // Smoothed image over the extended grid (generate_features.wgsl:149-172
@@ -322,27 +330,69 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
         }
     }
 
-    uint32_t score[4];
+    // Corner scores (detect_corners.wgsl:58-81) for the pixels the cull can
+    // keep: masked pixels of rows >= 1 and the 9-aligned windows of row 0.
+    // A run of >= 9 ring differences without wrap covers ring positions 7
+    // and 8, so for min_run >= 9 a pixel failing either of those two tests
+    // scores 0; the rest are gathered into a per-wave list and get the full
+    // ring.  Scores land in score_lds (0 elsewhere).
+    uint8_t* wscore = score_lds[wave];
+    uint8_t* wlist = list_lds[wave];
+    const bool quick = prm.min_run >= 9;
+    uint32_t nlist = 0;
+    uint64_t mwords[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t y = static_cast<uint32_t>(y0) + wave * 4 + k;
+        mwords[k] = (tile_x < G.wpr && y < h) ? P.mask[G.word_off + y * G.wpr + tile_x] : 0ull;
+    }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const uint32_t r = wave * 4 + k;
         const uint32_t y = static_cast<uint32_t>(y0) + r;
-        score[k] = 0;
+        wscore[k * 64 + lane] = 0;
+        bool need = false;
         if (x < w && y < h) {
             const float center = tile[r + kHalo][lane + kHalo];
             if (l > 0 && J.pyr) J.pyr[G.pyr_off + y * w + x] = center;
+            if (tile_x < G.wpr) need = y == 0 ? (x % 9) < 3 : ((mwords[k] >> lane) & 1ull) != 0;
+            if (need && quick) {
+                const float p7 = tile[r + kHalo + c_circle_y[7]][lane + kHalo + c_circle_x[7]];
+                const float p8 = tile[r + kHalo + c_circle_y[8]][lane + kHalo + c_circle_x[8]];
+                need = __builtin_fabsf(center - p7) > prm.threshold &&
+                       __builtin_fabsf(center - p8) > prm.threshold;
+            }
+        }
+        const uint64_t bits = __ballot(need);
+        if (need) {
+            const uint32_t pos = nlist + __popcll(bits & ((1ull << lane) - 1ull));
+            wlist[pos] = static_cast<uint8_t>((k << 6) | lane);
+        }
+        nlist += __popcll(bits);
+    }
+    wave_sync();
+    for (uint32_t e0 = 0; e0 < nlist; e0 += 64) {
+        const uint32_t e = e0 + lane;
+        if (e < nlist) {
+            const uint32_t v = wlist[e];
+            const uint32_t k = v >> 6, c = v & 63u;
+            const uint32_t r = wave * 4 + k;
+            const float center = tile[r + kHalo][c + kHalo];
             uint32_t curr = 1, streak = 1;
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
-                const float counter =
-                    tile[r + kHalo + c_circle_y[i]][lane + kHalo + c_circle_x[i]];
+                const float counter = tile[r + kHalo + c_circle_y[i]][c + kHalo + c_circle_x[i]];
                 const bool diff = __builtin_fabsf(center - counter) > prm.threshold;
                 curr = diff ? curr + 1 : 1;
                 streak = curr > streak ? curr : streak;
             }
-            score[k] = streak > prm.min_run ? streak : 0u;
+            wscore[k * 64 + c] = static_cast<uint8_t>(streak > prm.min_run ? streak : 0u);
         }
     }
+    wave_sync();
+    uint32_t score[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) score[k] = wscore[k * 64 + lane];
 
     // Row 0 of the level keeps only the first maximum of each 3-wide window
     // starting at a multiple of 9 (cull_corners.wgsl:36-56 with x = 9*gid.x).
@@ -366,8 +416,7 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
         const uint32_t r = wave * 4 + k;
         const uint32_t y = static_cast<uint32_t>(y0) + r;
         if (y >= h) continue;  // wave-uniform
-        const uint64_t mword = P.mask[G.word_off + y * G.wpr + tile_x];
-        bool keep = x < w && ((mword >> lane) & 1ull);
+        bool keep = x < w && ((mwords[k] >> lane) & 1ull);
         if (keep) {
             if (y == 0) {
                 const uint32_t s = x - x % 9;
