@@ -57,6 +57,19 @@ def kernel_bytes(dims, cap, counts):
     }
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy of
+    known size), or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", "traffic_plain_1280x720.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)["kernels"][kernel]
+        return int(t["fetch_bytes"] + t["write_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(frames, n_sample):
     """The oracle (scalar C++, one thread) on the first n_sample frames of
     the sequence: detect each and match consecutive pairs."""
@@ -246,7 +259,19 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the batched / mixed / matcher / serving phases")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 256 MiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = 256 << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n, "reps": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
               f"(--gpus {args.gpus} requested)", file=sys.stderr)
@@ -307,7 +332,8 @@ def main():
     dom = max(per_kernel, key=lambda n: per_kernel[n]["avg_us"] * per_kernel[n]["launches"])
     ach = per_kernel[dom]["GBps"]
     roofline = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
+                "traffic": measured_traffic(dom),
                 "avg_us": per_kernel[dom]["avg_us"], "bytes_per_launch": kb[dom]}
 
     extra = {}
