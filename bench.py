@@ -137,10 +137,10 @@ def phase_batched_sequence(torch, orbfe, ctx, d_frames, feats, matches, reps):
 def phase_mixed(torch, orbfe, synth, reps):
     """Config 2: 64 frames of random resolution and cap, each matched against
     its translated twin; plain per-frame launches vs one prepared batch per
-    stage in a HIP graph."""
+    stage in a HIP graph, its launch shapes picked by the autotuner."""
+    from uni_slam_amd import autotune
     items = synth.mixed_batch(2, n_frames=64)
     dev = torch.device("cuda:0")
-    ctx = orbfe.Context(752, 480, fast_threshold=thr20(), lowe_ratio=RATIO)
     per_res = {}
     frames, pairs, plain_calls = [], [], []
     for it in items:
@@ -169,6 +169,9 @@ def phase_mixed(torch, orbfe, synth, reps):
             c.detect_async(db, w, fb, stream=stream)
             c.match_pipeline_async(fa, fb, pm, stream=stream)
     t_plain = timed(torch, plain, reps)
+    tune = autotune.tune_detect(frames, 752, 480, reps=max(3, reps), fast_threshold=thr20(),
+                                lowe_ratio=RATIO)
+    ctx = orbfe.Context(752, 480, fast_threshold=thr20(), lowe_ratio=RATIO, **tune.config())
     b = ctx.batch_create(frames)
     mb = ctx.match_batch_create(pairs)
 
@@ -182,7 +185,10 @@ def phase_mixed(torch, orbfe, synth, reps):
             "plain_frames_per_s": round(n / t_plain, 1),
             "batched_frames_per_s": round(n / t_batch, 1),
             "speedup": round(t_plain / t_batch, 2),
-            "plain_ms": round(t_plain * 1e3, 3), "batched_ms": round(t_batch * 1e3, 3)}
+            "plain_ms": round(t_plain * 1e3, 3), "batched_ms": round(t_batch * 1e3, 3),
+            "tuned_shape": {"detect_tile_rows": tune.best[0], "describe_lanes": tune.best[1]},
+            "detect_stage_ms_by_shape": {f"{r}x{d}": round(t * 1e3, 4)
+                                         for (r, d), t in sorted(tune.seconds.items())}}
 
 
 def phase_matcher(torch, orbfe, synth, n, reps):

@@ -60,6 +60,13 @@ typedef struct orbfe_config {
     uint32_t border_zero;   /* textureLoad out of range: 0 = Tint clamp
                                (robustness.cc:344-355), 1 = read zero */
     float lowe_ratio;       /* kLoweTestThreshold, match_features_cpu.cpp:75 (0.85) */
+    /* Launch shapes (no counterpart in the reference; results do not depend
+       on them).  0 = default.  detect_tile_rows: rows of a detect tile, 8, 16
+       (default) or 32.  describe_lanes: lanes per keypoint in the describe
+       kernel, 8, 16 (default) or 32.  uni_slam_amd/autotune.py picks them by
+       timing the variants on a workload. */
+    uint32_t detect_tile_rows;
+    uint32_t describe_lanes;
 } orbfe_config;
 
 typedef struct orbfe_gpu_feature {

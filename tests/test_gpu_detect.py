@@ -149,3 +149,18 @@ def test_detect_batch_mixed_resolutions_equals_plain(torch_gpu):
     h, w = item["a"].shape
     rf, rp = _oracle(item["a"], w, h, item["cap"], thr20())
     _assert_same(plain[0][0], plain[0][1], rf, rp, 6, item["cap"])
+
+
+@pytest.mark.parametrize("rows,lanes", [(8, 8), (8, 32), (32, 8), (32, 16), (16, 32)])
+def test_launch_shapes_match_oracle(torch_gpu, rows, lanes):
+    """Every detect tile height and describe lane count gives the oracle's
+    features and pyramid, on frames smaller and larger than one tile."""
+    for (W, H, seed, bz) in ((17, 13, 3, 0), (131, 97, 2, 1), (752, 480, 5, 0)):
+        a, _, _ = synth.frame_pair(seed, W, H)
+        cap = max(W * H // 250, 8)
+        _say(f"detect {W}x{H} tile_rows={rows} lanes={lanes} border_zero={bz}")
+        with orbfe.Context(W, H, features_per_level=cap, fast_threshold=thr20(),
+                           border_zero=bz, detect_tile_rows=rows, describe_lanes=lanes) as ctx:
+            feats, pyr = ctx.detect(a, with_pyramid=True)
+        rf, rp = _oracle(a, W, H, cap, thr20(), bz)
+        _assert_same(feats, pyr, rf, rp, 6, cap)

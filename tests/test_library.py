@@ -33,6 +33,19 @@ def test_config_default_matches_reference_constants():
     assert np.float32(cfg.fast_threshold) == np.float32(0.15)
     assert np.float32(cfg.lowe_ratio) == np.float32(0.85)
     assert cfg.features_per_level == 0  # W*H/250 chosen at create time
+    assert (cfg.detect_tile_rows, cfg.describe_lanes) == (0, 0)  # library defaults
+
+
+def test_create_rejects_unsupported_launch_shapes():
+    """Launch shapes are checked before any device call, so this runs
+    without a GPU."""
+    import ctypes as C
+    for bad in ({"detect_tile_rows": 12}, {"detect_tile_rows": 64}, {"describe_lanes": 4},
+                {"describe_lanes": 64}):
+        cfg = orbfe.default_config(320, 240, **bad)
+        h = C.c_void_p()
+        assert orbfe.lib().orbfe_create(C.byref(cfg), 0, C.byref(h)) == orbfe.ERANGE, bad
+        assert not h.value
 
 
 def test_load_features_host_entry_point():

@@ -19,12 +19,12 @@
 
 namespace orbfe {
 hipError_t upload_pattern();
-hipError_t launch_detect(uint32_t blocks, BatchView bv, const Job& job0, const Params& prm,
-                         hipStream_t s);
+hipError_t launch_detect(uint32_t tile_h, uint32_t blocks, BatchView bv, const Job& job0,
+                         const Params& prm, hipStream_t s);
 hipError_t launch_compact(uint32_t blocks, BatchView bv, const Job& job0, uint32_t levels,
                           hipStream_t s);
-hipError_t launch_describe(uint32_t blocks, BatchView bv, const Job& job0, const Params& prm,
-                           hipStream_t s);
+hipError_t launch_describe(uint32_t lanes, uint32_t blocks, BatchView bv, const Job& job0,
+                           const Params& prm, hipStream_t s);
 hipError_t launch_match_index(uint32_t blocks, MatchBatchView bv, const MatchJob& job0,
                               hipStream_t s);
 hipError_t launch_match_dir(bool mutual, uint32_t blocks, MatchBatchView bv,
@@ -252,8 +252,10 @@ int get_plan(orbfe_ctx* ctx, uint32_t W0, uint32_t H0, uint32_t cap, PlanEntry**
     P.H0 = H0;
     P.L = L;
     P.cap = cap;
+    P.tile_h = ctx->cfg.detect_tile_rows;
+    P.desc_lanes = ctx->cfg.describe_lanes;
     P.feat_stride = 8 + 48 * cap;
-    P.desc_groups = ceil_div(cap, orbfe::kDescPerBlock);
+    P.desc_groups = ceil_div(cap, 256 / P.desc_lanes);
     P.query_groups = ceil_div(cap, orbfe::kQueriesPerBlock);
     P.max_dx = 0.1f * static_cast<float>(W0);
     P.max_dy = 0.1f * static_cast<float>(H0);
@@ -270,7 +272,7 @@ int get_plan(orbfe_ctx* ctx, uint32_t W0, uint32_t H0, uint32_t cap, PlanEntry**
         G.h = dh[l];
         G.wpr = ceil_div(G.w, 64);
         G.tiles_x = ceil_div(G.w + orbfe::kSmoothExt, orbfe::kTileW);
-        G.tiles_y = ceil_div(G.h + orbfe::kSmoothExt, orbfe::kTileH);
+        G.tiles_y = ceil_div(G.h + orbfe::kSmoothExt, P.tile_h);
         G.tile_base = tiles;
         tiles += G.tiles_x * G.tiles_y;
         G.pyr_off = l == 0 ? 0 : pyr;
@@ -339,13 +341,14 @@ int launch_detect_stage(orbfe_ctx* ctx, const BatchView& bv, const Job& job0, ui
                         uint32_t blocks_compact, uint32_t desc_blocks, hipStream_t s) {
     const Params prm = ctx->prm;
     HIP_TRY(ctx, timed(ctx, "k_detect", s, [&] {
-                return orbfe::launch_detect(tiles, bv, job0, prm, s);
+                return orbfe::launch_detect(ctx->cfg.detect_tile_rows, tiles, bv, job0, prm, s);
             }));
     HIP_TRY(ctx, timed(ctx, "k_compact", s, [&] {
                 return orbfe::launch_compact(blocks_compact, bv, job0, ctx->cfg.levels, s);
             }));
     HIP_TRY(ctx, timed(ctx, "k_describe", s, [&] {
-                return orbfe::launch_describe(desc_blocks, bv, job0, prm, s);
+                return orbfe::launch_describe(ctx->cfg.describe_lanes, desc_blocks, bv, job0, prm,
+                                              s);
             }));
     return ORBFE_OK;
 }
@@ -597,9 +600,15 @@ int orbfe_create(const orbfe_config* cfg, int device, orbfe_ctx** out) {
     if (!(cfg->scale_factor > 1.0)) return ORBFE_ERANGE;
     if (cfg->orientation_radius < 0 || cfg->orientation_radius > 15) return ORBFE_ERANGE;
     if (cfg->fast_min_run > 16) return ORBFE_ERANGE;
+    const uint32_t th = cfg->detect_tile_rows ? cfg->detect_tile_rows : orbfe::kTileHDefault;
+    const uint32_t dl = cfg->describe_lanes ? cfg->describe_lanes : orbfe::kDescLanesDefault;
+    if ((th != 8 && th != 16 && th != 32) || (dl != 8 && dl != 16 && dl != 32))
+        return ORBFE_ERANGE;
     auto ctx = std::make_unique<orbfe_ctx>();
     ctx->device = device;
     ctx->cfg = *cfg;
+    ctx->cfg.detect_tile_rows = th;  // every plan of a context shares the launch shapes
+    ctx->cfg.describe_lanes = dl;
     ctx->prm.threshold = cfg->fast_threshold;
     ctx->prm.min_run = cfg->fast_min_run;
     ctx->prm.orient_r = cfg->orientation_radius;

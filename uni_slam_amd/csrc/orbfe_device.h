@@ -21,17 +21,18 @@ namespace orbfe {
 
 constexpr uint32_t kMaxLevels = 8;
 
-// Detect tile: 64 columns (one wave per row segment) x 16 rows, 256 threads,
-// each thread scoring 4 rows.  The halo of 3 covers the radius-3 FAST ring.
+// Detect tile: 64 columns (one wave per row segment) x tile_h rows (8, 16
+// or 32), 256 threads, each thread scoring tile_h / 4 rows.  The halo of 3
+// covers the radius-3 FAST ring.
 constexpr int kTileW = 64;
-constexpr int kTileH = 16;
+constexpr int kTileHDefault = 16;
 constexpr int kHalo = 3;
 constexpr int kLdsW = kTileW + 2 * kHalo;  // 70
-constexpr int kLdsH = kTileH + 2 * kHalo;  // 22
 constexpr int kDetectThreads = 256;
 
-// k_describe: keypoints per workgroup (4 per wave, 16 lanes each).
-constexpr uint32_t kDescPerBlock = 16;
+// k_describe: desc_lanes lanes per keypoint (8, 16 or 32), so 256 /
+// desc_lanes keypoints per 256-thread workgroup.
+constexpr uint32_t kDescLanesDefault = 16;
 // k_match_dir: queries per workgroup (one per wave).
 constexpr uint32_t kQueriesPerBlock = 4;
 // Matcher cell index: LOD0 plane cut into kCellsPerAxis^2 cells of half the
@@ -71,11 +72,13 @@ struct LevelGeom {
 
 struct Plan {
     uint32_t W0, H0, L, cap;
+    uint32_t tile_h;      // detect tile rows
+    uint32_t desc_lanes;  // describe lanes per keypoint
     uint32_t tiles;       // detect tiles of one frame
     uint32_t words;       // 64-bit survivor words of one frame
     uint32_t pyr_floats;  // pyramid floats (levels >= 1)
     uint32_t feat_stride; // 8 + 48*cap
-    uint32_t desc_groups; // describe blocks per level = ceil(cap / kDescPerBlock)
+    uint32_t desc_groups; // describe blocks per level = ceil(cap / (256 / desc_lanes))
     uint32_t query_groups; // matcher blocks per level = ceil(cap / kQueriesPerBlock)
     float max_dx, max_dy; // 0.1f*W0, 0.1f*H0 (match_features_cpu.cpp:76-77)
     uint32_t sm_floats;   // smoothed-image floats of one frame

@@ -1,18 +1,19 @@
 // HIP kernels of the ORB front-end for gfx950 (CDNA4, wave64).
 //
-//   k_detect    pyramid levels + FAST-style score + cull mask, one 64x16 tile
-//               per workgroup, LDS-staged halo; survivors leave as a bitmap
-//               built with wave ballots (one 64-bit word per 64-pixel row
-//               segment).
+//   k_detect    pyramid levels, smoothed image, FAST-style score and cull
+//               mask over one 64 x TH tile per workgroup (TH = 8/16/32),
+//               LDS-staged halo; survivors leave as a bitmap built with wave
+//               ballots (one 64-bit word per 64-pixel row segment).
 //   k_compact   per (frame, level): raster-order prefix sum over the survivor
 //               bitmap, first `cap` survivors to a keypoint list, count.
-//   k_describe  one wave per keypoint: corner strength, intensity-centroid
-//               angle, 256 rotated 9-tap BRIEF tests (4 per lane, bits from
-//               ballots), 48-byte feature record.
-//   k_match_dir pipeline matcher, one wave per query: LOD+-1 pool, six
-//               concurrent lane-group searches of the y windows, XOR/popcount
-//               over the x window,
-//               best/second-best keys reduced across the wave.
+//   k_describe  8/16/32 lanes per keypoint: corner strength, intensity-
+//               centroid angle, 256 rotated BRIEF tests looked up in the
+//               smoothed image, 48-byte feature record.
+//   k_match_index  per (pair, side, level) counting sort of the records by
+//               LOD0 cell.
+//   k_match_dir pipeline matcher, one wave per query: LOD+-1 pool, candidates
+//               from the cells under the search window, XOR/popcount, best/
+//               second-best keys reduced across the wave.
 //   k_bf_partial / k_bf_merge  brute-force best/second-best Hamming search,
 //               queries in registers, train tiles in LDS.
 #include <hip/hip_runtime.h>
@@ -108,10 +109,20 @@ __device__ uint32_t fast_score_from_frame(const Job& J, const Plan& P, const Lev
 // ---------------------------------------------------------------------------
 // k_detect
 // ---------------------------------------------------------------------------
-// LDS bytes for the u8 source region of a level >= 1 tile: (70*s + 3) x
-// (22*s + 3) for level scale s, 10.4 KB at s = 1.2^5.  Larger regions (more
-// levels) fall back to per-texel gathers.
-constexpr uint32_t kSrcWords = 3072;
+// One LDS buffer holds first the u8 source region of a level >= 1 tile
+// ((TH+6)*s + 3 rows of 70*s + 3 bytes at level scale s: 6.4 KB, 10.4 KB,
+// 17.1 KB for TH = 8, 16, 32 at s = 1.2^5) and then, once the level tile is
+// staged, the horizontal lerps of the smoothing (3 x (TH+6) x 64 floats:
+// 10.5 KB, 16.5 KB, 28.5 KB).  Source regions that do not fit (levels past
+// the fifth) fall back to per-texel gathers.
+template <int TH>
+constexpr uint32_t stage_words() { return 3u * (TH + 6) * 64; }
+
+// Byte i of the staged source region, read as a dword and shifted, so the
+// load stays a 32-bit LDS read.
+__device__ inline uint32_t src_byte(const uint32_t* src, uint32_t i) {
+    return (src[i >> 2] >> ((i & 3u) * 8u)) & 0xFFu;
+}
 
 // Smoothed value at (x, y) evaluated directly from the frame (tiles whose
 // sampler taps fall outside the staged region: only at the far corner of
@@ -140,22 +151,28 @@ __device__ float smooth_direct(const Job& J, const Plan& P, const LevelGeom& G, 
     return sum;
 }
 
+template <int TH>
 __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job0,
                                                             Params prm) {
+    constexpr int LH = TH + 2 * kHalo;                     // staged rows
+    constexpr int RPT = TH / (kDetectThreads / 64);        // rows per thread
+    constexpr uint32_t kSrcWords = stage_words<TH>();
     __shared__ float lut[256];
-    __shared__ float tile[kLdsH][kLdsW];
-    __shared__ float hbuf[3][kLdsH][kTileW];  // horizontal lerps of the 9-tap Gaussian
-    __shared__ uint32_t src[kSrcWords];       // u8 source region (levels >= 1)
+    __shared__ float tile[LH][kLdsW];
+    // u8 source region (levels >= 1) while staging, then hbuf[3][LH][64],
+    // the horizontal lerps of the 9-tap Gaussian.
+    __shared__ uint32_t stage[kSrcWords];
+    uint32_t* const src = stage;
     __shared__ uint32_t col_i0[kLdsW], col_i1[kLdsW];
     __shared__ float col_a[kLdsW];
-    __shared__ uint32_t row_i0[kLdsH], row_i1[kLdsH];
-    __shared__ float row_b[kLdsH];
-    __shared__ uint32_t col_ok[kLdsW], row_ok[kLdsH];
-    __shared__ uint32_t srow_i0[3][kTileH], srow_i1[3][kTileH];
-    __shared__ float srow_b[3][kTileH];
+    __shared__ uint32_t row_i0[LH], row_i1[LH];
+    __shared__ float row_b[LH];
+    __shared__ uint32_t col_ok[kLdsW], row_ok[LH];
+    __shared__ uint32_t srow_i0[3][TH], srow_i1[3][TH];
+    __shared__ float srow_b[3][TH];
     __shared__ uint32_t row0_score[kTileW + 4];
-    __shared__ uint8_t score_lds[kDetectThreads / 64][4 * 64];  // per wave: 4 rows x 64
-    __shared__ uint8_t list_lds[kDetectThreads / 64][4 * 64];
+    __shared__ uint8_t score_lds[kDetectThreads / 64][RPT * 64];  // per wave: RPT rows x 64
+    __shared__ uint16_t list_lds[kDetectThreads / 64][RPT * 64];
 
     const uint32_t tid = threadIdx.x;
     uint32_t b = blockIdx.x;
@@ -173,7 +190,7 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     const uint32_t tile_y = t / G.tiles_x;
     const uint32_t tile_x = t - tile_y * G.tiles_x;
     const int32_t x0 = static_cast<int32_t>(tile_x) * kTileW;
-    const int32_t y0 = static_cast<int32_t>(tile_y) * kTileH;
+    const int32_t y0 = static_cast<int32_t>(tile_y) * TH;
     const uint32_t w = G.w, h = G.h;
     const uint32_t sw = w + kSmoothExt, sh = h + kSmoothExt;
     const float texel_x = G.texel_x;
@@ -184,7 +201,7 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     lut[tid] = static_cast<float>(tid) / 255.0f;
 
     // Per-column / per-row source taps of the staged region.
-    if (tid < kLdsW + kLdsH) {
+    if (tid < kLdsW + LH) {
         const bool is_col = tid < kLdsW;
         const uint32_t c = is_col ? tid : tid - kLdsW;
         const int32_t g = (is_col ? x0 : y0) - kHalo + static_cast<int32_t>(c);
@@ -207,10 +224,10 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
         } else {
             row_i0[c] = i0; row_i1[c] = i1; row_b[c] = frac; row_ok[c] = ok;
         }
-    } else if (tid < kLdsW + kLdsH + 3 * kTileH) {
-        // Row taps of the 9-tap Gaussian for the tile's 16 output rows.
-        const uint32_t e = tid - (kLdsW + kLdsH);
-        const uint32_t jj = e / kTileH, rr = e - jj * kTileH;
+    } else if (tid < kLdsW + LH + 3 * TH) {
+        // Row taps of the 9-tap Gaussian for the tile's TH output rows.
+        const uint32_t e = tid - (kLdsW + LH);
+        const uint32_t jj = e / TH, rr = e - jj * TH;
         const float uvy = (static_cast<float>(static_cast<uint32_t>(y0) + rr) + 0.5f) * texel_y;
         const AxisTaps ay = axis_taps(uvy + offs[jj] * texel_y, h);
         srow_i0[jj][rr] = ay.i0;
@@ -224,12 +241,12 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     // to the far edge) and levels whose rectangle does not fit use gathers.
     const uint32_t c_first = x0 >= kHalo ? 0u : static_cast<uint32_t>(kHalo - x0);
     const uint32_t r_first = y0 >= kHalo ? 0u : static_cast<uint32_t>(kHalo - y0);
-    uint32_t c_last = kLdsW - 1, r_last = kLdsH - 1;
+    uint32_t c_last = kLdsW - 1, r_last = LH - 1;
     if (prm.border_zero) {
         const int32_t cl_ = static_cast<int32_t>(w) - 1 - x0 + kHalo;
         const int32_t rl_ = static_cast<int32_t>(h) - 1 - y0 + kHalo;
         c_last = cl_ < kLdsW - 1 ? static_cast<uint32_t>(cl_ < 0 ? 0 : cl_) : kLdsW - 1;
-        r_last = rl_ < kLdsH - 1 ? static_cast<uint32_t>(rl_ < 0 ? 0 : rl_) : kLdsH - 1;
+        r_last = rl_ < LH - 1 ? static_cast<uint32_t>(rl_ < 0 ? 0 : rl_) : LH - 1;
     }
     const uint32_t colmin = col_i0[c_first], colmax = col_i1[c_last];
     const uint32_t rowmin = row_i0[r_first], rowmax = row_i1[r_last];
@@ -249,11 +266,10 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
         }
         __syncthreads();
     }
-    const uint8_t* srcb = reinterpret_cast<const uint8_t*>(src);
     const uint32_t spitch = nd * 4;
 
-    // Stage the level tile (22 x 70 texels, flat over the block).
-    for (uint32_t e = tid; e < kLdsW * kLdsH; e += kDetectThreads) {
+    // Stage the level tile ((TH + 6) x 70 texels, flat over the block).
+    for (uint32_t e = tid; e < kLdsW * LH; e += kDetectThreads) {
         const uint32_t r = e / kLdsW;
         const uint32_t c = e - r * kLdsW;
         float v = 0.0f;
@@ -263,16 +279,24 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
                 v = lut[J.img[row_i0[r] * J.pitch + ci0]];
             } else {
                 const uint32_t ci1 = col_i1[c];
-                const uint8_t* s0;
-                const uint8_t* s1;
+                uint32_t p00, p01, p10, p11;
                 if (use_src && c >= c_first && c <= c_last && r >= r_first && r <= r_last) {
-                    s0 = srcb + (row_i0[r] - rowmin) * spitch - cs;
-                    s1 = srcb + (row_i1[r] - rowmin) * spitch - cs;
+                    // Byte offsets inside the staged region (ci >= cs, rows
+                    // >= rowmin), so every address formed lies in the
+                    // region: a pointer before its start (srcb - cs) can
+                    // carry out of the LDS aperture in flat addressing.
+                    const uint32_t o0 = (row_i0[r] - rowmin) * spitch;
+                    const uint32_t o1 = (row_i1[r] - rowmin) * spitch;
+                    const uint32_t a0 = ci0 - cs, a1 = ci1 - cs;
+                    p00 = src_byte(src, o0 + a0); p01 = src_byte(src, o0 + a1);
+                    p10 = src_byte(src, o1 + a0); p11 = src_byte(src, o1 + a1);
                 } else {
-                    s0 = J.img + row_i0[r] * J.pitch;
-                    s1 = J.img + row_i1[r] * J.pitch;
+                    const uint8_t* s0 = J.img + row_i0[r] * J.pitch;
+                    const uint8_t* s1 = J.img + row_i1[r] * J.pitch;
+                    p00 = s0[ci0]; p01 = s0[ci1];
+                    p10 = s1[ci0]; p11 = s1[ci1];
                 }
-                v = lerp2(lut[s0[ci0]], lut[s0[ci1]], lut[s1[ci0]], lut[s1[ci1]], col_a[c], row_b[r]);
+                v = lerp2(lut[p00], lut[p01], lut[p10], lut[p11], col_a[c], row_b[r]);
             }
         }
         tile[r][c] = v;
@@ -294,17 +318,18 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
             for (int i = 0; i < 3; ++i) {
                 const AxisTaps ax = axis_taps(uvx + offs[i] * texel_x, w);
                 const uint32_t c0 = ax.i0 - x0 + kHalo, c1 = ax.i1 - x0 + kHalo;
-                for (uint32_t r = wave; r < kLdsH; r += kDetectThreads / 64) {
+                for (uint32_t r = wave; r < LH; r += kDetectThreads / 64) {
                     const float t0 = tile[r][c0];
-                    hbuf[i][r][lane] = t0 + ax.frac * (tile[r][c1] - t0);
+                    stage[(i * LH + r) * kTileW + lane] =
+                        __float_as_uint(t0 + ax.frac * (tile[r][c1] - t0));
                 }
             }
         }
         __syncthreads();
         if (x < sw) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const uint32_t rr = wave * 4 + k;
+            for (int k = 0; k < RPT; ++k) {
+                const uint32_t rr = wave * RPT + k;
                 const uint32_t y = static_cast<uint32_t>(y0) + rr;
                 if (y >= sh) continue;
                 float sum = 0.0f;
@@ -315,8 +340,9 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
                     const float bb = srow_b[jj][rr];
 #pragma unroll
                     for (int i = 0; i < 3; ++i) {
-                        const float top = hbuf[i][ra][lane];
-                        const float sv = top + bb * (hbuf[i][rb][lane] - top);
+                        const float top = __uint_as_float(stage[(i * LH + ra) * kTileW + lane]);
+                        const float sv =
+                            top + bb * (__uint_as_float(stage[(i * LH + rb) * kTileW + lane]) - top);
                         sum = sum + sv * wts[i] * wts[jj];
                     }
                 }
@@ -324,8 +350,8 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
             }
         }
     } else if (x < sw) {
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t y = static_cast<uint32_t>(y0) + wave * 4 + k;
+        for (int k = 0; k < RPT; ++k) {
+            const uint32_t y = static_cast<uint32_t>(y0) + wave * RPT + k;
             if (y < sh) J.smooth[G.sm_off + y * sw + x] = smooth_direct(J, P, G, l, lut, x, y);
         }
     }
@@ -337,18 +363,18 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     // scores 0; the rest are gathered into a per-wave list and get the full
     // ring.  Scores land in score_lds (0 elsewhere).
     uint8_t* wscore = score_lds[wave];
-    uint8_t* wlist = list_lds[wave];
+    uint16_t* wlist = list_lds[wave];
     const bool quick = prm.min_run >= 9;
     uint32_t nlist = 0;
-    uint64_t mwords[4];
+    uint64_t mwords[RPT];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t y = static_cast<uint32_t>(y0) + wave * 4 + k;
+    for (int k = 0; k < RPT; ++k) {
+        const uint32_t y = static_cast<uint32_t>(y0) + wave * RPT + k;
         mwords[k] = (tile_x < G.wpr && y < h) ? P.mask[G.word_off + y * G.wpr + tile_x] : 0ull;
     }
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t r = wave * 4 + k;
+    for (int k = 0; k < RPT; ++k) {
+        const uint32_t r = wave * RPT + k;
         const uint32_t y = static_cast<uint32_t>(y0) + r;
         wscore[k * 64 + lane] = 0;
         bool need = false;
@@ -366,7 +392,7 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
         const uint64_t bits = __ballot(need);
         if (need) {
             const uint32_t pos = nlist + __popcll(bits & ((1ull << lane) - 1ull));
-            wlist[pos] = static_cast<uint8_t>((k << 6) | lane);
+            wlist[pos] = static_cast<uint16_t>((k << 6) | lane);
         }
         nlist += __popcll(bits);
     }
@@ -376,7 +402,7 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
         if (e < nlist) {
             const uint32_t v = wlist[e];
             const uint32_t k = v >> 6, c = v & 63u;
-            const uint32_t r = wave * 4 + k;
+            const uint32_t r = wave * RPT + k;
             const float center = tile[r + kHalo][c + kHalo];
             uint32_t curr = 1, streak = 1;
 #pragma unroll
@@ -390,9 +416,9 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
         }
     }
     wave_sync();
-    uint32_t score[4];
+    uint32_t score[RPT];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) score[k] = wscore[k * 64 + lane];
+    for (int k = 0; k < RPT; ++k) score[k] = wscore[k * 64 + lane];
 
     // Row 0 of the level keeps only the first maximum of each 3-wide window
     // starting at a multiple of 9 (cull_corners.wgsl:36-56 with x = 9*gid.x).
@@ -412,8 +438,8 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
 
     if (tile_x >= G.wpr) return;  // extension-only tile: no corners
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t r = wave * 4 + k;
+    for (int k = 0; k < RPT; ++k) {
+        const uint32_t r = wave * RPT + k;
         const uint32_t y = static_cast<uint32_t>(y0) + r;
         if (y >= h) continue;  // wave-uniform
         bool keep = x < w && ((mwords[k] >> lane) & 1ull);
@@ -569,12 +595,21 @@ struct LevelSrc {
 
 }  // namespace
 
-// Four keypoints per wave, 16 lanes each (16 per workgroup): lane s of a
-// group scores ring pixel s and evaluates tests 16*s .. 16*s+15.
+// LANES lanes per keypoint (8, 16 or 32; 256 / LANES keypoints per
+// workgroup): lane s of a group scores ring pixels s, s + LANES, ... and
+// evaluates tests TPL*s .. TPL*s + TPL - 1, TPL = 256 / LANES.  Every
+// variant computes each record with the same operations in the same order.
 constexpr uint32_t kDescribeThreads = 256;
 
+template <uint32_t LANES>
 __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job job0,
                                                                 Params prm) {
+    constexpr uint32_t KPB = kDescribeThreads / LANES;  // keypoints per block
+    constexpr uint32_t TPL = 256 / LANES;               // tests per lane
+    constexpr uint32_t LPW = LANES / 8;                 // lanes per descriptor word
+    constexpr uint32_t RING = LANES < 16 ? 16 / LANES : 1;
+    constexpr uint32_t OUTS = (12 + LANES - 1) / LANES; // record words per lane
+    constexpr uint64_t GMASK = LANES == 64 ? ~0ull : ((1ull << LANES) - 1ull);
     __shared__ float lut[256];
     const uint32_t tid = threadIdx.x;
     lut[tid] = static_cast<float>(tid) / 255.0f;
@@ -591,9 +626,9 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
     const uint32_t l = b / P.desc_groups;
     const uint32_t g = b - l * P.desc_groups;
     const uint32_t lane = tid & 63;
-    const uint32_t sub = lane & 15;        // lane within the keypoint group
-    const uint32_t gbase = lane & ~15u;    // first lane of the group
-    const uint32_t slot = g * kDescPerBlock + (tid >> 4);
+    const uint32_t sub = lane & (LANES - 1);   // lane within the keypoint group
+    const uint32_t gbase = lane & ~(LANES - 1);  // first lane of the group
+    const uint32_t slot = g * KPB + tid / LANES;
     const LevelGeom& G = P.lv[l];
     uint8_t* block = J.feat + G.feat_off;
     const uint32_t count = reinterpret_cast<const uint32_t*>(block)[0];
@@ -602,7 +637,11 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
     const bool active = slot < kept;
     uint32_t* out = reinterpret_cast<uint32_t*>(block + 8 + 48ull * slot);
     if (__ballot(active) == 0) {  // whole wave empty: zero its slots
-        if (in_cap && sub < 12) out[sub] = 0u;
+#pragma unroll
+        for (uint32_t o = 0; o < OUTS; ++o) {
+            const uint32_t e = sub + o * LANES;
+            if (in_cap && e < 12) out[e] = 0u;
+        }
         return;
     }
     const uint32_t kpv = active ? J.kp[G.kp_off + slot] : 0u;
@@ -618,12 +657,22 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
     S.h = G.h;
     S.base = l == 0;
 
-    // Corner strength (detect_corners.wgsl:58-81): lane s holds ring pixel s.
+    // Corner strength (detect_corners.wgsl:58-81): ring bit i from lane
+    // i % LANES.
     const float center = S.at(x, y);
-    const float counter = S.load(static_cast<int32_t>(x) + c_circle_x[sub],
-                                 static_cast<int32_t>(y) + c_circle_y[sub], prm.border_zero);
-    const bool diff = __builtin_fabsf(center - counter) > prm.threshold;
-    const uint32_t ring = static_cast<uint32_t>(__ballot(diff) >> gbase) & 0xFFFFu;
+    uint32_t ring = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < RING; ++q) {
+        const uint32_t i = sub + q * LANES;
+        bool diff = false;
+        if (i < 16) {
+            const float counter = S.load(static_cast<int32_t>(x) + c_circle_x[i],
+                                         static_cast<int32_t>(y) + c_circle_y[i], prm.border_zero);
+            diff = __builtin_fabsf(center - counter) > prm.threshold;
+        }
+        ring |= static_cast<uint32_t>((__ballot(diff) >> gbase) & GMASK) << (q * LANES);
+    }
+    ring &= 0xFFFFu;
     uint32_t curr = 1, streak = 1;
     for (int i = 0; i < 16; ++i) {
         curr = ((ring >> i) & 1u) ? curr + 1 : 1;
@@ -648,15 +697,15 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
     sincos_own(theta, &sn, &cs);
 
     // Descriptor (generate_features.wgsl:101-144): lane s runs tests
-    // 16s .. 16s+15; G at the wrapped u32 coordinates (:116-129) is looked
-    // up in the smoothed image, where coordinates past the extension
-    // (including wrapped negatives) share the value at w+3 / h+3.
+    // TPL*s .. TPL*s + TPL - 1; G at the wrapped u32 coordinates (:116-129)
+    // is looked up in the smoothed image, where coordinates past the
+    // extension (including wrapped negatives) share the value at w+3 / h+3.
     const uint32_t sw = G.w + kSmoothExt, sh = G.h + kSmoothExt;
     const float* sm = J.smooth + G.sm_off;
     uint32_t bits = 0;
 #pragma unroll 4
-    for (uint32_t k = 0; k < 16; ++k) {
-        const uint32_t i = 16 * sub + k;
+    for (uint32_t k = 0; k < TPL; ++k) {
+        const uint32_t i = TPL * sub + k;
         const float pax = c_pattern[i][0], pay = c_pattern[i][1];
         const float pbx = c_pattern[i][2], pby = c_pattern[i][3];
         const int32_t ax = static_cast<int32_t>(__builtin_rintf(pax * cs - pay * sn));
@@ -671,24 +720,31 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
         const float sb = sm[pby_u * sw + pbx_u];
         bits |= (sa > sb ? 1u : 0u) << k;
     }
-    // Word w of the descriptor = bits of lanes 2w (low half) and 2w+1.
-    const uint32_t wsel = sub >= 4 ? sub - 4 : 0;
-    const uint32_t lo = __shfl(bits, static_cast<int>(gbase + 2 * wsel));
-    const uint32_t hi = __shfl(bits, static_cast<int>(gbase + 2 * wsel + 1));
-    if (in_cap && sub < 12) {
-        uint32_t v;
-        if (!active) {
-            v = 0u;
-        } else {
-            switch (sub) {
-                case 0: v = __float_as_uint(static_cast<float>(x)); break;
-                case 1: v = __float_as_uint(static_cast<float>(y)); break;
-                case 2: v = strength; break;
-                case 3: v = __float_as_uint(theta); break;
-                default: v = lo | (hi << 16); break;
+    // Record word e: x, y, strength, angle, then descriptor word e - 4 =
+    // bits of lanes LPW*(e-4) .. LPW*(e-4) + LPW - 1, low lane first.
+#pragma unroll
+    for (uint32_t o = 0; o < OUTS; ++o) {
+        const uint32_t e = sub + o * LANES;
+        const uint32_t wsel = e >= 4 && e < 12 ? e - 4 : 0;
+        uint32_t word = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < LPW; ++q)
+            word |= __shfl(bits, static_cast<int>(gbase + wsel * LPW + q)) << (q * TPL);
+        if (in_cap && e < 12) {
+            uint32_t v;
+            if (!active) {
+                v = 0u;
+            } else {
+                switch (e) {
+                    case 0: v = __float_as_uint(static_cast<float>(x)); break;
+                    case 1: v = __float_as_uint(static_cast<float>(y)); break;
+                    case 2: v = strength; break;
+                    case 3: v = __float_as_uint(theta); break;
+                    default: v = word; break;
+                }
             }
+            out[e] = v;
         }
-        out[sub] = v;
     }
 }
 
@@ -1020,9 +1076,14 @@ hipError_t upload_pattern() {
     return hipMemcpyToSymbol(HIP_SYMBOL(c_pattern), kOrbfeBriefPattern, sizeof(kOrbfeBriefPattern));
 }
 
-hipError_t launch_detect(uint32_t blocks, BatchView bv, const Job& job0, const Params& prm,
-                         hipStream_t s) {
-    hipLaunchKernelGGL(k_detect, dim3(blocks), dim3(kDetectThreads), 0, s, bv, job0, prm);
+hipError_t launch_detect(uint32_t tile_h, uint32_t blocks, BatchView bv, const Job& job0,
+                         const Params& prm, hipStream_t s) {
+    switch (tile_h) {
+        case 8: hipLaunchKernelGGL(k_detect<8>, dim3(blocks), dim3(kDetectThreads), 0, s, bv, job0, prm); break;
+        case 16: hipLaunchKernelGGL(k_detect<16>, dim3(blocks), dim3(kDetectThreads), 0, s, bv, job0, prm); break;
+        case 32: hipLaunchKernelGGL(k_detect<32>, dim3(blocks), dim3(kDetectThreads), 0, s, bv, job0, prm); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -1032,9 +1093,14 @@ hipError_t launch_compact(uint32_t blocks, BatchView bv, const Job& job0, uint32
     return hipGetLastError();
 }
 
-hipError_t launch_describe(uint32_t blocks, BatchView bv, const Job& job0, const Params& prm,
-                           hipStream_t s) {
-    hipLaunchKernelGGL(k_describe, dim3(blocks), dim3(kDescribeThreads), 0, s, bv, job0, prm);
+hipError_t launch_describe(uint32_t lanes, uint32_t blocks, BatchView bv, const Job& job0,
+                           const Params& prm, hipStream_t s) {
+    switch (lanes) {
+        case 8: hipLaunchKernelGGL(k_describe<8>, dim3(blocks), dim3(kDescribeThreads), 0, s, bv, job0, prm); break;
+        case 16: hipLaunchKernelGGL(k_describe<16>, dim3(blocks), dim3(kDescribeThreads), 0, s, bv, job0, prm); break;
+        case 32: hipLaunchKernelGGL(k_describe<32>, dim3(blocks), dim3(kDescribeThreads), 0, s, bv, job0, prm); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -17,6 +17,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liborbfe.so")
 
 OK = 0
+EINVAL, EHIP, ENOMEM, ERANGE = -1, -2, -3, -4  # ORBFE_* error codes of orbfe.h
 MAX_LEVELS = 8
 
 GPU_FEATURE_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("strength", "<u4"),
@@ -31,7 +32,8 @@ class Config(C.Structure):
                 ("levels", C.c_uint32), ("scale_factor", C.c_double),
                 ("fast_threshold", C.c_float), ("fast_min_run", C.c_uint32),
                 ("orientation_radius", C.c_int32), ("features_per_level", C.c_uint32),
-                ("border_zero", C.c_uint32), ("lowe_ratio", C.c_float)]
+                ("border_zero", C.c_uint32), ("lowe_ratio", C.c_float),
+                ("detect_tile_rows", C.c_uint32), ("describe_lanes", C.c_uint32)]
 
 
 class Frame(C.Structure):
