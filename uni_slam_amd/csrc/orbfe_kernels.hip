@@ -72,6 +72,23 @@ __device__ inline float level_texel_from_frame(const Job& J, const Plan& P,
                  tx.frac, ty.frac);
 }
 
+// float(u8) / 255, the value the LDS table of the kernels holds.
+__device__ inline float u8_unit(uint32_t v) { return static_cast<float>(v) / 255.0f; }
+
+// level_texel_from_frame without the LDS table (usable before it is filled).
+__device__ inline float level_texel_direct(const Job& J, const Plan& P, const LevelGeom& G,
+                                           uint32_t l, uint32_t x, uint32_t y) {
+    if (l == 0) return u8_unit(J.img[y * J.pitch + x]);
+    const float u = (static_cast<float>(x) + 0.5f) / static_cast<float>(G.w);
+    const float v = (static_cast<float>(y) + 0.5f) / static_cast<float>(G.h);
+    const AxisTaps tx = axis_taps(u, P.W0);
+    const AxisTaps ty = axis_taps(v, P.H0);
+    const uint8_t* r0 = J.img + ty.i0 * J.pitch;
+    const uint8_t* r1 = J.img + ty.i1 * J.pitch;
+    return lerp2(u8_unit(r0[tx.i0]), u8_unit(r0[tx.i1]), u8_unit(r1[tx.i0]),
+                 u8_unit(r1[tx.i1]), tx.frac, ty.frac);
+}
+
 // textureLoad(level l, (gx, gy)) with the robustness rule, from the frame.
 __device__ inline float load_from_frame(const Job& J, const Plan& P, const LevelGeom& G,
                                         uint32_t l, const float* lut, int32_t gx,
@@ -84,24 +101,6 @@ __device__ inline float load_from_frame(const Job& J, const Plan& P, const Level
     }
     return level_texel_from_frame(J, P, G, l, lut, tint_clamp(gx, G.w),
                                   tint_clamp(gy, G.h));
-}
-
-// detect_corners.wgsl:58-81 evaluated straight from the frame (used for the
-// few row-0 neighbours outside a tile).
-__device__ uint32_t fast_score_from_frame(const Job& J, const Plan& P, const LevelGeom& G,
-                                          uint32_t l, const float* lut, uint32_t x,
-                                          uint32_t y, const Params& prm) {
-    const float center = level_texel_from_frame(J, P, G, l, lut, x, y);
-    uint32_t curr = 1, streak = 1;
-    for (int i = 0; i < 16; ++i) {
-        const float counter =
-            load_from_frame(J, P, G, l, lut, static_cast<int32_t>(x) + c_circle_x[i],
-                            static_cast<int32_t>(y) + c_circle_y[i], prm.border_zero);
-        const bool diff = __builtin_fabsf(center - counter) > prm.threshold;
-        curr = diff ? curr + 1 : 1;
-        streak = curr > streak ? curr : streak;
-    }
-    return streak > prm.min_run ? streak : 0u;
 }
 
 }  // namespace
@@ -159,7 +158,7 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     constexpr uint32_t kSrcWords = stage_words<TH>();
     __shared__ float lut[256];
     __shared__ float tile[LH][kLdsW];
-    // u8 source region (levels >= 1) while staging, then hbuf[3][LH][64],
+    // u8 source region while staging, then hbuf[3][LH][64],
     // the horizontal lerps of the 9-tap Gaussian.
     __shared__ uint32_t stage[kSrcWords];
     uint32_t* const src = stage;
@@ -171,6 +170,8 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     __shared__ uint32_t srow_i0[3][TH], srow_i1[3][TH];
     __shared__ float srow_b[3][TH];
     __shared__ uint32_t row0_score[kTileW + 4];
+    __shared__ float row0_ring[4][17];      // row-0 neighbours: 16 ring texels + centre
+    __shared__ uint64_t mask_lds[TH];       // cull-mask word of each tile row
     __shared__ uint8_t score_lds[kDetectThreads / 64][RPT * 64];  // per wave: RPT rows x 64
     __shared__ uint16_t list_lds[kDetectThreads / 64][RPT * 64];
 
@@ -200,39 +201,77 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
 
     lut[tid] = static_cast<float>(tid) / 255.0f;
 
-    // Per-column / per-row source taps of the staged region.
-    if (tid < kLdsW + LH) {
-        const bool is_col = tid < kLdsW;
-        const uint32_t c = is_col ? tid : tid - kLdsW;
-        const int32_t g = (is_col ? x0 : y0) - kHalo + static_cast<int32_t>(c);
-        const uint32_t n = is_col ? w : h;
-        const bool inside = g >= 0 && g < static_cast<int32_t>(n);
-        const uint32_t cl = prm.border_zero ? (inside ? static_cast<uint32_t>(g) : 0u)
-                                            : tint_clamp(g, n);
-        uint32_t i0 = cl, i1 = cl;
-        float frac = 0.0f;
-        if (l > 0) {
-            const float uv = (static_cast<float>(cl) + 0.5f) / static_cast<float>(n);
-            const AxisTaps tp = axis_taps(uv, is_col ? P.W0 : P.H0);
-            i0 = tp.i0;
-            i1 = tp.i1;
-            frac = tp.frac;
-        }
-        const uint32_t ok = (!prm.border_zero || inside) ? 1u : 0u;
-        if (is_col) {
-            col_i0[c] = i0; col_i1[c] = i1; col_a[c] = frac; col_ok[c] = ok;
+    // Set-up items, spread over the block: per-column / per-row source
+    // taps of the staged region, row taps of the 9-tap Gaussian for the
+    // TH output rows, the cull-mask words of the tile rows and, on row-0
+    // tiles, the texels of the four row-0 neighbours outside the tile
+    // (x0-2, x0-1, x0+64, x0+65) whose scores the 9-aligned windows of
+    // cull_corners.wgsl:36-56 may need.
+    const bool has_row0 = y0 == 0 && tile_x < G.wpr;
+    constexpr uint32_t kTapItems = kLdsW + LH;
+    constexpr uint32_t kSrowItems = kTapItems + 3 * TH;
+    constexpr uint32_t kMaskItems = kSrowItems + TH;
+    const uint32_t n_items = kMaskItems + (has_row0 ? 4 * 17 : 0);
+    for (uint32_t e = tid; e < n_items; e += kDetectThreads) {
+        if (e < kTapItems) {
+            const bool is_col = e < kLdsW;
+            const uint32_t c = is_col ? e : e - kLdsW;
+            const int32_t g = (is_col ? x0 : y0) - kHalo + static_cast<int32_t>(c);
+            const uint32_t n = is_col ? w : h;
+            const bool inside = g >= 0 && g < static_cast<int32_t>(n);
+            const uint32_t cl = prm.border_zero ? (inside ? static_cast<uint32_t>(g) : 0u)
+                                                : tint_clamp(g, n);
+            uint32_t i0 = cl, i1 = cl;
+            float frac = 0.0f;
+            if (l > 0) {
+                const float uv = (static_cast<float>(cl) + 0.5f) / static_cast<float>(n);
+                const AxisTaps tp = axis_taps(uv, is_col ? P.W0 : P.H0);
+                i0 = tp.i0;
+                i1 = tp.i1;
+                frac = tp.frac;
+            }
+            const uint32_t ok = (!prm.border_zero || inside) ? 1u : 0u;
+            if (is_col) {
+                col_i0[c] = i0; col_i1[c] = i1; col_a[c] = frac; col_ok[c] = ok;
+            } else {
+                row_i0[c] = i0; row_i1[c] = i1; row_b[c] = frac; row_ok[c] = ok;
+            }
+        } else if (e < kSrowItems) {
+            const uint32_t q = e - kTapItems;
+            const uint32_t jj = q / TH, rr = q - jj * TH;
+            const float uvy =
+                (static_cast<float>(static_cast<uint32_t>(y0) + rr) + 0.5f) * texel_y;
+            const AxisTaps ay = axis_taps(uvy + offs[jj] * texel_y, h);
+            srow_i0[jj][rr] = ay.i0;
+            srow_i1[jj][rr] = ay.i1;
+            srow_b[jj][rr] = ay.frac;
+        } else if (e < kMaskItems) {
+            const uint32_t rr = e - kSrowItems;
+            const uint32_t y = static_cast<uint32_t>(y0) + rr;
+            mask_lds[rr] = (tile_x < G.wpr && y < h) ? P.mask[G.word_off + y * G.wpr + tile_x]
+                                                      : 0ull;
         } else {
-            row_i0[c] = i0; row_i1[c] = i1; row_b[c] = frac; row_ok[c] = ok;
+            const uint32_t q = e - kMaskItems;
+            const uint32_t k = q / 17, i = q - k * 17;
+            const int32_t xe = k < 2 ? x0 - 2 + static_cast<int32_t>(k)
+                                     : x0 + kTileW + static_cast<int32_t>(k) - 2;
+            float v = 0.0f;
+            if (xe >= 0 && xe < static_cast<int32_t>(w)) {
+                if (i == 16) {
+                    v = level_texel_direct(J, P, G, l, static_cast<uint32_t>(xe), 0);
+                } else {
+                    const int32_t gx = xe + c_circle_x[i], gy = c_circle_y[i];
+                    if (prm.border_zero) {
+                        if (gx >= 0 && gy >= 0 && gx < static_cast<int32_t>(w) &&
+                            gy < static_cast<int32_t>(h))
+                            v = level_texel_direct(J, P, G, l, gx, gy);
+                    } else {
+                        v = level_texel_direct(J, P, G, l, tint_clamp(gx, w), tint_clamp(gy, h));
+                    }
+                }
+            }
+            row0_ring[k][i] = v;
         }
-    } else if (tid < kLdsW + LH + 3 * TH) {
-        // Row taps of the 9-tap Gaussian for the tile's TH output rows.
-        const uint32_t e = tid - (kLdsW + LH);
-        const uint32_t jj = e / TH, rr = e - jj * TH;
-        const float uvy = (static_cast<float>(static_cast<uint32_t>(y0) + rr) + 0.5f) * texel_y;
-        const AxisTaps ay = axis_taps(uvy + offs[jj] * texel_y, h);
-        srow_i0[jj][rr] = ay.i0;
-        srow_i1[jj][rr] = ay.i1;
-        srow_b[jj][rr] = ay.frac;
     }
     __syncthreads();
 
@@ -253,19 +292,34 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     const uint32_t cs = colmin & ~3u;
     const uint32_t nd = (colmax - cs) / 4 + 1;  // dwords per source row
     const uint32_t nrows = rowmax - rowmin + 1;
-    const bool use_src = l > 0 && colmax >= colmin && rowmax >= rowmin &&
+    const bool use_src = colmax >= colmin && rowmax >= rowmin && nd <= 64 &&
                          (J.pitch & 3u) == 0 && (reinterpret_cast<uintptr_t>(J.img) & 3u) == 0 &&
                          nd * nrows <= kSrcWords;
     const uint32_t lane = tid & 63;
     const uint32_t wave = tid >> 6;
     constexpr uint32_t kWaves = kDetectThreads / 64;
     if (use_src) {
-        for (uint32_t rr = wave; rr < nrows; rr += kWaves) {
-            const uint32_t* row = reinterpret_cast<const uint32_t*>(J.img + (rowmin + rr) * J.pitch) + cs / 4;
-            for (uint32_t d = lane; d < nd; d += 64) src[rr * nd + d] = row[d];
+        // Wave v copies rows v, v+4, ...; lane d copies dword d of a row.
+        // Each group of kBatch loads is issued before any of its LDS
+        // stores, so their latencies overlap.
+        constexpr uint32_t kBatch = 8;
+        const uint8_t* col0 = J.img + cs;
+        for (uint32_t r0 = wave; r0 < nrows; r0 += kWaves * kBatch) {
+            uint32_t v[kBatch];
+#pragma unroll
+            for (uint32_t k = 0; k < kBatch; ++k) {
+                const uint32_t rr = r0 + k * kWaves;
+                if (rr < nrows && lane < nd)
+                    v[k] = reinterpret_cast<const uint32_t*>(col0 + (rowmin + rr) * J.pitch)[lane];
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < kBatch; ++k) {
+                const uint32_t rr = r0 + k * kWaves;
+                if (rr < nrows && lane < nd) src[rr * nd + lane] = v[k];
+            }
         }
-        __syncthreads();
     }
+    __syncthreads();
     const uint32_t spitch = nd * 4;
 
     // Stage the level tile ((TH + 6) x 70 texels, flat over the block).
@@ -275,12 +329,15 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
         float v = 0.0f;
         if (col_ok[c] & row_ok[r]) {
             const uint32_t ci0 = col_i0[c];
+            const bool in_src = use_src && c >= c_first && c <= c_last && r >= r_first &&
+                                r <= r_last;
             if (l == 0) {
-                v = lut[J.img[row_i0[r] * J.pitch + ci0]];
+                v = lut[in_src ? src_byte(src, (row_i0[r] - rowmin) * spitch + (ci0 - cs))
+                               : J.img[row_i0[r] * J.pitch + ci0]];
             } else {
                 const uint32_t ci1 = col_i1[c];
                 uint32_t p00, p01, p10, p11;
-                if (use_src && c >= c_first && c <= c_last && r >= r_first && r <= r_last) {
+                if (in_src) {
                     // Byte offsets inside the staged region (ci >= cs, rows
                     // >= rowmin), so every address formed lies in the
                     // region: a pointer before its start (srcb - cs) can
@@ -366,12 +423,6 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     uint16_t* wlist = list_lds[wave];
     const bool quick = prm.min_run >= 9;
     uint32_t nlist = 0;
-    uint64_t mwords[RPT];
-#pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-        const uint32_t y = static_cast<uint32_t>(y0) + wave * RPT + k;
-        mwords[k] = (tile_x < G.wpr && y < h) ? P.mask[G.word_off + y * G.wpr + tile_x] : 0ull;
-    }
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
         const uint32_t r = wave * RPT + k;
@@ -381,7 +432,8 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
         if (x < w && y < h) {
             const float center = tile[r + kHalo][lane + kHalo];
             if (l > 0 && J.pyr) J.pyr[G.pyr_off + y * w + x] = center;
-            if (tile_x < G.wpr) need = y == 0 ? (x % 9) < 3 : ((mwords[k] >> lane) & 1ull) != 0;
+            if (tile_x < G.wpr)
+                need = y == 0 ? (x % 9) < 3 : ((mask_lds[r] >> lane) & 1ull) != 0;
             if (need && quick) {
                 const float p7 = tile[r + kHalo + c_circle_y[7]][lane + kHalo + c_circle_x[7]];
                 const float p8 = tile[r + kHalo + c_circle_y[8]][lane + kHalo + c_circle_x[8]];
@@ -422,16 +474,22 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
 
     // Row 0 of the level keeps only the first maximum of each 3-wide window
     // starting at a multiple of 9 (cull_corners.wgsl:36-56 with x = 9*gid.x).
-    const bool has_row0 = y0 == 0 && tile_x < G.wpr;
     if (has_row0) {
         if (wave == 0) row0_score[2 + lane] = score[0];
         if (tid >= 64 && tid < 68) {
-            const int32_t k = static_cast<int32_t>(tid) - 64;  // 0..3
-            const int32_t xe = k < 2 ? x0 - 2 + k : x0 + kTileW + (k - 2);
-            uint32_t s = 0;
-            if (xe >= 0 && xe < static_cast<int32_t>(w))
-                s = fast_score_from_frame(J, P, G, l, lut, static_cast<uint32_t>(xe), 0, prm);
-            row0_score[k < 2 ? k : kTileW + k] = s;
+            // detect_corners.wgsl:58-81 over the neighbour's staged ring.
+            const uint32_t k = tid - 64;
+            const float center = row0_ring[k][16];
+            uint32_t curr = 1, streak = 1;
+            for (int i = 0; i < 16; ++i) {
+                const bool diff = __builtin_fabsf(center - row0_ring[k][i]) > prm.threshold;
+                curr = diff ? curr + 1 : 1;
+                streak = curr > streak ? curr : streak;
+            }
+            const int32_t xe = k < 2 ? x0 - 2 + static_cast<int32_t>(k)
+                                     : x0 + kTileW + static_cast<int32_t>(k) - 2;
+            const bool valid = xe >= 0 && xe < static_cast<int32_t>(w);
+            row0_score[k < 2 ? k : kTileW + k] = valid && streak > prm.min_run ? streak : 0u;
         }
     }
     __syncthreads();
@@ -442,7 +500,7 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
         const uint32_t r = wave * RPT + k;
         const uint32_t y = static_cast<uint32_t>(y0) + r;
         if (y >= h) continue;  // wave-uniform
-        bool keep = x < w && ((mwords[k] >> lane) & 1ull);
+        bool keep = x < w && ((mask_lds[r] >> lane) & 1ull);
         if (keep) {
             if (y == 0) {
                 const uint32_t s = x - x % 9;
