@@ -164,3 +164,23 @@ def test_launch_shapes_match_oracle(torch_gpu, rows, lanes):
             feats, pyr = ctx.detect(a, with_pyramid=True)
         rf, rp = _oracle(a, W, H, cap, thr20(), bz)
         _assert_same(feats, pyr, rf, rp, 6, cap)
+
+
+@pytest.mark.parametrize("radius,min_run", [(0, 9), (2, 9), (5, 9), (8, 12), (3, 7), (3, 5)])
+def test_orientation_radius_and_min_run_match_oracle(torch_gpu, radius, min_run):
+    """Radii up to 3 read the staged neighbourhood in the describe kernel,
+    larger ones the level itself; min_run below 9 turns off the detect
+    kernel's two-test quick reject.  Both sides of each switch agree with
+    the oracle."""
+    for (W, H, seed, bz) in ((131, 97, 2, 0), (200, 150, 4, 1)):
+        a, _, _ = synth.frame_pair(seed, W, H)
+        cap = max(W * H // 250, 8)
+        _say(f"detect {W}x{H} radius={radius} min_run={min_run} border_zero={bz}")
+        with orbfe.Context(W, H, features_per_level=cap, fast_threshold=thr20(),
+                           border_zero=bz, orientation_radius=radius,
+                           fast_min_run=min_run) as ctx:
+            feats, pyr = ctx.detect(a, with_pyramid=True)
+        p = O.params(W, H, threshold=thr20(), cap=cap, border_zero=bz, radius=radius,
+                     min_run=min_run)
+        rf, rp = O.detect_frame(a, p, with_pyramid=True)
+        _assert_same(feats, pyr, rf, rp, 6, cap)

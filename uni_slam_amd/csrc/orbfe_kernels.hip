@@ -89,20 +89,6 @@ __device__ inline float level_texel_direct(const Job& J, const Plan& P, const Le
                  u8_unit(r1[tx.i1]), tx.frac, ty.frac);
 }
 
-// textureLoad(level l, (gx, gy)) with the robustness rule, from the frame.
-__device__ inline float load_from_frame(const Job& J, const Plan& P, const LevelGeom& G,
-                                        uint32_t l, const float* lut, int32_t gx,
-                                        int32_t gy, uint32_t border_zero) {
-    if (border_zero) {
-        if (gx < 0 || gy < 0 || gx >= static_cast<int32_t>(G.w) ||
-            gy >= static_cast<int32_t>(G.h))
-            return 0.0f;
-        return level_texel_from_frame(J, P, G, l, lut, gx, gy);
-    }
-    return level_texel_from_frame(J, P, G, l, lut, tint_clamp(gx, G.w),
-                                  tint_clamp(gy, G.h));
-}
-
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -668,9 +654,17 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
     constexpr uint32_t RING = LANES < 16 ? 16 / LANES : 1;
     constexpr uint32_t OUTS = (12 + LANES - 1) / LANES; // record words per lane
     constexpr uint64_t GMASK = LANES == 64 ? ~0ull : ((1ull << LANES) - 1ull);
+    // The 7 x 7 neighbourhood of a keypoint covers the FAST ring and, for
+    // orientation radii up to 3, the moment disc.
+    constexpr int32_t kBoxR = 3;
+    constexpr uint32_t kBoxSide = 2 * kBoxR + 1, kBoxN = kBoxSide * kBoxSide;
     __shared__ float lut[256];
+    __shared__ float4 pat[256];          // BRIEF pattern as floats (ax, ay, bx, by)
+    __shared__ float box[KPB][kBoxN];    // per keypoint: level texels around it
     const uint32_t tid = threadIdx.x;
     lut[tid] = static_cast<float>(tid) / 255.0f;
+    pat[tid] = make_float4(c_pattern[tid][0], c_pattern[tid][1], c_pattern[tid][2],
+                           c_pattern[tid][3]);
     __syncthreads();
 
     uint32_t b = blockIdx.x;
@@ -715,17 +709,35 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
     S.h = G.h;
     S.base = l == 0;
 
+    // The group's lanes load the keypoint's 7 x 7 neighbourhood once (with
+    // the textureLoad robustness rule); the ring and the moment disc read it
+    // from LDS.  Radii above 3 read the level directly.
+    const int32_t r = prm.orient_r;
+    const bool boxed = r <= kBoxR;
+    float* bx = box[tid / LANES];
+    if (boxed) {
+        for (uint32_t e = sub; e < kBoxN; e += LANES) {
+            const int32_t dy = static_cast<int32_t>(e / kBoxSide) - kBoxR;
+            const int32_t dx = static_cast<int32_t>(e % kBoxSide) - kBoxR;
+            bx[e] = S.load(static_cast<int32_t>(x) + dx, static_cast<int32_t>(y) + dy,
+                           prm.border_zero);
+        }
+        wave_sync();
+    }
+
     // Corner strength (detect_corners.wgsl:58-81): ring bit i from lane
     // i % LANES.
-    const float center = S.at(x, y);
+    const float center = boxed ? bx[kBoxR * kBoxSide + kBoxR] : S.at(x, y);
     uint32_t ring = 0;
 #pragma unroll
     for (uint32_t q = 0; q < RING; ++q) {
         const uint32_t i = sub + q * LANES;
         bool diff = false;
         if (i < 16) {
-            const float counter = S.load(static_cast<int32_t>(x) + c_circle_x[i],
-                                         static_cast<int32_t>(y) + c_circle_y[i], prm.border_zero);
+            const float counter =
+                boxed ? bx[(kBoxR + c_circle_y[i]) * kBoxSide + kBoxR + c_circle_x[i]]
+                      : S.load(static_cast<int32_t>(x) + c_circle_x[i],
+                               static_cast<int32_t>(y) + c_circle_y[i], prm.border_zero);
             diff = __builtin_fabsf(center - counter) > prm.threshold;
         }
         ring |= static_cast<uint32_t>((__ballot(diff) >> gbase) & GMASK) << (q * LANES);
@@ -739,13 +751,14 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
     const uint32_t strength = streak > prm.min_run ? streak : 0u;
 
     // Orientation (generate_features.wgsl:80-99): sequential moments.
-    const int32_t r = prm.orient_r;
     float m10 = 0.0f, m01 = 0.0f;
     for (int32_t dy = -r; dy <= r; ++dy) {
         for (int32_t dx = -r; dx <= r; ++dx) {
             if (dx * dx + dy * dy > r * r) continue;
-            const float pixel = S.load(static_cast<int32_t>(x) + dx,
-                                       static_cast<int32_t>(y) + dy, prm.border_zero);
+            const float pixel =
+                boxed ? bx[(kBoxR + dy) * static_cast<int32_t>(kBoxSide) + kBoxR + dx]
+                      : S.load(static_cast<int32_t>(x) + dx, static_cast<int32_t>(y) + dy,
+                               prm.border_zero);
             m10 += static_cast<float>(dx) * pixel;
             m01 += static_cast<float>(dy) * pixel;
         }
@@ -763,9 +776,8 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
     uint32_t bits = 0;
 #pragma unroll 4
     for (uint32_t k = 0; k < TPL; ++k) {
-        const uint32_t i = TPL * sub + k;
-        const float pax = c_pattern[i][0], pay = c_pattern[i][1];
-        const float pbx = c_pattern[i][2], pby = c_pattern[i][3];
+        const float4 pt = pat[TPL * sub + k];
+        const float pax = pt.x, pay = pt.y, pbx = pt.z, pby = pt.w;
         const int32_t ax = static_cast<int32_t>(__builtin_rintf(pax * cs - pay * sn));
         const int32_t ay = static_cast<int32_t>(__builtin_rintf(pax * sn + pay * cs));
         const int32_t bx = static_cast<int32_t>(__builtin_rintf(pbx * cs - pby * sn));

@@ -16,6 +16,7 @@ compute each frame exactly as the per-frame launch does.
 from __future__ import annotations
 
 import collections
+import gc
 import time
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence, Tuple
@@ -79,7 +80,7 @@ class Server:
     def __init__(self, specs: Sequence[StreamSpec], device: int = 0,
                  hbm_budget: int = 1 << 30, max_batch: int = 64, inflight: int = 2,
                  fast_threshold: float = 0.15, lowe_ratio: float = 0.85,
-                 keep_outputs: bool = False):
+                 keep_outputs: bool = False, warmup: bool = True, **launch_shape):
         self.dev = torch.device(f"cuda:{device}")
         self.specs = list(specs)
         self.budget = hbm_budget
@@ -89,7 +90,8 @@ class Server:
         self._snap: Dict[Tuple[int, int], Tuple[torch.Tensor, Optional[torch.Tensor]]] = {}
         w0, h0 = self.specs[0].width, self.specs[0].height
         self.ctxs = [orbfe.Context(w0, h0, device=device, fast_threshold=fast_threshold,
-                                   lowe_ratio=lowe_ratio) for _ in range(inflight)]
+                                   lowe_ratio=lowe_ratio, **launch_shape)
+                     for _ in range(inflight)]
         self.hip_streams = [torch.cuda.Stream(device=self.dev) for _ in range(inflight)]
         self.busy: List[Optional[Tuple[torch.cuda.Event, List[_Request], int]]] = [None] * inflight
         self.streams: List[_Stream] = []
@@ -109,6 +111,33 @@ class Server:
         self.in_flight_bytes = 0
         self.max_batch_seen = 0
         self.batches = 0
+        if warmup:
+            self.warmup()
+
+    def warmup(self):
+        """Creates every stream's plan in each in-flight context and grows
+        the contexts' scratch arenas and pinned staging slots to the largest
+        batch admission can form (one frame of every stream), so serving
+        never allocates device or pinned memory.  Stream state is reset."""
+        for ctx, hs in zip(self.ctxs, self.hip_streams):
+            frames, pairs = [], []
+            for st in self.streams:
+                sp = st.spec
+                frames.append({"image": st.upload[0], "width": sp.width, "height": sp.height,
+                               "pitch": sp.width, "cap": sp.cap, "pyramid": st.pyramid,
+                               "features": st.feats[0]})
+                pairs.append({"prev": st.feats[0], "curr": st.feats[1], "prev_match": st.match,
+                              "width": sp.width, "height": sp.height, "cap": sp.cap})
+            with torch.cuda.stream(hs):
+                for _ in range(4):  # each slot of the staging ring
+                    ctx.detect_batch_async(frames, stream=hs)
+                    ctx.match_batch_async(pairs, stream=hs)
+        torch.cuda.synchronize()
+        for st in self.streams:
+            for f in st.feats:
+                f.zero_()
+            st.match.fill_(-1)
+        torch.cuda.synchronize()
 
     # -- requests --------------------------------------------------------------
     def submit(self, stream: int, seq: int, host_frame: torch.Tensor, t_arrival: float):
@@ -225,7 +254,20 @@ def run_schedule(server: Server, arrivals, frame_pool, time_scale: float = 1.0,
                  spin_s: float = 20e-6):
     """Replays arrivals in real time (scaled by time_scale) through server.
     frame_pool[stream] is a list of pinned host frames, used cyclically.
-    Returns (wall seconds, completed list)."""
+    Returns (wall seconds, completed list).  Python's cyclic garbage
+    collector is paused for the run (its pauses would land in the latency
+    tail)."""
+    gc_was_enabled = gc.isenabled()
+    gc.collect()
+    gc.disable()
+    try:
+        return _run(server, arrivals, frame_pool, time_scale, spin_s)
+    finally:
+        if gc_was_enabled:
+            gc.enable()
+
+
+def _run(server, arrivals, frame_pool, time_scale, spin_s):
     t0 = time.perf_counter()
     i = 0
     n = len(arrivals)
