@@ -44,12 +44,41 @@ __device__ inline void wave_sync() {
     asm volatile("" ::: "memory");
 }
 
+// Plans, job tables and prefix tables do not change while a kernel runs and
+// are read at workgroup-uniform addresses: reading them through the
+// constant address space lets the compiler use scalar loads.
+#define ORBFE_CONST __attribute__((address_space(4)))
+
+template <typename T>
+__device__ inline const ORBFE_CONST T* const_ptr(const T* p) {
+    return reinterpret_cast<const ORBFE_CONST T*>(reinterpret_cast<uintptr_t>(p));
+}
+
+template <typename T>
+__device__ inline const ORBFE_CONST T& as_const(const T* p) {
+    return *const_ptr(p);
+}
+
+// Copy of a table entry, dword by dword (the loads stay scalar).
+template <typename T>
+__device__ inline T copy_const(const ORBFE_CONST T* p) {
+    static_assert(sizeof(T) % 4 == 0, "dword-sized entries");
+    T out;
+    const ORBFE_CONST uint32_t* src = reinterpret_cast<const ORBFE_CONST uint32_t*>(p);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+    for (uint32_t i = 0; i < sizeof(T) / 4; ++i) dst[i] = src[i];
+    return out;
+}
+
+using ConstPlan = const ORBFE_CONST Plan&;
+
 __device__ inline uint32_t find_job(const uint32_t* prefix, uint32_t n, uint32_t b) {
     // largest j with prefix[j] <= b (prefix[0] = 0, prefix[n] = total)
     uint32_t lo = 0, hi = n;
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (prefix[mid] <= b) lo = mid; else hi = mid;
+        if (as_const(prefix + mid) <= b) lo = mid; else hi = mid;
     }
     return lo;
 }
@@ -57,7 +86,7 @@ __device__ inline uint32_t find_job(const uint32_t* prefix, uint32_t n, uint32_t
 // Level-l texel (x, y) computed from the u8 frame: level 0 is float(u8)/255
 // (data/src/provider_base.cpp:121-124); level l >= 1 is the bilinear sample of
 // level 0 at the texel centre (create_pyramid.wgsl:5-14).  x, y in range.
-__device__ inline float level_texel_from_frame(const Job& J, const Plan& P,
+__device__ inline float level_texel_from_frame(const Job& J, ConstPlan P,
                                                const LevelGeom& G, uint32_t l,
                                                const float* lut, uint32_t x,
                                                uint32_t y) {
@@ -76,7 +105,7 @@ __device__ inline float level_texel_from_frame(const Job& J, const Plan& P,
 __device__ inline float u8_unit(uint32_t v) { return static_cast<float>(v) / 255.0f; }
 
 // level_texel_from_frame without the LDS table (usable before it is filled).
-__device__ inline float level_texel_direct(const Job& J, const Plan& P, const LevelGeom& G,
+__device__ inline float level_texel_direct(const Job& J, ConstPlan P, const LevelGeom& G,
                                            uint32_t l, uint32_t x, uint32_t y) {
     if (l == 0) return u8_unit(J.img[y * J.pitch + x]);
     const float u = (static_cast<float>(x) + 0.5f) / static_cast<float>(G.w);
@@ -112,7 +141,7 @@ __device__ inline uint32_t src_byte(const uint32_t* src, uint32_t i) {
 // Smoothed value at (x, y) evaluated directly from the frame (tiles whose
 // sampler taps fall outside the staged region: only at the far corner of
 // the extended grid).
-__device__ float smooth_direct(const Job& J, const Plan& P, const LevelGeom& G, uint32_t l,
+__device__ float smooth_direct(const Job& J, ConstPlan P, const LevelGeom& G, uint32_t l,
                                const float* lut, uint32_t x, uint32_t y) {
     const float offs[3] = {-1.2f, 0.0f, 1.2f};
     const float wts[3] = {0.3125f, 0.375f, 0.3125f};
@@ -144,9 +173,11 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     constexpr uint32_t kSrcWords = stage_words<TH>();
     __shared__ float lut[256];
     __shared__ float tile[LH][kLdsW];
-    // u8 source region while staging, then hbuf[3][LH][64],
-    // the horizontal lerps of the 9-tap Gaussian.
+    // u8 source region while staging, then hbuf[3][LH][64], the horizontal
+    // lerps of the 9-tap Gaussian, then the corner scores and candidate
+    // lists of the FAST pass (per wave RPT x 64 bytes and RPT x 64 u16).
     __shared__ uint32_t stage[kSrcWords];
+    static_assert(kDetectThreads / 64 * RPT * 64 * 3 <= kSrcWords * 4, "FAST scratch fits");
     uint32_t* const src = stage;
     __shared__ uint32_t col_i0[kLdsW], col_i1[kLdsW];
     __shared__ float col_a[kLdsW];
@@ -158,21 +189,19 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     __shared__ uint32_t row0_score[kTileW + 4];
     __shared__ float row0_ring[4][17];      // row-0 neighbours: 16 ring texels + centre
     __shared__ uint64_t mask_lds[TH];       // cull-mask word of each tile row
-    __shared__ uint8_t score_lds[kDetectThreads / 64][RPT * 64];  // per wave: RPT rows x 64
-    __shared__ uint16_t list_lds[kDetectThreads / 64][RPT * 64];
 
     const uint32_t tid = threadIdx.x;
     uint32_t b = blockIdx.x;
     uint32_t j = 0;
     if (bv.n) {
         j = find_job(bv.tile_prefix, bv.n, b);
-        b -= bv.tile_prefix[j];
+        b -= as_const(bv.tile_prefix + j);
     }
-    const Job J = bv.n ? bv.jobs[j] : job0;
-    const Plan& P = *J.plan;
+    const Job J = bv.n ? copy_const(const_ptr(bv.jobs + j)) : job0;
+    ConstPlan P = as_const(J.plan);
     uint32_t l = 0;
     while (l + 1 < P.L && b >= P.lv[l + 1].tile_base) ++l;
-    const LevelGeom& G = P.lv[l];
+    const LevelGeom G = copy_const(&P.lv[l]);  // by value: stores below cannot alias it
     const uint32_t t = b - G.tile_base;
     const uint32_t tile_y = t / G.tiles_x;
     const uint32_t tile_x = t - tile_y * G.tiles_x;
@@ -405,8 +434,10 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     // and 8, so for min_run >= 9 a pixel failing either of those two tests
     // scores 0; the rest are gathered into a per-wave list and get the full
     // ring.  Scores land in score_lds (0 elsewhere).
-    uint8_t* wscore = score_lds[wave];
-    uint16_t* wlist = list_lds[wave];
+    __syncthreads();  // hbuf no longer read: its words become the FAST scratch
+    uint8_t* wscore = reinterpret_cast<uint8_t*>(stage) + wave * RPT * 64;
+    uint16_t* wlist = reinterpret_cast<uint16_t*>(reinterpret_cast<uint8_t*>(stage) +
+                                                  kDetectThreads * RPT) + wave * RPT * 64;
     const bool quick = prm.min_run >= 9;
     uint32_t nlist = 0;
 #pragma unroll
@@ -551,9 +582,9 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(BatchView bv, Job j
     const uint32_t wave = tid >> 6;
     const uint32_t j = blockIdx.x / levels;
     const uint32_t l = blockIdx.x - j * levels;
-    const Job J = bv.n ? bv.jobs[j] : job0;
-    const Plan& P = *J.plan;
-    const LevelGeom& G = P.lv[l];
+    const Job J = bv.n ? copy_const(const_ptr(bv.jobs + j)) : job0;
+    ConstPlan P = as_const(J.plan);
+    const LevelGeom G = copy_const(&P.lv[l]);  // by value: stores below cannot alias it
     const uint32_t words = G.h * G.wpr;
     const uint64_t* surv = J.surv + G.word_off;
     const uint32_t seg = (words + kCompactWaves - 1) / kCompactWaves;
@@ -671,17 +702,17 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
     uint32_t j = 0;
     if (bv.n) {
         j = find_job(bv.desc_prefix, bv.n, b);
-        b -= bv.desc_prefix[j];
+        b -= as_const(bv.desc_prefix + j);
     }
-    const Job J = bv.n ? bv.jobs[j] : job0;
-    const Plan& P = *J.plan;
+    const Job J = bv.n ? copy_const(const_ptr(bv.jobs + j)) : job0;
+    ConstPlan P = as_const(J.plan);
     const uint32_t l = b / P.desc_groups;
     const uint32_t g = b - l * P.desc_groups;
     const uint32_t lane = tid & 63;
     const uint32_t sub = lane & (LANES - 1);   // lane within the keypoint group
     const uint32_t gbase = lane & ~(LANES - 1);  // first lane of the group
     const uint32_t slot = g * KPB + tid / LANES;
-    const LevelGeom& G = P.lv[l];
+    const LevelGeom G = copy_const(&P.lv[l]);  // by value: stores below cannot alias it
     uint8_t* block = J.feat + G.feat_off;
     const uint32_t count = reinterpret_cast<const uint32_t*>(block)[0];
     const uint32_t kept = count < P.cap ? count : P.cap;
@@ -858,13 +889,13 @@ __device__ inline uint32_t cell_of(float v, float inv) {
 __global__ __launch_bounds__(256) void k_match_index(MatchBatchView bv, MatchJob job0) {
     __shared__ uint32_t hist[kCells + 1];
     const uint32_t tid = threadIdx.x;
-    const MatchJob M = bv.n ? bv.jobs[0] : job0;  // layout constants only
-    const uint32_t L = M.plan->L;
+    const MatchJob M = bv.n ? copy_const(const_ptr(bv.jobs)) : job0;  // layout constants only
+    const uint32_t L = as_const(M.plan).L;
     const uint32_t j = blockIdx.x / (2 * L);
     const uint32_t side = (blockIdx.x / L) & 1;
     const uint32_t l = blockIdx.x % L;
-    const MatchJob J = bv.n ? bv.jobs[j] : job0;
-    const Plan& P = *J.plan;
+    const MatchJob J = bv.n ? copy_const(const_ptr(bv.jobs + j)) : job0;
+    ConstPlan P = as_const(J.plan);
     const FeatView A{side ? J.curr : J.prev, P.feat_stride};
     uint32_t* out = (side ? J.index_curr : J.index_prev) + l * P.idx_stride;
     const uint32_t n = A.count(l, P.cap);
@@ -918,10 +949,10 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
     uint32_t j = 0;
     if (bv.n) {
         j = find_job(bv.block_prefix, bv.n, b);
-        b -= bv.block_prefix[j];
+        b -= as_const(bv.block_prefix + j);
     }
-    const MatchJob M = bv.n ? bv.jobs[j] : job0;
-    const Plan& P = *M.plan;
+    const MatchJob M = bv.n ? copy_const(const_ptr(bv.jobs + j)) : job0;
+    ConstPlan P = as_const(M.plan);
     const uint32_t l = b / P.query_groups;
     const uint32_t g = b - l * P.query_groups;
     const uint32_t slot = g * kQueriesPerBlock + wave;
