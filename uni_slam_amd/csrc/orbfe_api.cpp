@@ -261,7 +261,13 @@ int get_plan(orbfe_ctx* ctx, uint32_t W0, uint32_t H0, uint32_t cap, PlanEntry**
     P.max_dy = 0.1f * static_cast<float>(H0);
     P.cell_inv_x = 2.0f / P.max_dx;
     P.cell_inv_y = 2.0f / P.max_dy;
-    P.idx_stride = orbfe::kCells + 1 + cap;
+    {
+        const auto al4 = [](uint32_t v) { return (v + 3u) & ~3u; };  // 16-byte aligned
+        P.idx_xy = al4(orbfe::kCells + 1);
+        P.idx_desc = al4(P.idx_xy + 2 * cap);
+        P.idx_pos = P.idx_desc + 8 * cap;
+        P.idx_stride = al4(P.idx_pos + cap);
+    }
     uint32_t tiles = 0, words = 0, pyr = 0, sm = 0;
     float scale = 1.0f;
     for (uint32_t l = 0; l < L; ++l) {

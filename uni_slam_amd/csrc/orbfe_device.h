@@ -83,7 +83,12 @@ struct Plan {
     float max_dx, max_dy; // 0.1f*W0, 0.1f*H0 (match_features_cpu.cpp:76-77)
     uint32_t sm_floats;   // smoothed-image floats of one frame
     float cell_inv_x, cell_inv_y;  // 1 / cell size (cells of max_dx/2 x max_dy/2)
-    uint32_t idx_stride;  // u32 per level of a cell index: kCells + 1 + cap
+    // Cell index of one level (u32 words from the level's base): cell
+    // starts [kCells + 1], then the level's records sorted by cell as
+    // arrays of LOD0 positions (float2), descriptors (2 x uint4) and record
+    // indices (u32).
+    uint32_t idx_stride;  // words per level
+    uint32_t idx_xy, idx_desc, idx_pos;  // word offsets of the three arrays
     const uint64_t* mask; // static cull mask, `words` words
     LevelGeom lv[kMaxLevels];
 };

@@ -869,6 +869,9 @@ struct FeatView {
 }  // namespace
 
 constexpr uint32_t kMatchThreads = 256;  // 4 waves = 4 queries
+// Candidates a lane loads per round (their loads are issued together; more
+// would cost occupancy).
+constexpr uint32_t kCandPerLane = 2;
 
 // LOD0 position of a record (toLod0, match_features_cpu.cpp:58-64).
 __device__ inline void rec_lod0(const uint32_t* r, float scale, float* x, float* y) {
@@ -883,9 +886,11 @@ __device__ inline uint32_t cell_of(float v, float inv) {
 }
 
 // k_match_index: per (pair, side, level) a counting sort of the level's
-// records by LOD0 cell: cell_start[kCells + 1] then record indices.  Cell
-// order is monotone in the coordinates, so every record inside a search
-// window lies in the cells the window's (margin-widened) corners map to.
+// records by LOD0 cell: cell_start[kCells + 1], then the records in cell
+// order as arrays of LOD0 positions, descriptors and record indices, so a
+// cell range of candidates is read with contiguous loads.  Cell order is
+// monotone in the coordinates, so every record inside a search window lies
+// in the cells the window's (margin-widened) corners map to.
 __global__ __launch_bounds__(256) void k_match_index(MatchBatchView bv, MatchJob job0) {
     __shared__ uint32_t hist[kCells + 1];
     const uint32_t tid = threadIdx.x;
@@ -928,11 +933,19 @@ __global__ __launch_bounds__(256) void k_match_index(MatchBatchView bv, MatchJob
         if (tid == 63) out[kCells] = run;
     }
     __syncthreads();
+    float2* sxy = reinterpret_cast<float2*>(out + P.idx_xy);
+    uint4* sdesc = reinterpret_cast<uint4*>(out + P.idx_desc);
+    uint32_t* spos = out + P.idx_pos;
     for (uint32_t i = tid; i < n; i += 256) {
+        const uint32_t* r = A.rec(l, i);
         float x, y;
-        rec_lod0(A.rec(l, i), sc, &x, &y);
+        rec_lod0(r, sc, &x, &y);
         const uint32_t c = cell_of(y, P.cell_inv_y) * kCellsPerAxis + cell_of(x, P.cell_inv_x);
-        out[kCells + 1 + atomicAdd(&hist[c], 1u)] = i;
+        const uint32_t k = atomicAdd(&hist[c], 1u);
+        sxy[k] = make_float2(x, y);
+        sdesc[2 * k] = *reinterpret_cast<const uint4*>(r + 4);
+        sdesc[2 * k + 1] = *reinterpret_cast<const uint4*>(r + 8);
+        spos[k] = i;
     }
 }
 
@@ -1017,11 +1030,14 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
             r_off[lane] = incl - len;
             r_src[lane] = a | (sk << 30);
             wave_sync();
-            for (uint32_t t0 = 0; t0 < total_c; t0 += 256) {
-                uint32_t ent[4];
-                const uint32_t* crs[4];
+            for (uint32_t t0 = 0; t0 < total_c; t0 += 64 * kCandPerLane) {
+                // Up to kCandPerLane candidates per lane: sorted position,
+                // then position, descriptor and index in one batch of loads.
+                uint32_t ent[kCandPerLane];
+                float2 cxy[kCandPerLane];
+                uint4 dlo[kCandPerLane], dhi[kCandPerLane];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (uint32_t u = 0; u < kCandPerLane; ++u) {
                     const uint32_t t = t0 + lane + 64 * u;
                     ent[u] = 0xFFFFFFFFu;
                     if (t < total_c) {
@@ -1034,23 +1050,23 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
                         const uint32_t src = r_src[lo];
                         const uint32_t s2 = src >> 30;
                         const uint32_t cl = s2 == 0 ? pl[0] : (s2 == 1 ? pl[1] : pl[2]);
-                        const uint32_t i = cidx[cl * P.idx_stride + kCells + 1 +
-                                                (src & 0x3FFFFFFFu) + (t - r_off[lo])];
-                        ent[u] = (s2 << kMatchIdxBits) | i;
-                        crs[u] = C.rec(cl, i);
+                        const uint32_t* base = cidx + cl * P.idx_stride;
+                        const uint32_t k = (src & 0x3FFFFFFFu) + (t - r_off[lo]);
+                        cxy[u] = reinterpret_cast<const float2*>(base + P.idx_xy)[k];
+                        dlo[u] = reinterpret_cast<const uint4*>(base + P.idx_desc)[2 * k];
+                        dhi[u] = reinterpret_cast<const uint4*>(base + P.idx_desc)[2 * k + 1];
+                        ent[u] = (s2 << kMatchIdxBits) | base[P.idx_pos + k];
                     }
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (uint32_t u = 0; u < kCandPerLane; ++u) {
                     if (ent[u] == 0xFFFFFFFFu) continue;
-                    const uint32_t s2 = ent[u] >> kMatchIdxBits;
-                    const uint32_t cl = s2 == 0 ? pl[0] : (s2 == 1 ? pl[1] : pl[2]);
-                    float cx, cy;
-                    rec_lod0(crs[u], P.lv[cl].lod0_scale, &cx, &cy);
+                    const float cx = cxy[u].x, cy = cxy[u].y;
                     if (cx < lo_x || cx > hi_x || __builtin_fabsf(qy - cy) > dy) continue;
-                    uint32_t d = 0;
-#pragma unroll
-                    for (int q = 0; q < 8; ++q) d += __popc(qd[q] ^ crs[u][4 + q]);
+                    const uint32_t d = __popc(qd[0] ^ dlo[u].x) + __popc(qd[1] ^ dlo[u].y) +
+                                       __popc(qd[2] ^ dlo[u].z) + __popc(qd[3] ^ dlo[u].w) +
+                                       __popc(qd[4] ^ dhi[u].x) + __popc(qd[5] ^ dhi[u].y) +
+                                       __popc(qd[6] ^ dhi[u].z) + __popc(qd[7] ^ dhi[u].w);
                     const uint32_t key = (d << kMatchKeyShift) | ent[u];
                     min2 = umax(min1, umin(min2, key));
                     min1 = umin(min1, key);

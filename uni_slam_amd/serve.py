@@ -30,15 +30,23 @@ LEVELS = 6
 
 
 def frame_scratch_bytes(width: int, height: int, cap: int) -> int:
-    """Device scratch one in-flight frame uses in a batch (survivor bitmap,
-    keypoint list, smoothed image), as laid out by liborbfe."""
+    """Device scratch one in-flight frame uses in a batch, as laid out by
+    liborbfe: survivor bitmap, keypoint list and smoothed image of the
+    detect stage; match table and the two sides' cell indices (cell starts
+    and the records sorted by cell) of the match stage."""
     dims = orbfe.level_dims(width, height)
     words = sum(h * ((w + 63) // 64) for w, h in dims)
     smooth = sum((w + 4) * (h + 4) for w, h in dims)
 
     def al(v):
         return (v + 255) // 256 * 256
-    return al(8 * words) + al(4 * LEVELS * cap) + al(4 * smooth) + al(4 * LEVELS * cap)
+
+    def al4(v):
+        return (v + 3) // 4 * 4
+    idx_words = al4(al4(al4(24 * 24 + 1) + 2 * cap) + 8 * cap + cap)
+    detect = al(8 * words) + al(4 * LEVELS * cap) + al(4 * smooth)
+    match = al(4 * LEVELS * cap) + 2 * al(4 * LEVELS * idx_words)
+    return detect + match
 
 
 @dataclass
