@@ -70,6 +70,21 @@ def measured_traffic(kernel):
         return None
 
 
+def measured_valu_issue(kernel):
+    """VALU issue rate of `kernel` as a fraction of the MI355X peak (one
+    wave64 VALU instruction per CU per clock) on this workload, plain and
+    batched, from the committed rocprofv3 SQ_INSTS_VALU summary; None when
+    no summary covers it."""
+    path = os.path.join(ROOT, "profiles", "valu_1280x720.json")
+    try:
+        with open(path) as f:
+            v = json.load(f)
+        return {"plain": v["plain"][kernel]["util"], "batched": v["batched"][kernel]["util"],
+                "source": "profiles/valu_1280x720.json"}
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(frames, n_sample):
     """The oracle (scalar C++, one thread) on the first n_sample frames of
     the sequence: detect each and match consecutive pairs."""
@@ -116,7 +131,7 @@ def timed(torch, fn, reps):
 
 
 def phase_batched_sequence(torch, orbfe, ctx, d_frames, feats, matches, reps):
-    """The config-1 sequence as one prepared batch (32 detects) + one
+    """The 1280x720 32-frame sequence as one prepared batch (32 detects) + one
     prepared match batch (31 pairs), captured in a HIP graph."""
     frames = [{"image": d_frames[t], "width": W, "height": H, "pitch": W, "cap": CAP,
                "features": feats[t]} for t in range(N_FRAMES)]
@@ -135,7 +150,7 @@ def phase_batched_sequence(torch, orbfe, ctx, d_frames, feats, matches, reps):
 
 
 def phase_mixed(torch, orbfe, synth, reps):
-    """Config 2: 64 frames of random resolution and cap, each matched against
+    """Mixed-resolution batch: 64 frames of random resolution and cap, each matched against
     its translated twin; plain per-frame launches vs one prepared batch per
     stage in a HIP graph, its launch shapes picked by the autotuner."""
     from uni_slam_amd import autotune
@@ -192,7 +207,7 @@ def phase_mixed(torch, orbfe, synth, reps):
 
 
 def phase_matcher(torch, orbfe, synth, n, reps):
-    """Config 3: brute-force best/second-best over n x n 256-bit descriptors."""
+    """Brute-force matcher stress: best/second-best over n x n 256-bit descriptors."""
     q, t, _, _ = synth.descriptors(n, n, n)
     dev = torch.device("cuda:0")
     dq = torch.from_numpy(q).to(dev)
@@ -222,7 +237,7 @@ def phase_matcher(torch, orbfe, synth, n, reps):
 
 
 def phase_serving(torch, orbfe, synth, seconds, seed=4):
-    """Config 4: 32 streams at mixed resolutions, Poisson arrivals at
+    """Serving: 32 streams at mixed resolutions, Poisson arrivals at
     20-60 fps each, continuous batching on two in-flight slots."""
     from uni_slam_amd import serve
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -267,6 +282,10 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device-to-device copy (counter calibration)")
+    ap.add_argument("--workload", default="plain",
+                    choices=["plain", "batched", "mixed", "matcher", "serving"],
+                    help="plain: the full report; the others run only that phase and "
+                         "print its JSON (for profiling one path)")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -295,6 +314,19 @@ def main():
     matches = [torch.zeros(LEVELS * CAP, dtype=torch.int32, device=dev)
                for _ in range(N_FRAMES - 1)]
     stream = torch.cuda.current_stream()
+
+    if args.workload != "plain":
+        if args.workload == "batched":
+            out, _keep = phase_batched_sequence(torch, orbfe, ctx, d_frames, feats, matches,
+                                                max(2, args.steps))
+        elif args.workload == "mixed":
+            out = phase_mixed(torch, orbfe, synth, max(2, args.steps))
+        elif args.workload == "matcher":
+            out = phase_matcher(torch, orbfe, synth, 65536, max(2, args.steps))
+        else:
+            out = phase_serving(torch, orbfe, synth, args.serve_seconds)
+        print(json.dumps({"workload": args.workload, **out}), flush=True)
+        return
 
     def step():
         for t in range(N_FRAMES):
@@ -340,6 +372,7 @@ def main():
     roofline = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
                 "traffic": measured_traffic(dom),
+                "valu_issue_frac": measured_valu_issue(dom),
                 "avg_us": per_kernel[dom]["avg_us"], "bytes_per_launch": kb[dom]}
 
     extra = {}
@@ -370,7 +403,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u8/f32/u32",
         "data": "synthetic (seeded value noise + rectangles + N(0,3) noise)",
-        "config": {"workload": "config 1: plain per-frame path, 1280x720, 32-frame shifted "
+        "config": {"workload": "plain per-frame path: 1280x720, 32-frame shifted "
                                "sequence, cap 2000/level, threshold 20/255, ratio 0.8",
                    "frames_per_step": N_FRAMES, "resolution": [W, H], "cap": CAP,
                    "parallelism": "single GPU"},

@@ -2,7 +2,7 @@
 
 * golden.json: SHA-256 of the seeded inputs and of the oracle's outputs at
   the benchmark configurations (config-0 pair at 752x480 cap 1000 threshold
-  20/255; frame 0 and 1 of the config-1 sequence at 1280x720 cap 2000; the
+  20/255; frame 0 and 1 of the shifted 1280x720 sequence at cap 2000; the
   pipeline match of each pair with ratio 0.8), plus per-level counts.
 * small.npz: a 96x64 pair with every output in full (feature arrays,
   pyramids, match map), small enough to diff by eye.
@@ -56,10 +56,10 @@ def entry(a, b, W, H, cap, ratio):
 def main():
     out = {}
     a, b, shift = synth.frame_pair(0, 752, 480)
-    out["config0_pair"] = entry(a, b, 752, 480, 1000, 0.8)
-    out["config0_pair"]["shift"] = list(shift)
+    out["pair_752x480"] = entry(a, b, 752, 480, 1000, 0.8)
+    out["pair_752x480"]["shift"] = list(shift)
     frames, _ = synth.sequence(1, 1280, 720, n_frames=2)
-    out["config1_frames01"] = entry(frames[0], frames[1], 1280, 720, 2000, 0.8)
+    out["seq_1280x720_frames01"] = entry(frames[0], frames[1], 1280, 720, 2000, 0.8)
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
 

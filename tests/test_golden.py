@@ -25,7 +25,7 @@ def golden():
 
 
 def inputs(name):
-    if name == "config0_pair":
+    if name == "pair_752x480":
         a, b, _ = synth.frame_pair(0, 752, 480)
     else:
         frames, _ = synth.sequence(1, 1280, 720, n_frames=2)
@@ -33,7 +33,7 @@ def inputs(name):
     return a, b
 
 
-@pytest.mark.parametrize("name", ["config0_pair", "config1_frames01"])
+@pytest.mark.parametrize("name", ["pair_752x480", "seq_1280x720_frames01"])
 def test_generator_and_oracle_reproduce_golden(name):
     g = golden()[name]
     a, b = inputs(name)
@@ -58,7 +58,7 @@ def test_oracle_reproduces_small_fixture():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["config0_pair", "config1_frames01"])
+@pytest.mark.parametrize("name", ["pair_752x480", "seq_1280x720_frames01"])
 def test_gpu_reproduces_golden(torch_gpu, name):
     g = golden()[name]
     a, b = inputs(name)

@@ -47,11 +47,11 @@ def test_detect_small_frames(torch_gpu, W, H, seed, border_zero):
         _assert_same(feats, pyr, rf, rp, 6, cap)
 
 
-def test_detect_config0_pair(torch_gpu):
+def test_detect_pair_752x480(torch_gpu):
     a, b, _ = synth.frame_pair(0, 752, 480)
     with orbfe.Context(752, 480, features_per_level=1000, fast_threshold=thr20()) as ctx:
         for name, img in (("A", a), ("B", b)):
-            _say(f"detect config-0 frame {name} 752x480 cap 1000")
+            _say(f"detect pair frame {name} 752x480 cap 1000")
             feats, pyr = ctx.detect(img, with_pyramid=True)
             rf, rp = _oracle(img, 752, 480, 1000, thr20())
             _assert_same(feats, pyr, rf, rp, 6, 1000)
@@ -67,11 +67,11 @@ def test_detect_reference_default_cap_and_threshold(torch_gpu):
     _assert_same(feats, pyr, rf, rp, 6, 1443)
 
 
-def test_detect_config1_frames(torch_gpu):
+def test_detect_sequence_1280x720(torch_gpu):
     frames, _ = synth.sequence(11, 1280, 720, n_frames=2)
     with orbfe.Context(1280, 720, features_per_level=2000, fast_threshold=thr20()) as ctx:
         for t in range(2):
-            _say(f"detect config-1 frame {t} 1280x720 cap 2000")
+            _say(f"detect sequence frame {t} 1280x720 cap 2000")
             feats, pyr = ctx.detect(frames[t], with_pyramid=True)
             rf, rp = _oracle(frames[t], 1280, 720, 2000, thr20())
             _assert_same(feats, pyr, rf, rp, 6, 2000)

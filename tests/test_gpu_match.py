@@ -23,11 +23,11 @@ def _oracle_pipeline(fa, fb, W, H, cap, ratio):
 
 
 @pytest.mark.parametrize("ratio", [0.85, 0.8])
-def test_match_pipeline_config0_pair(torch_gpu, ratio):
+def test_match_pipeline_pair_752x480(torch_gpu, ratio):
     a, b, _ = synth.frame_pair(0, 752, 480)
     with orbfe.Context(752, 480, features_per_level=1000, fast_threshold=thr20(),
                        lowe_ratio=ratio) as ctx:
-        _say(f"detect + match config-0 pair, ratio {ratio}")
+        _say(f"detect + match 752x480 pair, ratio {ratio}")
         fa, _ = ctx.detect(a)
         fb, _ = ctx.detect(b)
         got, n = ctx.match_pipeline(fa, fb)
@@ -36,21 +36,21 @@ def test_match_pipeline_config0_pair(torch_gpu, ratio):
     assert np.array_equal(got, want)
 
 
-def test_match_pipeline_config1_sequence_device(torch_gpu):
+def test_match_pipeline_sequence_1280x720_device(torch_gpu):
     torch = torch_gpu
     frames, _ = synth.sequence(11, 1280, 720, n_frames=3)
     with orbfe.Context(1280, 720, features_per_level=2000, fast_threshold=thr20(),
                        lowe_ratio=0.8) as ctx:
         feats = []
         for t in range(3):
-            _say(f"detect_async config-1 frame {t}")
+            _say(f"detect_async 1280x720 frame {t}")
             d_img = torch.from_numpy(frames[t]).cuda()
             d_f = torch.zeros(ctx.feature_bytes, dtype=torch.uint8, device="cuda")
             ctx.detect_async(d_img, 1280, d_f)
             feats.append(d_f)
         outs = []
         for t in range(1, 3):
-            _say(f"match_pipeline_async config-1 frames {t - 1}->{t}")
+            _say(f"match_pipeline_async 1280x720 frames {t - 1}->{t}")
             pm = torch.full((6 * ctx.cap,), -7, dtype=torch.int32, device="cuda")
             ctx.match_pipeline_async(feats[t - 1], feats[t], pm)
             outs.append(pm)
