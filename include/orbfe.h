@@ -63,10 +63,12 @@ typedef struct orbfe_config {
     /* Launch shapes (no counterpart in the reference; results do not depend
        on them).  0 = default.  detect_tile_rows: rows of a detect tile, 8, 16
        (default) or 32.  describe_lanes: lanes per keypoint in the describe
-       kernel, 8, 16 (default) or 32.  uni_slam_amd/autotune.py picks them by
-       timing the variants on a workload. */
+       kernel, 8, 16 (default) or 32.  match_lanes: lanes per query in the
+       pipeline matcher, 16 (default), 32 or 64.  uni_slam_amd/autotune.py
+       picks them by timing the variants on a workload. */
     uint32_t detect_tile_rows;
     uint32_t describe_lanes;
+    uint32_t match_lanes;
 } orbfe_config;
 
 typedef struct orbfe_gpu_feature {

@@ -22,12 +22,12 @@ def _oracle_pipeline(fa, fb, W, H, cap, ratio):
     return O.prev_match_to_padded(pm, ca, cap), n
 
 
-@pytest.mark.parametrize("ratio", [0.85, 0.8])
-def test_match_pipeline_pair_752x480(torch_gpu, ratio):
+@pytest.mark.parametrize("ratio,lanes", [(0.85, 16), (0.8, 16), (0.8, 32), (0.8, 64)])
+def test_match_pipeline_pair_752x480(torch_gpu, ratio, lanes):
     a, b, _ = synth.frame_pair(0, 752, 480)
     with orbfe.Context(752, 480, features_per_level=1000, fast_threshold=thr20(),
-                       lowe_ratio=ratio) as ctx:
-        _say(f"detect + match 752x480 pair, ratio {ratio}")
+                       lowe_ratio=ratio, match_lanes=lanes) as ctx:
+        _say(f"detect + match 752x480 pair, ratio {ratio}, {lanes} lanes per query")
         fa, _ = ctx.detect(a)
         fb, _ = ctx.detect(b)
         got, n = ctx.match_pipeline(fa, fb)
@@ -94,9 +94,10 @@ def test_match_pipeline_synthetic_sets(torch_gpu, seed):
         return buf
 
     fa, fb = make(), make()
-    for ratio in (0.85, 1.0):
-        with orbfe.Context(W, H, features_per_level=cap, lowe_ratio=ratio) as ctx:
-            _say(f"match synthetic sets seed {seed} ratio {ratio}")
+    for ratio, lanes in ((0.85, 16), (1.0, 16), (1.0, 32), (0.85, 64)):
+        with orbfe.Context(W, H, features_per_level=cap, lowe_ratio=ratio,
+                           match_lanes=lanes) as ctx:
+            _say(f"match synthetic sets seed {seed} ratio {ratio} lanes {lanes}")
             got, n = ctx.match_pipeline(fa, fb)
         want, wn = _oracle_pipeline(fa, fb, W, H, cap, ratio)
         assert np.array_equal(got, want)

@@ -33,7 +33,7 @@ def test_config_default_matches_reference_constants():
     assert np.float32(cfg.fast_threshold) == np.float32(0.15)
     assert np.float32(cfg.lowe_ratio) == np.float32(0.85)
     assert cfg.features_per_level == 0  # W*H/250 chosen at create time
-    assert (cfg.detect_tile_rows, cfg.describe_lanes) == (0, 0)  # library defaults
+    assert (cfg.detect_tile_rows, cfg.describe_lanes, cfg.match_lanes) == (0, 0, 0)  # defaults
 
 
 def test_create_rejects_unsupported_launch_shapes():
@@ -41,7 +41,7 @@ def test_create_rejects_unsupported_launch_shapes():
     without a GPU."""
     import ctypes as C
     for bad in ({"detect_tile_rows": 12}, {"detect_tile_rows": 64}, {"describe_lanes": 4},
-                {"describe_lanes": 64}):
+                {"describe_lanes": 64}, {"match_lanes": 8}, {"match_lanes": 128}):
         cfg = orbfe.default_config(320, 240, **bad)
         h = C.c_void_p()
         assert orbfe.lib().orbfe_create(C.byref(cfg), 0, C.byref(h)) == orbfe.ERANGE, bad

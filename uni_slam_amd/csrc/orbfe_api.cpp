@@ -27,7 +27,7 @@ hipError_t launch_describe(uint32_t lanes, uint32_t blocks, BatchView bv, const 
                            const Params& prm, hipStream_t s);
 hipError_t launch_match_index(uint32_t blocks, MatchBatchView bv, const MatchJob& job0,
                               hipStream_t s);
-hipError_t launch_match_dir(bool mutual, uint32_t blocks, MatchBatchView bv,
+hipError_t launch_match_dir(uint32_t lanes, bool mutual, uint32_t blocks, MatchBatchView bv,
                             const MatchJob& job0, float ratio, hipStream_t s);
 uint32_t bf_queries_per_block();
 hipError_t launch_bf(const uint32_t* q, uint32_t nq, const uint32_t* t, uint32_t nt,
@@ -110,10 +110,44 @@ void build_level_mask(uint32_t W0, uint32_t H0, uint32_t w, uint32_t h, uint32_t
     }
 }
 
+orbfe::TapEntry tap_entry(const orbfe::AxisTaps& t) {
+    return {t.i0 | (t.i1 << 16), t.frac};
+}
+
+// Tap tables of one level (layout: level_tap_offsets).  The host computes
+// them with the same f32 operations as the device code (correctly rounded
+// division, no contraction), so the entries are the ones the kernels would
+// compute: level texels sample level 0 at uv = (c + 0.5) / n
+// (create_pyramid.wgsl:5-14); the 9-tap Gaussian samples the level at
+// uv = (c + 0.5) * texel + off * texel (generate_features.wgsl:149-172).
+void build_level_taps(const Plan& P, const LevelGeom& G, orbfe::TapEntry* out) {
+    uint32_t lc, lr, sc, sr;
+    orbfe::level_tap_offsets(0, G.w, G.h, &lc, &lr, &sc, &sr);
+    for (uint32_t c = 0; c < G.w; ++c)
+        out[lc + c] = tap_entry(orbfe::axis_taps(
+            (static_cast<float>(c) + 0.5f) / static_cast<float>(G.w), P.W0));
+    for (uint32_t c = 0; c < G.h; ++c)
+        out[lr + c] = tap_entry(orbfe::axis_taps(
+            (static_cast<float>(c) + 0.5f) / static_cast<float>(G.h), P.H0));
+    const float offs[3] = {-1.2f, 0.0f, 1.2f};
+    const uint32_t sw = G.w + orbfe::kSmoothExt, sh = G.h + orbfe::kSmoothExt;
+    for (uint32_t i = 0; i < 3; ++i) {
+        for (uint32_t x = 0; x < sw; ++x) {
+            const float uv = (static_cast<float>(x) + 0.5f) * G.texel_x;
+            out[sc + i * sw + x] = tap_entry(orbfe::axis_taps(uv + offs[i] * G.texel_x, G.w));
+        }
+        for (uint32_t y = 0; y < sh; ++y) {
+            const float uv = (static_cast<float>(y) + 0.5f) * G.texel_y;
+            out[sr + i * sh + y] = tap_entry(orbfe::axis_taps(uv + offs[i] * G.texel_y, G.h));
+        }
+    }
+}
+
 struct PlanEntry {
     Plan host{};
     Plan* dev = nullptr;
     uint64_t* mask = nullptr;
+    orbfe::TapEntry* taps = nullptr;
     // per-frame scratch sizes (bytes), 256-aligned
     size_t surv_bytes = 0, kp_bytes = 0, pyr_bytes = 0, feat_bytes = 0, match_bytes = 0;
     size_t sm_bytes = 0, index_bytes = 0;
@@ -256,19 +290,22 @@ int get_plan(orbfe_ctx* ctx, uint32_t W0, uint32_t H0, uint32_t cap, PlanEntry**
     P.desc_lanes = ctx->cfg.describe_lanes;
     P.feat_stride = 8 + 48 * cap;
     P.desc_groups = ceil_div(cap, 256 / P.desc_lanes);
-    P.query_groups = ceil_div(cap, orbfe::kQueriesPerBlock);
+    P.match_lanes = ctx->cfg.match_lanes;
+    P.query_groups = ceil_div(cap, 256 / P.match_lanes);
     P.max_dx = 0.1f * static_cast<float>(W0);
     P.max_dy = 0.1f * static_cast<float>(H0);
     P.cell_inv_x = 2.0f / P.max_dx;
     P.cell_inv_y = 2.0f / P.max_dy;
     {
         const auto al4 = [](uint32_t v) { return (v + 3u) & ~3u; };  // 16-byte aligned
-        P.idx_xy = al4(orbfe::kCells + 1);
-        P.idx_desc = al4(P.idx_xy + 2 * cap);
-        P.idx_pos = P.idx_desc + 8 * cap;
-        P.idx_stride = al4(P.idx_pos + cap);
+        const uint32_t recs = L * cap;
+        P.idx_xy = al4(L * (orbfe::kCells + 1));
+        P.idx_dlo = al4(P.idx_xy + 2 * recs);
+        P.idx_dhi = P.idx_dlo + 4 * recs;
+        P.idx_pos = P.idx_dhi + 4 * recs;
+        P.idx_words = al4(P.idx_pos + recs);
     }
-    uint32_t tiles = 0, words = 0, pyr = 0, sm = 0;
+    uint32_t tiles = 0, words = 0, pyr = 0, sm = 0, ntaps = 0;
     float scale = 1.0f;
     for (uint32_t l = 0; l < L; ++l) {
         if (dw[l] == 0 || dh[l] == 0)
@@ -292,6 +329,8 @@ int get_plan(orbfe_ctx* ctx, uint32_t W0, uint32_t H0, uint32_t cap, PlanEntry**
         G.lod0_scale = scale;  // toLod0 (match_features_cpu.cpp:58-64)
         G.texel_x = 1.0f / static_cast<float>(G.w);
         G.texel_y = 1.0f / static_cast<float>(G.h);
+        G.tap_off = ntaps;
+        ntaps += orbfe::level_tap_entries(G.w, G.h);
         scale *= static_cast<float>(1.2);
     }
     P.tiles = tiles;
@@ -306,6 +345,13 @@ int get_plan(orbfe_ctx* ctx, uint32_t W0, uint32_t H0, uint32_t cap, PlanEntry**
     if (rc) return rc;
     HIP_TRY(ctx, hipMemcpy(pe->mask, mask.data(), mask.size() * 8, hipMemcpyHostToDevice));
     P.mask = pe->mask;
+    std::vector<orbfe::TapEntry> taps(ntaps);
+    for (uint32_t l = 0; l < L; ++l) build_level_taps(P, P.lv[l], taps.data() + P.lv[l].tap_off);
+    rc = dalloc(ctx, &pe->taps, taps.size() * sizeof(orbfe::TapEntry));
+    if (rc) return rc;
+    HIP_TRY(ctx, hipMemcpy(pe->taps, taps.data(), taps.size() * sizeof(orbfe::TapEntry),
+                           hipMemcpyHostToDevice));
+    P.taps = pe->taps;
     rc = dalloc(ctx, &pe->dev, sizeof(Plan));
     if (rc) return rc;
     HIP_TRY(ctx, hipMemcpy(pe->dev, &P, sizeof(Plan), hipMemcpyHostToDevice));
@@ -315,7 +361,7 @@ int get_plan(orbfe_ctx* ctx, uint32_t W0, uint32_t H0, uint32_t cap, PlanEntry**
     pe->feat_bytes = align_up(static_cast<size_t>(L) * P.feat_stride);
     pe->match_bytes = align_up(static_cast<size_t>(L) * cap * 4);
     pe->sm_bytes = align_up(static_cast<size_t>(sm) * 4);
-    pe->index_bytes = align_up(static_cast<size_t>(L) * P.idx_stride * 4);
+    pe->index_bytes = align_up(static_cast<size_t>(P.idx_words) * 4);
     *out = pe.get();
     ctx->plans.emplace(key, std::move(pe));
     return ORBFE_OK;
@@ -367,10 +413,12 @@ int launch_match_stage(orbfe_ctx* ctx, const MatchBatchView& bv, const MatchJob&
                 return orbfe::launch_match_index(pairs * 2 * ctx->cfg.levels, bv, job0, s);
             }));
     HIP_TRY(ctx, timed(ctx, "k_match_dir", s, [&] {
-                return orbfe::launch_match_dir(false, blocks, bv, job0, ratio, s);
+                return orbfe::launch_match_dir(ctx->cfg.match_lanes, false, blocks, bv, job0,
+                                               ratio, s);
             }));
     HIP_TRY(ctx, timed(ctx, "k_match_mutual", s, [&] {
-                return orbfe::launch_match_dir(true, blocks, bv, job0, ratio, s);
+                return orbfe::launch_match_dir(ctx->cfg.match_lanes, true, blocks, bv, job0,
+                                               ratio, s);
             }));
     return ORBFE_OK;
 }
@@ -569,6 +617,7 @@ void orbfe_destroy(orbfe_ctx* ctx) {
     (void)hipDeviceSynchronize();
     for (auto& kv : ctx->plans) {
         dfree(kv.second->mask);
+        dfree(kv.second->taps);
         dfree(kv.second->dev);
     }
     dfree(ctx->surv);
@@ -608,13 +657,16 @@ int orbfe_create(const orbfe_config* cfg, int device, orbfe_ctx** out) {
     if (cfg->fast_min_run > 16) return ORBFE_ERANGE;
     const uint32_t th = cfg->detect_tile_rows ? cfg->detect_tile_rows : orbfe::kTileHDefault;
     const uint32_t dl = cfg->describe_lanes ? cfg->describe_lanes : orbfe::kDescLanesDefault;
-    if ((th != 8 && th != 16 && th != 32) || (dl != 8 && dl != 16 && dl != 32))
+    const uint32_t ml = cfg->match_lanes ? cfg->match_lanes : orbfe::kMatchLanesDefault;
+    if ((th != 8 && th != 16 && th != 32) || (dl != 8 && dl != 16 && dl != 32) ||
+        (ml != 16 && ml != 32 && ml != 64))
         return ORBFE_ERANGE;
     auto ctx = std::make_unique<orbfe_ctx>();
     ctx->device = device;
     ctx->cfg = *cfg;
     ctx->cfg.detect_tile_rows = th;  // every plan of a context shares the launch shapes
     ctx->cfg.describe_lanes = dl;
+    ctx->cfg.match_lanes = ml;
     ctx->prm.threshold = cfg->fast_threshold;
     ctx->prm.min_run = cfg->fast_min_run;
     ctx->prm.orient_r = cfg->orientation_radius;

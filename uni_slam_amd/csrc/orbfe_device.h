@@ -33,14 +33,15 @@ constexpr int kDetectThreads = 256;
 // k_describe: desc_lanes lanes per keypoint (8, 16 or 32), so 256 /
 // desc_lanes keypoints per 256-thread workgroup.
 constexpr uint32_t kDescLanesDefault = 16;
-// k_match_dir: queries per workgroup (one per wave).
-constexpr uint32_t kQueriesPerBlock = 4;
+// k_match_dir: match_lanes lanes per query (16, 32 or 64), so 256 /
+// match_lanes queries per 256-thread workgroup.
+constexpr uint32_t kMatchLanesDefault = 16;
 // Matcher cell index: LOD0 plane cut into kCellsPerAxis^2 cells of half the
 // search window (0.05*W0 x 0.05*H0); 24 cells cover 1.2x the frame.
 constexpr uint32_t kCellsPerAxis = 24;
 constexpr uint32_t kCells = kCellsPerAxis * kCellsPerAxis;
-// Per-wave LDS words of the matcher (range offsets and sources, 64 each).
-constexpr uint32_t kCandCap = 128;
+// Candidate runs of one query: up to 3 pool levels x kCellsPerAxis cell rows.
+constexpr uint32_t kMaxRanges = 3 * kCellsPerAxis;
 
 // Key packing of the pipeline matcher: distance (<= 256) above a 22-bit
 // candidate id (2 bits pool level slot, 20 bits index).
@@ -56,6 +57,13 @@ constexpr uint32_t kBfKeyShift = 23;
 // last column, so G(P) = G(w+3); the same holds for rows.
 constexpr uint32_t kSmoothExt = 4;
 
+// Sampler taps of one coordinate along one axis: i0 | i1 << 16 and the
+// fraction, as axis_taps computes them.
+struct TapEntry {
+    uint32_t i01;
+    float frac;
+};
+
 struct LevelGeom {
     uint32_t w, h;
     uint32_t wpr;        // 64-pixel words per row, ceil(w / 64)
@@ -68,28 +76,36 @@ struct LevelGeom {
     uint32_t sm_off;     // float offset of the level's smoothed image
     float lod0_scale;    // product of l factors 1.2f (toLod0)
     float texel_x, texel_y;  // 1.0f / w, 1.0f / h (generate_features.wgsl:151)
+    // Entry offset of the level's tap tables in Plan::taps: level texel
+    // column cx (w entries) and row cy (h entries) sampled from level 0 at
+    // (c + 0.5) / n, then the 9-tap Gaussian's column taps [3][w + 4] and
+    // row taps [3][h + 4] at offsets -1.2, 0, 1.2 texels.
+    uint32_t tap_off;
+    uint32_t pad_;
 };
 
 struct Plan {
     uint32_t W0, H0, L, cap;
     uint32_t tile_h;      // detect tile rows
     uint32_t desc_lanes;  // describe lanes per keypoint
+    uint32_t match_lanes; // matcher lanes per query
     uint32_t tiles;       // detect tiles of one frame
     uint32_t words;       // 64-bit survivor words of one frame
     uint32_t pyr_floats;  // pyramid floats (levels >= 1)
     uint32_t feat_stride; // 8 + 48*cap
     uint32_t desc_groups; // describe blocks per level = ceil(cap / (256 / desc_lanes))
-    uint32_t query_groups; // matcher blocks per level = ceil(cap / kQueriesPerBlock)
+    uint32_t query_groups; // matcher blocks per level = ceil(cap / (256 / match_lanes))
     float max_dx, max_dy; // 0.1f*W0, 0.1f*H0 (match_features_cpu.cpp:76-77)
     uint32_t sm_floats;   // smoothed-image floats of one frame
     float cell_inv_x, cell_inv_y;  // 1 / cell size (cells of max_dx/2 x max_dy/2)
-    // Cell index of one level (u32 words from the level's base): cell
-    // starts [kCells + 1], then the level's records sorted by cell as
-    // arrays of LOD0 positions (float2), descriptors (2 x uint4) and record
-    // indices (u32).
-    uint32_t idx_stride;  // words per level
-    uint32_t idx_xy, idx_desc, idx_pos;  // word offsets of the three arrays
+    // Cell index of one side (u32 words): cell starts [L][kCells + 1], then
+    // the records of every level in cell order as arrays over the global
+    // record number g = level * cap + k: LOD0 positions (float2), the low and
+    // high descriptor halves (uint4 each) and record indices (u32).
+    uint32_t idx_words;   // words of one side's index
+    uint32_t idx_xy, idx_dlo, idx_dhi, idx_pos;  // word offsets of the arrays
     const uint64_t* mask; // static cull mask, `words` words
+    const TapEntry* taps; // sampler tap tables of all levels
     LevelGeom lv[kMaxLevels];
 };
 
@@ -110,7 +126,7 @@ struct MatchJob {
     const uint8_t* curr;
     int32_t* prev_match;  // L*cap
     int32_t* b_to_a;      // scratch L*cap
-    uint32_t* index_prev; // scratch: cell index of prev, L * plan->idx_stride
+    uint32_t* index_prev; // scratch: cell index of prev, plan->idx_words
     uint32_t* index_curr; // scratch: cell index of curr
 };
 
@@ -236,6 +252,21 @@ __host__ __device__ inline AxisTaps axis_taps(float coord, uint32_t n) {
     t.i0 = static_cast<uint32_t>(c0);
     t.i1 = static_cast<uint32_t>(c1);
     return t;
+}
+
+// Tap tables of one level (LevelGeom::tap_off), filled on the host with the
+// operations the kernels would otherwise repeat per tile.
+__host__ __device__ inline void level_tap_offsets(uint32_t base, uint32_t w, uint32_t h,
+                                                  uint32_t* lvl_col, uint32_t* lvl_row,
+                                                  uint32_t* sm_col, uint32_t* sm_row) {
+    *lvl_col = base;
+    *lvl_row = base + w;
+    *sm_col = base + w + h;
+    *sm_row = base + w + h + 3 * (w + kSmoothExt);
+}
+
+__host__ __device__ inline uint32_t level_tap_entries(uint32_t w, uint32_t h) {
+    return w + h + 3 * (w + kSmoothExt) + 3 * (h + kSmoothExt);
 }
 
 __host__ __device__ inline float lerp2(float t00, float t10, float t01, float t11,

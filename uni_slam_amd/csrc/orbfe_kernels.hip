@@ -59,6 +59,22 @@ __device__ inline const ORBFE_CONST T& as_const(const T* p) {
     return *const_ptr(p);
 }
 
+// f32 at byte offset `off` of a global buffer whose base is uniform: the
+// load takes the base from SGPRs and a 32-bit offset from one VGPR.
+#define ORBFE_GLOBAL __attribute__((address_space(1)))
+
+__device__ inline float load_f32(const uint8_t* base, uint32_t off) {
+    const ORBFE_GLOBAL uint8_t* g =
+        reinterpret_cast<const ORBFE_GLOBAL uint8_t*>(reinterpret_cast<uintptr_t>(base));
+    return *reinterpret_cast<const ORBFE_GLOBAL float*>(g + off);
+}
+
+__device__ inline TapEntry load_tap(const TapEntry* t, uint32_t i) {
+    const ORBFE_GLOBAL uint32_t* g =
+        reinterpret_cast<const ORBFE_GLOBAL uint32_t*>(reinterpret_cast<uintptr_t>(t)) + 2 * i;
+    return TapEntry{g[0], __uint_as_float(g[1])};
+}
+
 // Copy of a table entry, dword by dword (the loads stay scalar).
 template <typename T>
 __device__ inline T copy_const(const ORBFE_CONST T* p) {
@@ -209,19 +225,19 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     const int32_t y0 = static_cast<int32_t>(tile_y) * TH;
     const uint32_t w = G.w, h = G.h;
     const uint32_t sw = w + kSmoothExt, sh = h + kSmoothExt;
-    const float texel_x = G.texel_x;
-    const float texel_y = G.texel_y;
-    const float offs[3] = {-1.2f, 0.0f, 1.2f};
     const float wts[3] = {0.3125f, 0.375f, 0.3125f};
 
     lut[tid] = static_cast<float>(tid) / 255.0f;
+    uint32_t t_lc, t_lr, t_sc, t_sr;
+    level_tap_offsets(G.tap_off, w, h, &t_lc, &t_lr, &t_sc, &t_sr);
+    const TapEntry* taps = P.taps;
 
     // Set-up items, spread over the block: per-column / per-row source
     // taps of the staged region, row taps of the 9-tap Gaussian for the
     // TH output rows, the cull-mask words of the tile rows and, on row-0
     // tiles, the texels of the four row-0 neighbours outside the tile
     // (x0-2, x0-1, x0+64, x0+65) whose scores the 9-aligned windows of
-    // cull_corners.wgsl:36-56 may need.
+    // cull_corners.wgsl:36-56 may need.  Taps come from the plan's tables.
     const bool has_row0 = y0 == 0 && tile_x < G.wpr;
     constexpr uint32_t kTapItems = kLdsW + LH;
     constexpr uint32_t kSrowItems = kTapItems + 3 * TH;
@@ -239,10 +255,9 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
             uint32_t i0 = cl, i1 = cl;
             float frac = 0.0f;
             if (l > 0) {
-                const float uv = (static_cast<float>(cl) + 0.5f) / static_cast<float>(n);
-                const AxisTaps tp = axis_taps(uv, is_col ? P.W0 : P.H0);
-                i0 = tp.i0;
-                i1 = tp.i1;
+                const TapEntry tp = load_tap(taps, (is_col ? t_lc : t_lr) + cl);
+                i0 = tp.i01 & 0xFFFFu;
+                i1 = tp.i01 >> 16;
                 frac = tp.frac;
             }
             const uint32_t ok = (!prm.border_zero || inside) ? 1u : 0u;
@@ -254,12 +269,17 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
         } else if (e < kSrowItems) {
             const uint32_t q = e - kTapItems;
             const uint32_t jj = q / TH, rr = q - jj * TH;
-            const float uvy =
-                (static_cast<float>(static_cast<uint32_t>(y0) + rr) + 0.5f) * texel_y;
-            const AxisTaps ay = axis_taps(uvy + offs[jj] * texel_y, h);
-            srow_i0[jj][rr] = ay.i0;
-            srow_i1[jj][rr] = ay.i1;
-            srow_b[jj][rr] = ay.frac;
+            const uint32_t y = static_cast<uint32_t>(y0) + rr;
+            uint32_t i01 = 0;
+            float frac = 0.0f;
+            if (y < sh) {  // rows past the extended grid are not smoothed
+                const TapEntry tp = load_tap(taps, t_sr + jj * sh + y);
+                i01 = tp.i01;
+                frac = tp.frac;
+            }
+            srow_i0[jj][rr] = i01 & 0xFFFFu;
+            srow_i1[jj][rr] = i01 >> 16;
+            srow_b[jj][rr] = frac;
         } else if (e < kMaskItems) {
             const uint32_t rr = e - kSrowItems;
             const uint32_t y = static_cast<uint32_t>(y0) + rr;
@@ -336,42 +356,61 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     }
     __syncthreads();
     const uint32_t spitch = nd * 4;
+    const uint8_t* srcb = reinterpret_cast<const uint8_t*>(src);
 
-    // Stage the level tile ((TH + 6) x 70 texels, flat over the block).
-    for (uint32_t e = tid; e < kLdsW * LH; e += kDetectThreads) {
-        const uint32_t r = e / kLdsW;
-        const uint32_t c = e - r * kLdsW;
-        float v = 0.0f;
-        if (col_ok[c] & row_ok[r]) {
-            const uint32_t ci0 = col_i0[c];
-            const bool in_src = use_src && c >= c_first && c <= c_last && r >= r_first &&
-                                r <= r_last;
-            if (l == 0) {
-                v = lut[in_src ? src_byte(src, (row_i0[r] - rowmin) * spitch + (ci0 - cs))
-                               : J.img[row_i0[r] * J.pitch + ci0]];
-            } else {
-                const uint32_t ci1 = col_i1[c];
-                uint32_t p00, p01, p10, p11;
-                if (in_src) {
-                    // Byte offsets inside the staged region (ci >= cs, rows
-                    // >= rowmin), so every address formed lies in the
-                    // region: a pointer before its start (srcb - cs) can
-                    // carry out of the LDS aperture in flat addressing.
-                    const uint32_t o0 = (row_i0[r] - rowmin) * spitch;
-                    const uint32_t o1 = (row_i1[r] - rowmin) * spitch;
-                    const uint32_t a0 = ci0 - cs, a1 = ci1 - cs;
-                    p00 = src_byte(src, o0 + a0); p01 = src_byte(src, o0 + a1);
-                    p10 = src_byte(src, o1 + a0); p11 = src_byte(src, o1 + a1);
-                } else {
-                    const uint8_t* s0 = J.img + row_i0[r] * J.pitch;
-                    const uint8_t* s1 = J.img + row_i1[r] * J.pitch;
-                    p00 = s0[ci0]; p01 = s0[ci1];
-                    p10 = s1[ci0]; p11 = s1[ci1];
-                }
-                v = lerp2(lut[p00], lut[p01], lut[p10], lut[p11], col_a[c], row_b[r]);
+    // One texel of the level tile from its row and column taps: level 0 is
+    // the LUT of the source byte, levels >= 1 the bilinear sample of level 0
+    // (create_pyramid.wgsl:5-14).  Source bytes come from the staged region
+    // (byte offsets >= 0 inside it) or, outside it, from the frame.
+    const auto texel = [&](uint32_t ri0, uint32_t ri1, float rb, bool row_in, uint32_t ci0,
+                           uint32_t ci1, float ca, bool col_in) -> float {
+        if (row_in && col_in) {
+            const uint32_t o0 = (ri0 - rowmin) * spitch, a0 = ci0 - cs;
+            if (l == 0) return lut[srcb[o0 + a0]];
+            const uint32_t o1 = (ri1 - rowmin) * spitch, a1 = ci1 - cs;
+            return lerp2(lut[srcb[o0 + a0]], lut[srcb[o0 + a1]], lut[srcb[o1 + a0]],
+                         lut[srcb[o1 + a1]], ca, rb);
+        }
+        const uint8_t* s0 = J.img + ri0 * J.pitch;
+        if (l == 0) return lut[s0[ci0]];
+        const uint8_t* s1 = J.img + ri1 * J.pitch;
+        return lerp2(lut[s0[ci0]], lut[s0[ci1]], lut[s1[ci0]], lut[s1[ci1]], ca, rb);
+    };
+
+    // Stage the level tile ((TH + 6) x 70 texels).  Lane c stages column c
+    // of rows wave, wave + 4, ... with its column taps in registers and the
+    // row taps wave-uniform; columns 64..69 follow, one texel per thread.
+    {
+        const uint32_t c = lane;
+        const uint32_t ci0 = col_i0[c], ci1 = col_i1[c];
+        const float ca = col_a[c];
+        const bool cok = col_ok[c] != 0;
+        const bool cin = use_src && c >= c_first && c <= c_last;
+#pragma unroll
+        for (uint32_t k = 0; k < (LH + kWaves - 1) / kWaves; ++k) {
+            const uint32_t r = wave + k * kWaves;
+            if (r < LH) {
+                const uint32_t ri0 = __builtin_amdgcn_readfirstlane(row_i0[r]);
+                const uint32_t ri1 = __builtin_amdgcn_readfirstlane(row_i1[r]);
+                const float rb = __uint_as_float(
+                    __builtin_amdgcn_readfirstlane(__float_as_uint(row_b[r])));
+                const bool rok = __builtin_amdgcn_readfirstlane(row_ok[r]) != 0;
+                const bool rin = use_src && r >= r_first && r <= r_last;
+                tile[r][c] = (cok && rok) ? texel(ri0, ri1, rb, rin, ci0, ci1, ca, cin) : 0.0f;
             }
         }
-        tile[r][c] = v;
+        constexpr uint32_t kExtra = (kLdsW - kTileW) * LH;
+        static_assert(kExtra <= kDetectThreads, "one extra texel per thread");
+        if (tid < kExtra) {
+            const uint32_t r = tid / (kLdsW - kTileW);
+            const uint32_t ce = kTileW + (tid - r * (kLdsW - kTileW));
+            const bool in = use_src && ce >= c_first && ce <= c_last && r >= r_first &&
+                            r <= r_last;
+            tile[r][ce] = (col_ok[ce] & row_ok[r])
+                              ? texel(row_i0[r], row_i1[r], row_b[r], in, col_i0[ce], col_i1[ce],
+                                      col_a[ce], in)
+                              : 0.0f;
+        }
     }
     __syncthreads();
 
@@ -385,15 +424,22 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     const bool slow = x0 > static_cast<int32_t>(w) + 2 || y0 > static_cast<int32_t>(h) + 2;
     if (!slow) {
         if (x < sw) {
-            const float uvx = (static_cast<float>(x) + 0.5f) * texel_x;
+            TapEntry tp[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) tp[i] = load_tap(taps, t_sc + i * sw + x);
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-                const AxisTaps ax = axis_taps(uvx + offs[i] * texel_x, w);
-                const uint32_t c0 = ax.i0 - x0 + kHalo, c1 = ax.i1 - x0 + kHalo;
-                for (uint32_t r = wave; r < LH; r += kDetectThreads / 64) {
-                    const float t0 = tile[r][c0];
-                    stage[(i * LH + r) * kTileW + lane] =
-                        __float_as_uint(t0 + ax.frac * (tile[r][c1] - t0));
+                const uint32_t c0 = (tp[i].i01 & 0xFFFFu) - x0 + kHalo;
+                const uint32_t c1 = (tp[i].i01 >> 16) - x0 + kHalo;
+                const float fr = tp[i].frac;
+#pragma unroll
+                for (uint32_t k = 0; k < (LH + kWaves - 1) / kWaves; ++k) {
+                    const uint32_t r = wave + k * kWaves;
+                    if (r < LH) {
+                        const float t0 = tile[r][c0];
+                        stage[(i * LH + r) * kTileW + lane] =
+                            __float_as_uint(t0 + fr * (tile[r][c1] - t0));
+                    }
                 }
             }
         }
@@ -407,9 +453,13 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
                 float sum = 0.0f;
 #pragma unroll
                 for (int jj = 0; jj < 3; ++jj) {
-                    const uint32_t ra = srow_i0[jj][rr] - y0 + kHalo;
-                    const uint32_t rb = srow_i1[jj][rr] - y0 + kHalo;
-                    const float bb = srow_b[jj][rr];
+                    // wave-uniform row taps (rr is)
+                    const uint32_t ra =
+                        __builtin_amdgcn_readfirstlane(srow_i0[jj][rr]) - y0 + kHalo;
+                    const uint32_t rb =
+                        __builtin_amdgcn_readfirstlane(srow_i1[jj][rr]) - y0 + kHalo;
+                    const float bb = __uint_as_float(
+                        __builtin_amdgcn_readfirstlane(__float_as_uint(srow_b[jj][rr])));
 #pragma unroll
                     for (int i = 0; i < 3; ++i) {
                         const float top = __uint_as_float(stage[(i * LH + ra) * kTileW + lane]);
@@ -802,24 +852,34 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
     // TPL*s .. TPL*s + TPL - 1; G at the wrapped u32 coordinates (:116-129)
     // is looked up in the smoothed image, where coordinates past the
     // extension (including wrapped negatives) share the value at w+3 / h+3.
+    //
+    // round() is half-to-even.  For |v| < 2^22, v + 1.5*2^23 lies in
+    // [2^23, 2^24), where the float spacing is 1, so the addition rounds v
+    // to the nearest integer, ties to even, and leaves that integer in the
+    // mantissa: i32(round(v)) = bits(v + 1.5*2^23) - bits(1.5*2^23).  The
+    // pattern coordinates are at most 13 in magnitude, so |v| <= 19.  The
+    // subtraction is folded into the keypoint coordinate (mod 2^32).
+    constexpr float kRound = 12582912.0f;  // 1.5 * 2^23
+    constexpr uint32_t kRoundBits = 0x4B400000u;
     const uint32_t sw = G.w + kSmoothExt, sh = G.h + kSmoothExt;
-    const float* sm = J.smooth + G.sm_off;
+    const uint8_t* sm = reinterpret_cast<const uint8_t*>(J.smooth + G.sm_off);
+    const uint32_t xb = x - kRoundBits, yb = y - kRoundBits;
+    // Columns are clamped as byte offsets: 4 * (x + ax) stays above
+    // 4 * (sw - 1) for the wrapped negatives (|ax| <= 19).
+    const uint32_t xb4 = 4 * xb, sw4 = 4 * sw, xmax4 = 4 * (sw - 1);
     uint32_t bits = 0;
-#pragma unroll 4
-    for (uint32_t k = 0; k < TPL; ++k) {
+#pragma unroll
+    for (int k = static_cast<int>(TPL) - 1; k >= 0; --k) {
         const float4 pt = pat[TPL * sub + k];
         const float pax = pt.x, pay = pt.y, pbx = pt.z, pby = pt.w;
-        const int32_t ax = static_cast<int32_t>(__builtin_rintf(pax * cs - pay * sn));
-        const int32_t ay = static_cast<int32_t>(__builtin_rintf(pax * sn + pay * cs));
-        const int32_t bx = static_cast<int32_t>(__builtin_rintf(pbx * cs - pby * sn));
-        const int32_t by = static_cast<int32_t>(__builtin_rintf(pbx * sn + pby * cs));
-        const uint32_t pax_u = umin(static_cast<uint32_t>(static_cast<int32_t>(x) + ax), sw - 1);
-        const uint32_t pay_u = umin(static_cast<uint32_t>(static_cast<int32_t>(y) + ay), sh - 1);
-        const uint32_t pbx_u = umin(static_cast<uint32_t>(static_cast<int32_t>(x) + bx), sw - 1);
-        const uint32_t pby_u = umin(static_cast<uint32_t>(static_cast<int32_t>(y) + by), sh - 1);
-        const float sa = sm[pay_u * sw + pax_u];
-        const float sb = sm[pby_u * sw + pbx_u];
-        bits |= (sa > sb ? 1u : 0u) << k;
+        const uint32_t ax4 = umin(4 * __float_as_uint((pax * cs - pay * sn) + kRound) + xb4, xmax4);
+        const uint32_t ay = umin(__float_as_uint((pax * sn + pay * cs) + kRound) + yb, sh - 1);
+        const uint32_t bx4 = umin(4 * __float_as_uint((pbx * cs - pby * sn) + kRound) + xb4, xmax4);
+        const uint32_t by = umin(__float_as_uint((pbx * sn + pby * cs) + kRound) + yb, sh - 1);
+        // byte offsets < 4 * sw * sh (24-bit factors)
+        const float sa = load_f32(sm, __umul24(ay, sw4) + ax4);
+        const float sb = load_f32(sm, __umul24(by, sw4) + bx4);
+        bits = (bits << 1) | (sa > sb ? 1u : 0u);
     }
     // Record word e: x, y, strength, angle, then descriptor word e - 4 =
     // bits of lanes LPW*(e-4) .. LPW*(e-4) + LPW - 1, low lane first.
@@ -868,10 +928,13 @@ struct FeatView {
 
 }  // namespace
 
-constexpr uint32_t kMatchThreads = 256;  // 4 waves = 4 queries
+constexpr uint32_t kMatchThreads = 256;
 // Candidates a lane loads per round (their loads are issued together; more
 // would cost occupancy).
 constexpr uint32_t kCandPerLane = 2;
+// Native vector types for loads through the global address space.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 // LOD0 position of a record (toLod0, match_features_cpu.cpp:58-64).
 __device__ inline void rec_lod0(const uint32_t* r, float scale, float* x, float* y) {
@@ -886,8 +949,10 @@ __device__ inline uint32_t cell_of(float v, float inv) {
 }
 
 // k_match_index: per (pair, side, level) a counting sort of the level's
-// records by LOD0 cell: cell_start[kCells + 1], then the records in cell
-// order as arrays of LOD0 positions, descriptors and record indices, so a
+// records by LOD0 cell.  The index of one side holds the cell starts of
+// every level ([L][kCells + 1]) and, for every level's records in cell
+// order, the LOD0 positions, the two descriptor halves and the record
+// indices as arrays over a global record number g = level * cap + k, so a
 // cell range of candidates is read with contiguous loads.  Cell order is
 // monotone in the coordinates, so every record inside a search window lies
 // in the cells the window's (margin-widened) corners map to.
@@ -902,7 +967,8 @@ __global__ __launch_bounds__(256) void k_match_index(MatchBatchView bv, MatchJob
     const MatchJob J = bv.n ? copy_const(const_ptr(bv.jobs + j)) : job0;
     ConstPlan P = as_const(J.plan);
     const FeatView A{side ? J.curr : J.prev, P.feat_stride};
-    uint32_t* out = (side ? J.index_curr : J.index_prev) + l * P.idx_stride;
+    uint32_t* base = side ? J.index_curr : J.index_prev;
+    uint32_t* starts = base + l * (kCells + 1);
     const uint32_t n = A.count(l, P.cap);
     const float sc = P.lv[l].lod0_scale;
     for (uint32_t c = tid; c <= kCells; c += 256) hist[c] = 0;
@@ -927,15 +993,17 @@ __global__ __launch_bounds__(256) void k_match_index(MatchBatchView bv, MatchJob
 #pragma unroll
         for (uint32_t k = 0; k < kPer; ++k) {
             hist[tid * kPer + k] = run;
-            out[tid * kPer + k] = run;
+            starts[tid * kPer + k] = run;
             run += v[k];
         }
-        if (tid == 63) out[kCells] = run;
+        if (tid == 63) starts[kCells] = run;
     }
     __syncthreads();
-    float2* sxy = reinterpret_cast<float2*>(out + P.idx_xy);
-    uint4* sdesc = reinterpret_cast<uint4*>(out + P.idx_desc);
-    uint32_t* spos = out + P.idx_pos;
+    const size_t g0 = static_cast<size_t>(l) * P.cap;
+    float2* sxy = reinterpret_cast<float2*>(base + P.idx_xy) + g0;
+    uint4* sdlo = reinterpret_cast<uint4*>(base + P.idx_dlo) + g0;
+    uint4* sdhi = reinterpret_cast<uint4*>(base + P.idx_dhi) + g0;
+    uint32_t* spos = base + P.idx_pos + g0;
     for (uint32_t i = tid; i < n; i += 256) {
         const uint32_t* r = A.rec(l, i);
         float x, y;
@@ -943,21 +1011,63 @@ __global__ __launch_bounds__(256) void k_match_index(MatchBatchView bv, MatchJob
         const uint32_t c = cell_of(y, P.cell_inv_y) * kCellsPerAxis + cell_of(x, P.cell_inv_x);
         const uint32_t k = atomicAdd(&hist[c], 1u);
         sxy[k] = make_float2(x, y);
-        sdesc[2 * k] = *reinterpret_cast<const uint4*>(r + 4);
-        sdesc[2 * k + 1] = *reinterpret_cast<const uint4*>(r + 8);
+        sdlo[k] = *reinterpret_cast<const uint4*>(r + 4);
+        sdhi[k] = *reinterpret_cast<const uint4*>(r + 8);
         spos[k] = i;
     }
 }
 
+namespace {
+
+// Inclusive sum over the aligned group of W lanes that holds `sub`.
+template <uint32_t W>
+__device__ inline uint32_t group_inclusive_sum(uint32_t v, uint32_t sub) {
+#pragma unroll
+    for (uint32_t off = 1; off < W; off <<= 1) {
+        const uint32_t o = __shfl_up(v, off, W);
+        if (sub >= off) v += o;
+    }
+    return v;
+}
+
+// Element i of a global array with a uniform base (32-bit offsets).
+__device__ inline uint32_t load_u32(const uint32_t* base, uint32_t i) {
+    return reinterpret_cast<const ORBFE_GLOBAL uint32_t*>(reinterpret_cast<uintptr_t>(base))[i];
+}
+
+__device__ inline float2 load_f2(const float2* base, uint32_t i) {
+    const f32x2 v =
+        reinterpret_cast<const ORBFE_GLOBAL f32x2*>(reinterpret_cast<uintptr_t>(base))[i];
+    return make_float2(v.x, v.y);
+}
+
+__device__ inline uint4 load_u4(const uint4* base, uint32_t i) {
+    const u32x4 v = reinterpret_cast<const ORBFE_GLOBAL u32x4*>(reinterpret_cast<uintptr_t>(base))[i];
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
+}  // namespace
+
+// Pipeline matcher, LANES lanes per query (256 / LANES queries per block).
 // kMutual = false: queries = curr, pool from prev -> b_to_a.
 // kMutual = true:  queries = prev, pool from curr, mutual check -> prev_match.
-template <bool kMutual>
+//
+// A query's candidates are the records of its pool levels (l, l-1, l+1, the
+// reference's concatenation order) in the cells under its search window:
+// one contiguous run of the index per (pool level, cell row).  The group
+// flattens the runs with a prefix sum (kept in LDS), then lane s takes
+// flattened candidates s, s + LANES, ... and walks forward through the runs.
+// Keys (d << 22 | pool slot << 20 | record index) order candidates like the
+// reference's scan with strict '<' (match_features_cpu.cpp:133-140).
+template <uint32_t LANES, bool kMutual>
 __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, MatchJob job0,
                                                               float ratio) {
-    __shared__ uint32_t cand[kMatchThreads / 64][kCandCap];
+    constexpr uint32_t QPB = kMatchThreads / LANES;
+    __shared__ uint32_t r_end[QPB][kMaxRanges];
+    __shared__ uint32_t r_gb[QPB][kMaxRanges];
     const uint32_t tid = threadIdx.x;
-    const uint32_t lane = tid & 63;
-    const uint32_t wave = tid >> 6;
+    const uint32_t sub = tid & (LANES - 1);
+    const uint32_t grp = tid / LANES;
     uint32_t b = blockIdx.x;
     uint32_t j = 0;
     if (bv.n) {
@@ -967,8 +1077,7 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
     const MatchJob M = bv.n ? copy_const(const_ptr(bv.jobs + j)) : job0;
     ConstPlan P = as_const(M.plan);
     const uint32_t l = b / P.query_groups;
-    const uint32_t g = b - l * P.query_groups;
-    const uint32_t slot = g * kQueriesPerBlock + wave;
+    const uint32_t slot = (b - l * P.query_groups) * QPB + grp;
     if (slot >= P.cap) return;
     const FeatView Q{kMutual ? M.prev : M.curr, P.feat_stride};
     const FeatView C{kMutual ? M.curr : M.prev, P.feat_stride};
@@ -976,19 +1085,21 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
     int32_t* out = kMutual ? M.prev_match : M.b_to_a;
     const uint32_t nq = Q.count(l, P.cap);
     if (slot >= nq) {
-        if (lane == 0) out[l * P.cap + slot] = -1;
+        if (sub == 0) out[l * P.cap + slot] = -1;
         return;
     }
 
     // Pool levels in the reference's concatenation order: l, l-1, l+1.
-    uint32_t pl[3], pn[3];
-    uint32_t npool = 0, total = 0;
-    pl[npool] = l; pn[npool] = C.count(l, P.cap); total += pn[npool]; ++npool;
-    if (l > 0) { pl[npool] = l - 1; pn[npool] = C.count(l - 1, P.cap); total += pn[npool]; ++npool; }
-    if (l + 1 < P.L) { pl[npool] = l + 1; pn[npool] = C.count(l + 1, P.cap); total += pn[npool]; ++npool; }
+    const uint32_t p0 = l;
+    const uint32_t p1 = l > 0 ? l - 1 : l + 1;
+    const uint32_t p2 = l + 1;
+    uint32_t npool = 1, total_pool = C.count(l, P.cap);
+    if (l > 0) { total_pool += C.count(l - 1, P.cap); ++npool; }
+    if (l + 1 < P.L) { total_pool += C.count(l + 1, P.cap); ++npool; }
+    const auto pool_level = [&](uint32_t s) { return s == 0 ? p0 : (s == 1 ? p1 : p2); };
 
     int32_t result = -1;
-    if (total >= 2) {
+    if (total_pool >= 2) {
         const uint32_t* qr = Q.rec(l, slot);
         float qx, qy;
         rec_lod0(qr, P.lv[l].lod0_scale, &qx, &qy);
@@ -1004,79 +1115,92 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
         const uint32_t cx1 = cell_of(hi_x + 1.0f, P.cell_inv_x);
         const uint32_t cy0 = cell_of(qy - dy - 1.0f, P.cell_inv_y);
         const uint32_t cy1 = cell_of(qy + dy + 1.0f, P.cell_inv_y);
-
-        uint32_t min1 = 0xFFFFFFFFu, min2 = 0xFFFFFFFFu;
-        // Candidate ranges: for each pool slot s and cell row, the cells
-        // cx0..cx1 are contiguous in the index.  Lane k of a round fetches
-        // the bounds of range k; a wave scan of the lengths flattens them.
         const uint32_t rows = cy1 - cy0 + 1;
-        const uint32_t nranges = npool * rows;
-        uint32_t* r_off = cand[wave];                 // [64] flattened offsets
-        uint32_t* r_src = cand[wave] + 64;            // [64] index position | slot
-        for (uint32_t r0 = 0; r0 < nranges; r0 += 64) {
-            const uint32_t k = r0 + lane;
-            uint32_t a = 0, len = 0, sk = 0;
+        const uint32_t nranges = npool * rows;  // <= kMaxRanges
+
+        // Runs (pool slot s, cell row): r_end = flattened end offset,
+        // r_gb = global record number minus flattened offset.
+        uint32_t total = 0;
+        for (uint32_t k0 = 0; k0 < nranges; k0 += LANES) {
+            const uint32_t k = k0 + sub;
+            uint32_t len = 0, gstart = 0;
             if (k < nranges) {
-                sk = k / rows;
-                const uint32_t row = cy0 + (k - sk * rows);
-                const uint32_t* base = cidx + (sk == 0 ? pl[0] : (sk == 1 ? pl[1] : pl[2])) * P.idx_stride;
-                a = base[row * kCellsPerAxis + cx0];
-                len = base[row * kCellsPerAxis + cx1 + 1] - a;
+                const uint32_t sk = k / rows;
+                const uint32_t lv = pool_level(sk);
+                const uint32_t* st =
+                    cidx + lv * (kCells + 1) + (cy0 + (k - sk * rows)) * kCellsPerAxis;
+                const uint32_t a = st[cx0];
+                len = st[cx1 + 1] - a;
+                gstart = lv * P.cap + a;
             }
-            const uint32_t incl = wave_inclusive_sum(len, lane);
-            const uint32_t total_c = __shfl(incl, 63);
-            const uint32_t nr = umin(64u, nranges - r0);
-            wave_sync();
-            r_off[lane] = incl - len;
-            r_src[lane] = a | (sk << 30);
-            wave_sync();
-            for (uint32_t t0 = 0; t0 < total_c; t0 += 64 * kCandPerLane) {
-                // Up to kCandPerLane candidates per lane: sorted position,
-                // then position, descriptor and index in one batch of loads.
-                uint32_t ent[kCandPerLane];
-                float2 cxy[kCandPerLane];
-                uint4 dlo[kCandPerLane], dhi[kCandPerLane];
-#pragma unroll
-                for (uint32_t u = 0; u < kCandPerLane; ++u) {
-                    const uint32_t t = t0 + lane + 64 * u;
-                    ent[u] = 0xFFFFFFFFu;
-                    if (t < total_c) {
-                        // last range with offset <= t
-                        uint32_t lo = 0, hi = nr;
-                        while (hi - lo > 1) {
-                            const uint32_t mid = (lo + hi) >> 1;
-                            if (r_off[mid] <= t) lo = mid; else hi = mid;
-                        }
-                        const uint32_t src = r_src[lo];
-                        const uint32_t s2 = src >> 30;
-                        const uint32_t cl = s2 == 0 ? pl[0] : (s2 == 1 ? pl[1] : pl[2]);
-                        const uint32_t* base = cidx + cl * P.idx_stride;
-                        const uint32_t k = (src & 0x3FFFFFFFu) + (t - r_off[lo]);
-                        cxy[u] = reinterpret_cast<const float2*>(base + P.idx_xy)[k];
-                        dlo[u] = reinterpret_cast<const uint4*>(base + P.idx_desc)[2 * k];
-                        dhi[u] = reinterpret_cast<const uint4*>(base + P.idx_desc)[2 * k + 1];
-                        ent[u] = (s2 << kMatchIdxBits) | base[P.idx_pos + k];
-                    }
-                }
-#pragma unroll
-                for (uint32_t u = 0; u < kCandPerLane; ++u) {
-                    if (ent[u] == 0xFFFFFFFFu) continue;
-                    const float cx = cxy[u].x, cy = cxy[u].y;
-                    if (cx < lo_x || cx > hi_x || __builtin_fabsf(qy - cy) > dy) continue;
-                    const uint32_t d = __popc(qd[0] ^ dlo[u].x) + __popc(qd[1] ^ dlo[u].y) +
-                                       __popc(qd[2] ^ dlo[u].z) + __popc(qd[3] ^ dlo[u].w) +
-                                       __popc(qd[4] ^ dhi[u].x) + __popc(qd[5] ^ dhi[u].y) +
-                                       __popc(qd[6] ^ dhi[u].z) + __popc(qd[7] ^ dhi[u].w);
-                    const uint32_t key = (d << kMatchKeyShift) | ent[u];
-                    min2 = umax(min1, umin(min2, key));
-                    min1 = umin(min1, key);
-                }
+            const uint32_t incl = total + group_inclusive_sum<LANES>(len, sub);
+            if (k < nranges) {
+                r_end[grp][k] = incl;
+                r_gb[grp][k] = gstart - (incl - len);
             }
-            wave_sync();
+            total = __shfl(incl, static_cast<int>(LANES - 1), LANES);
         }
-        for (int off = 32; off > 0; off >>= 1) {
-            const uint32_t o1 = __shfl_xor(min1, off);
-            const uint32_t o2 = __shfl_xor(min2, off);
+        wave_sync();
+
+        const float2* cxy = reinterpret_cast<const float2*>(cidx + P.idx_xy);
+        const uint4* cdlo = reinterpret_cast<const uint4*>(cidx + P.idx_dlo);
+        const uint4* cdhi = reinterpret_cast<const uint4*>(cidx + P.idx_dhi);
+        const uint32_t* cpos = cidx + P.idx_pos;
+        uint32_t min1 = 0xFFFFFFFFu, min2 = 0xFFFFFFFFu;
+        uint32_t k = 0;
+        uint32_t kend = total ? r_end[grp][0] : 0u, kgb = total ? r_gb[grp][0] : 0u;
+        uint32_t kslot = 0;  // pool slot of run k (k / rows)
+        for (uint32_t t0 = 0; t0 < total; t0 += kCandPerLane * LANES) {
+            uint32_t g[kCandPerLane], sl[kCandPerLane];
+            bool ok[kCandPerLane];
+#pragma unroll
+            for (uint32_t u = 0; u < kCandPerLane; ++u) {
+                const uint32_t t = t0 + sub + u * LANES;
+                ok[u] = t < total;
+                g[u] = 0;
+                sl[u] = 0;
+                if (ok[u]) {
+                    while (t >= kend) {  // ends at the last run: its end is total
+                        ++k;
+                        kend = r_end[grp][k];
+                        kgb = r_gb[grp][k];
+                        kslot = (k >= rows ? 1u : 0u) + (k >= 2 * rows ? 1u : 0u);
+                    }
+                    g[u] = kgb + t;
+                    sl[u] = kslot;
+                }
+            }
+            float2 xy[kCandPerLane];
+            uint32_t pos[kCandPerLane];
+            uint4 dlo[kCandPerLane], dhi[kCandPerLane];
+#pragma unroll
+            for (uint32_t u = 0; u < kCandPerLane; ++u) {
+                if (ok[u]) {
+                    xy[u] = load_f2(cxy, g[u]);
+                    pos[u] = load_u32(cpos, g[u]);
+                    dlo[u] = load_u4(cdlo, g[u]);
+                    dhi[u] = load_u4(cdhi, g[u]);
+                }
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kCandPerLane; ++u) {
+                if (!ok[u]) continue;
+                const float cx = xy[u].x, cy = xy[u].y;
+                if (cx < lo_x || cx > hi_x || __builtin_fabsf(qy - cy) > dy) continue;
+                const uint32_t d = __popc(qd[0] ^ dlo[u].x) + __popc(qd[1] ^ dlo[u].y) +
+                                   __popc(qd[2] ^ dlo[u].z) + __popc(qd[3] ^ dlo[u].w) +
+                                   __popc(qd[4] ^ dhi[u].x) + __popc(qd[5] ^ dhi[u].y) +
+                                   __popc(qd[6] ^ dhi[u].z) + __popc(qd[7] ^ dhi[u].w);
+                const uint32_t key = (d << kMatchKeyShift) | (sl[u] << kMatchIdxBits) | pos[u];
+                // (min1, min2) <- the two smallest of (min1, min2, key)
+                min2 = umax(min1, umin(min2, key));
+                min1 = umin(min1, key);
+            }
+        }
+#pragma unroll
+        for (uint32_t off = LANES / 2; off > 0; off >>= 1) {
+            const uint32_t o1 = __shfl_xor(min1, static_cast<int>(off), LANES);
+            const uint32_t o2 = __shfl_xor(min2, static_cast<int>(off), LANES);
             const uint32_t n1 = umin(min1, o1);
             const uint32_t n2 = umin(umax(min1, o1), umin(min2, o2));
             min1 = n1;
@@ -1088,20 +1212,24 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
             if (static_cast<float>(d1) < ratio * static_cast<float>(d2)) {
                 const uint32_t s = (min1 >> kMatchIdxBits) & 3u;
                 const uint32_t idx = min1 & ((1u << kMatchIdxBits) - 1u);
-                result = static_cast<int32_t>(((s == 0 ? pl[0] : (s == 1 ? pl[1] : pl[2])) << 24) | idx);
+                result = static_cast<int32_t>((pool_level(s) << 24) | idx);
             }
         }
-        if (kMutual && result >= 0) {
+        if (kMutual && result >= 0 && sub == 0) {
             const uint32_t bl = static_cast<uint32_t>(result) >> 24;
             const uint32_t bi = static_cast<uint32_t>(result) & 0xFFFFFFu;
             if (M.b_to_a[bl * P.cap + bi] != static_cast<int32_t>((l << 24) | slot)) result = -1;
         }
     }
-    if (lane == 0) out[l * P.cap + slot] = result;
+    if (sub == 0) out[l * P.cap + slot] = result;
 }
 
-template __global__ void k_match_dir<false>(MatchBatchView, MatchJob, float);
-template __global__ void k_match_dir<true>(MatchBatchView, MatchJob, float);
+template __global__ void k_match_dir<16, false>(MatchBatchView, MatchJob, float);
+template __global__ void k_match_dir<16, true>(MatchBatchView, MatchJob, float);
+template __global__ void k_match_dir<32, false>(MatchBatchView, MatchJob, float);
+template __global__ void k_match_dir<32, true>(MatchBatchView, MatchJob, float);
+template __global__ void k_match_dir<64, false>(MatchBatchView, MatchJob, float);
+template __global__ void k_match_dir<64, true>(MatchBatchView, MatchJob, float);
 
 // ---------------------------------------------------------------------------
 // Brute-force matcher.
@@ -1227,12 +1355,25 @@ hipError_t launch_match_index(uint32_t blocks, MatchBatchView bv, const MatchJob
     return hipGetLastError();
 }
 
-hipError_t launch_match_dir(bool mutual, uint32_t blocks, MatchBatchView bv, const MatchJob& job0,
-                            float ratio, hipStream_t s) {
+template <uint32_t LANES>
+void launch_match_dir_t(bool mutual, uint32_t blocks, MatchBatchView bv, const MatchJob& job0,
+                        float ratio, hipStream_t s) {
     if (mutual)
-        hipLaunchKernelGGL(k_match_dir<true>, dim3(blocks), dim3(kMatchThreads), 0, s, bv, job0, ratio);
+        hipLaunchKernelGGL((k_match_dir<LANES, true>), dim3(blocks), dim3(kMatchThreads), 0, s, bv,
+                           job0, ratio);
     else
-        hipLaunchKernelGGL(k_match_dir<false>, dim3(blocks), dim3(kMatchThreads), 0, s, bv, job0, ratio);
+        hipLaunchKernelGGL((k_match_dir<LANES, false>), dim3(blocks), dim3(kMatchThreads), 0, s,
+                           bv, job0, ratio);
+}
+
+hipError_t launch_match_dir(uint32_t lanes, bool mutual, uint32_t blocks, MatchBatchView bv,
+                            const MatchJob& job0, float ratio, hipStream_t s) {
+    switch (lanes) {
+        case 16: launch_match_dir_t<16>(mutual, blocks, bv, job0, ratio, s); break;
+        case 32: launch_match_dir_t<32>(mutual, blocks, bv, job0, ratio, s); break;
+        case 64: launch_match_dir_t<64>(mutual, blocks, bv, job0, ratio, s); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -33,7 +33,8 @@ class Config(C.Structure):
                 ("fast_threshold", C.c_float), ("fast_min_run", C.c_uint32),
                 ("orientation_radius", C.c_int32), ("features_per_level", C.c_uint32),
                 ("border_zero", C.c_uint32), ("lowe_ratio", C.c_float),
-                ("detect_tile_rows", C.c_uint32), ("describe_lanes", C.c_uint32)]
+                ("detect_tile_rows", C.c_uint32), ("describe_lanes", C.c_uint32),
+                ("match_lanes", C.c_uint32)]
 
 
 class Frame(C.Structure):
