@@ -423,17 +423,30 @@ int launch_match_stage(orbfe_ctx* ctx, const MatchBatchView& bv, const MatchJob&
     return ORBFE_OK;
 }
 
-// Layout of a detect batch: table (jobs, tile prefix, desc prefix) and per
-// frame scratch in one arena.
+// Job index of every block of a batched launch (u16 per block), so a block
+// finds its frame with one scalar load.
+constexpr uint32_t kMaxBatchJobs = 65535;
+
+void fill_block_jobs(const std::vector<uint32_t>& prefix, std::vector<uint16_t>& out) {
+    const size_t n = prefix.size() - 1;
+    out.assign(prefix[n] + 1, 0);  // even count padding: read as dwords
+    for (size_t j = 0; j < n; ++j)
+        std::fill(out.begin() + prefix[j], out.begin() + prefix[j + 1], static_cast<uint16_t>(j));
+}
+
+// Layout of a detect batch: table (jobs, tile prefix, desc prefix, tile and
+// describe block job indices) and per frame scratch in one arena.
 struct DetectLayout {
     std::vector<Job> jobs;
     std::vector<uint32_t> tile_prefix, desc_prefix;
+    std::vector<uint16_t> tile_job, desc_job;
     size_t table_bytes = 0, arena_bytes = 0;
-    size_t off_tiles = 0, off_desc = 0;
+    size_t off_tiles = 0, off_desc = 0, off_tile_job = 0, off_desc_job = 0;
 };
 
 int layout_detect(orbfe_ctx* ctx, const orbfe_frame* frames, uint32_t n, DetectLayout& lay,
                   uint8_t* arena) {
+    if (n > kMaxBatchJobs) return fail(ctx, ORBFE_ERANGE, "batch of %u frames > %u", n, kMaxBatchJobs);
     lay.jobs.resize(n);
     lay.tile_prefix.assign(n + 1, 0);
     lay.desc_prefix.assign(n + 1, 0);
@@ -468,9 +481,13 @@ int layout_detect(orbfe_ctx* ctx, const orbfe_frame* frames, uint32_t n, DetectL
         lay.desc_prefix[i + 1] = lay.desc_prefix[i] + pe->host.L * pe->host.desc_groups;
     }
     lay.arena_bytes = off;
+    fill_block_jobs(lay.tile_prefix, lay.tile_job);
+    fill_block_jobs(lay.desc_prefix, lay.desc_job);
     lay.off_tiles = align_up(sizeof(Job) * n);
     lay.off_desc = lay.off_tiles + align_up(4 * (n + 1));
-    lay.table_bytes = lay.off_desc + align_up(4 * (n + 1));
+    lay.off_tile_job = lay.off_desc + align_up(4 * (n + 1));
+    lay.off_desc_job = lay.off_tile_job + align_up(2 * lay.tile_job.size());
+    lay.table_bytes = lay.off_desc_job + align_up(2 * lay.desc_job.size());
     return ORBFE_OK;
 }
 
@@ -478,6 +495,8 @@ void pack_detect_table(const DetectLayout& lay, uint8_t* host) {
     std::memcpy(host, lay.jobs.data(), sizeof(Job) * lay.jobs.size());
     std::memcpy(host + lay.off_tiles, lay.tile_prefix.data(), 4 * lay.tile_prefix.size());
     std::memcpy(host + lay.off_desc, lay.desc_prefix.data(), 4 * lay.desc_prefix.size());
+    std::memcpy(host + lay.off_tile_job, lay.tile_job.data(), 2 * lay.tile_job.size());
+    std::memcpy(host + lay.off_desc_job, lay.desc_job.data(), 2 * lay.desc_job.size());
 }
 
 BatchView detect_view(const DetectLayout& lay, const uint8_t* dev) {
@@ -485,6 +504,8 @@ BatchView detect_view(const DetectLayout& lay, const uint8_t* dev) {
     v.jobs = reinterpret_cast<const Job*>(dev);
     v.tile_prefix = reinterpret_cast<const uint32_t*>(dev + lay.off_tiles);
     v.desc_prefix = reinterpret_cast<const uint32_t*>(dev + lay.off_desc);
+    v.tile_job = reinterpret_cast<const uint16_t*>(dev + lay.off_tile_job);
+    v.desc_job = reinterpret_cast<const uint16_t*>(dev + lay.off_desc_job);
     v.n = static_cast<uint32_t>(lay.jobs.size());
     return v;
 }
@@ -492,11 +513,13 @@ BatchView detect_view(const DetectLayout& lay, const uint8_t* dev) {
 struct MatchLayout {
     std::vector<MatchJob> jobs;
     std::vector<uint32_t> prefix;
-    size_t table_bytes = 0, arena_bytes = 0, off_prefix = 0;
+    std::vector<uint16_t> block_job;
+    size_t table_bytes = 0, arena_bytes = 0, off_prefix = 0, off_block_job = 0;
 };
 
 int layout_match(orbfe_ctx* ctx, const orbfe_match_pair* pairs, uint32_t n, MatchLayout& lay,
                  uint8_t* arena) {
+    if (n > kMaxBatchJobs) return fail(ctx, ORBFE_ERANGE, "batch of %u pairs > %u", n, kMaxBatchJobs);
     lay.jobs.resize(n);
     lay.prefix.assign(n + 1, 0);
     size_t off = 0;
@@ -521,20 +544,24 @@ int layout_match(orbfe_ctx* ctx, const orbfe_match_pair* pairs, uint32_t n, Matc
         lay.prefix[i + 1] = lay.prefix[i] + pe->host.L * pe->host.query_groups;
     }
     lay.arena_bytes = off;
+    fill_block_jobs(lay.prefix, lay.block_job);
     lay.off_prefix = align_up(sizeof(MatchJob) * n);
-    lay.table_bytes = lay.off_prefix + align_up(4 * (n + 1));
+    lay.off_block_job = lay.off_prefix + align_up(4 * (n + 1));
+    lay.table_bytes = lay.off_block_job + align_up(2 * lay.block_job.size());
     return ORBFE_OK;
 }
 
 void pack_match_table(const MatchLayout& lay, uint8_t* host) {
     std::memcpy(host, lay.jobs.data(), sizeof(MatchJob) * lay.jobs.size());
     std::memcpy(host + lay.off_prefix, lay.prefix.data(), 4 * lay.prefix.size());
+    std::memcpy(host + lay.off_block_job, lay.block_job.data(), 2 * lay.block_job.size());
 }
 
 MatchBatchView match_view(const MatchLayout& lay, const uint8_t* dev) {
     MatchBatchView v;
     v.jobs = reinterpret_cast<const MatchJob*>(dev);
     v.block_prefix = reinterpret_cast<const uint32_t*>(dev + lay.off_prefix);
+    v.block_job = reinterpret_cast<const uint16_t*>(dev + lay.off_block_job);
     v.n = static_cast<uint32_t>(lay.jobs.size());
     return v;
 }
@@ -734,7 +761,7 @@ int orbfe_detect_async(orbfe_ctx* ctx, const uint8_t* image, size_t pitch, float
     job0.surv = ctx->surv;
     job0.kp = ctx->kp;
     job0.smooth = ctx->smooth;
-    const BatchView bv{nullptr, nullptr, nullptr, 0};
+    const BatchView bv{nullptr, nullptr, nullptr, nullptr, nullptr, 0};
     return launch_detect_stage(ctx, bv, job0, P.tiles, P.L, P.L * P.desc_groups, stream);
 }
 
@@ -875,7 +902,7 @@ int orbfe_match_pipeline_async(orbfe_ctx* ctx, const void* prev_features,
     job0.b_to_a = ctx->b_to_a;
     job0.index_prev = ctx->index[0];
     job0.index_curr = ctx->index[1];
-    const MatchBatchView bv{nullptr, nullptr, 0};
+    const MatchBatchView bv{nullptr, nullptr, nullptr, 0};
     return launch_match_stage(ctx, bv, job0, P.L * P.query_groups, stream);
 }
 

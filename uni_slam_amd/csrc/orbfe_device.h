@@ -145,12 +145,15 @@ struct BatchView {
     const Job* jobs;
     const uint32_t* tile_prefix;  // n+1
     const uint32_t* desc_prefix;  // n+1
+    const uint16_t* tile_job;     // job of every detect tile (even-padded)
+    const uint16_t* desc_job;     // job of every describe block (even-padded)
     uint32_t n;
 };
 
 struct MatchBatchView {
     const MatchJob* jobs;
     const uint32_t* block_prefix;  // n+1
+    const uint16_t* block_job;     // job of every matcher block (even-padded)
     uint32_t n;
 };
 

@@ -18,6 +18,9 @@
 //               queries in registers, train tiles in LDS.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
+
 #include "brief_pattern.h"
 #include "orbfe_device.h"
 
@@ -25,6 +28,20 @@ namespace orbfe {
 
 // BRIEF test pattern (brief_pattern.h), uploaded once per device.
 __constant__ int8_t c_pattern[256][4];
+
+// Gather order of the describe kernel's sample points, per lanes-per-keypoint
+// variant (8, 16, 32), angle bin and half: for lane s of a keypoint group,
+// its PPL = 256 / LANES point ids (u8) in load order.  Built on the host
+// (build_desc_order) and uploaded once per device.
+constexpr uint32_t kAngleBins = 64;
+constexpr float kAngleBinScale = 10.185916357881302f;  // kAngleBins / (2 pi)
+constexpr size_t kDescOrderBytes = kAngleBins * 2 * 256;
+__device__ uint8_t g_desc_order[3][kAngleBins * 2 * 256];
+
+template <uint32_t LANES>
+__device__ inline const uint8_t* desc_order() {
+    return g_desc_order[LANES == 8 ? 0 : (LANES == 16 ? 1 : 2)];
+}
 
 namespace {
 
@@ -69,6 +86,11 @@ __device__ inline float load_f32(const uint8_t* base, uint32_t off) {
     return *reinterpret_cast<const ORBFE_GLOBAL float*>(g + off);
 }
 
+// Element i of a global array with a uniform base (32-bit offsets).
+__device__ inline uint32_t load_u32(const uint32_t* base, uint32_t i) {
+    return reinterpret_cast<const ORBFE_GLOBAL uint32_t*>(reinterpret_cast<uintptr_t>(base))[i];
+}
+
 __device__ inline TapEntry load_tap(const TapEntry* t, uint32_t i) {
     const ORBFE_GLOBAL uint32_t* g =
         reinterpret_cast<const ORBFE_GLOBAL uint32_t*>(reinterpret_cast<uintptr_t>(t)) + 2 * i;
@@ -89,14 +111,10 @@ __device__ inline T copy_const(const ORBFE_CONST T* p) {
 
 using ConstPlan = const ORBFE_CONST Plan&;
 
-__device__ inline uint32_t find_job(const uint32_t* prefix, uint32_t n, uint32_t b) {
-    // largest j with prefix[j] <= b (prefix[0] = 0, prefix[n] = total)
-    uint32_t lo = 0, hi = n;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (as_const(prefix + mid) <= b) lo = mid; else hi = mid;
-    }
-    return lo;
+// Job of block b from the batch's u16 block-job table (one scalar load).
+__device__ inline uint32_t job_of(const uint16_t* table, uint32_t b) {
+    const uint32_t w = as_const(reinterpret_cast<const uint32_t*>(table) + (b >> 1));
+    return (w >> ((b & 1u) * 16u)) & 0xFFFFu;
 }
 
 // Level-l texel (x, y) computed from the u8 frame: level 0 is float(u8)/255
@@ -181,12 +199,168 @@ __device__ float smooth_direct(const Job& J, ConstPlan P, const LevelGeom& G, ui
     return sum;
 }
 
+// Everything a workgroup needs to know about one detect tile, from scalar
+// loads: the frame's job, the level geometry, the tile origin, the tap
+// table offsets and the source rectangle of level 0 that the tile's texels
+// sample (staged columns/rows with g >= 0 map monotonically onto the
+// frame, so their source texels form one rectangle; negative g, which Tint
+// sends to the far edge, and rectangles that do not fit use gathers).
+struct DetectTile {
+    Job J;
+    LevelGeom G;
+    uint32_t l, tile_x, tile_y;
+    int32_t x0, y0;
+    uint32_t t_lc, t_lr, t_sc, t_sr;
+    uint32_t c_first, r_first, c_last, r_last;
+    uint32_t rowmin, cs, nd, nrows;
+    bool use_src;
+};
+
+template <int TH>
+__device__ inline DetectTile decode_tile(const BatchView& bv, const Job& job0, const Params& prm,
+                                         uint32_t b) {
+    constexpr int LH = TH + 2 * kHalo;
+    constexpr uint32_t kSrcWords = stage_words<TH>();
+    DetectTile T;
+    uint32_t j = 0;
+    if (bv.n) {
+        j = job_of(bv.tile_job, b);
+        b -= as_const(bv.tile_prefix + j);
+    }
+    T.J = bv.n ? copy_const(const_ptr(bv.jobs + j)) : job0;
+    ConstPlan P = as_const(T.J.plan);
+    uint32_t l = 0;
+    while (l + 1 < P.L && b >= P.lv[l + 1].tile_base) ++l;
+    T.l = l;
+    T.G = copy_const(&P.lv[l]);  // by value: stores cannot alias it
+    const LevelGeom& G = T.G;
+    const uint32_t t = b - G.tile_base;
+    T.tile_y = t / G.tiles_x;
+    T.tile_x = t - T.tile_y * G.tiles_x;
+    const int32_t x0 = static_cast<int32_t>(T.tile_x) * kTileW;
+    const int32_t y0 = static_cast<int32_t>(T.tile_y) * TH;
+    T.x0 = x0;
+    T.y0 = y0;
+    const uint32_t w = G.w, h = G.h;
+    level_tap_offsets(G.tap_off, w, h, &T.t_lc, &T.t_lr, &T.t_sc, &T.t_sr);
+    T.c_first = x0 >= kHalo ? 0u : static_cast<uint32_t>(kHalo - x0);
+    T.r_first = y0 >= kHalo ? 0u : static_cast<uint32_t>(kHalo - y0);
+    T.c_last = kLdsW - 1;
+    T.r_last = LH - 1;
+    if (prm.border_zero) {
+        const int32_t cl_ = static_cast<int32_t>(w) - 1 - x0 + kHalo;
+        const int32_t rl_ = static_cast<int32_t>(h) - 1 - y0 + kHalo;
+        T.c_last = cl_ < kLdsW - 1 ? static_cast<uint32_t>(cl_ < 0 ? 0 : cl_) : kLdsW - 1;
+        T.r_last = rl_ < LH - 1 ? static_cast<uint32_t>(rl_ < 0 ? 0 : rl_) : LH - 1;
+    }
+    // Level coordinate of staged column/row g (textureLoad robustness rule)
+    // and the rectangle's corners: the taps of the first and last staged
+    // column and row.
+    const auto clamp_coord = [&](int32_t g, uint32_t n) -> uint32_t {
+        const bool inside = g >= 0 && g < static_cast<int32_t>(n);
+        return prm.border_zero ? (inside ? static_cast<uint32_t>(g) : 0u) : tint_clamp(g, n);
+    };
+    const TapEntry* taps = P.taps;
+    const auto tap_i01 = [&](uint32_t idx) { return copy_const(const_ptr(taps + idx)).i01; };
+    uint32_t colmin = clamp_coord(x0 - kHalo + static_cast<int32_t>(T.c_first), w);
+    uint32_t colmax = clamp_coord(x0 - kHalo + static_cast<int32_t>(T.c_last), w);
+    uint32_t rowmin = clamp_coord(y0 - kHalo + static_cast<int32_t>(T.r_first), h);
+    uint32_t rowmax = clamp_coord(y0 - kHalo + static_cast<int32_t>(T.r_last), h);
+    if (l > 0) {
+        colmin = tap_i01(T.t_lc + colmin) & 0xFFFFu;
+        colmax = tap_i01(T.t_lc + colmax) >> 16;
+        rowmin = tap_i01(T.t_lr + rowmin) & 0xFFFFu;
+        rowmax = tap_i01(T.t_lr + rowmax) >> 16;
+    }
+    T.rowmin = rowmin;
+    T.cs = colmin & ~3u;
+    T.nd = (colmax - T.cs) / 4 + 1;  // dwords per source row
+    T.nrows = rowmax - rowmin + 1;
+    T.use_src = colmax >= colmin && rowmax >= rowmin && T.nd <= 64 &&
+                (T.J.pitch & 3u) == 0 && (reinterpret_cast<uintptr_t>(T.J.img) & 3u) == 0 &&
+                T.nd * T.nrows <= kSrcWords;
+    return T;
+}
+
+// Set-up items of a tile, one per thread: e < kLdsW column taps, then LH row
+// taps, then 3 x TH row taps of the 9-tap Gaussian, then TH cull-mask words
+// (row-0 neighbour texels of row-0 tiles follow; they are not prefetched).
+template <int TH>
+struct SetupItems {
+    static constexpr uint32_t kTap = kLdsW + TH + 2 * kHalo;
+    static constexpr uint32_t kSrow = kTap + 3 * TH;
+    static constexpr uint32_t kMask = kSrow + TH;
+};
+
+// Loads of a tile's set-up item e (raw dwords) and of its first kPF source
+// rows per wave, issued together before any of them is stored.
+template <int TH, uint32_t kPF>
+__device__ inline void prefetch_tile(const DetectTile& T, const Params& prm, uint32_t tid,
+                                     uint32_t (&pf)[kPF], uint2& item) {
+    using SI = SetupItems<TH>;
+    constexpr uint32_t kWaves = kDetectThreads / 64;
+    const uint32_t lane = tid & 63, wave = tid >> 6;
+    // Unconditional loads from clamped (in-range) addresses; the stores of
+    // the staging step pick the rows and lanes that belong to the rectangle.
+    const uint32_t* base = reinterpret_cast<const uint32_t*>(T.J.img);
+    uint32_t dw = 0, dpitch = 0, rmax = 0;
+    if (T.use_src) {
+        dw = static_cast<uint32_t>((T.cs + T.rowmin * T.J.pitch) / 4) + umin(lane, T.nd - 1);
+        dpitch = static_cast<uint32_t>(T.J.pitch) / 4;
+        rmax = T.nrows - 1;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kPF; ++k)
+        pf[k] = load_u32(base, dw + umin(wave + k * kWaves, rmax) * dpitch);
+    ConstPlan P = as_const(T.J.plan);
+    const uint32_t e = tid;
+    const uint32_t w = T.G.w, h = T.G.h;
+    item = make_uint2(0u, 0u);
+    if (e < SI::kTap) {
+        const bool is_col = e < kLdsW;
+        const uint32_t c = is_col ? e : e - kLdsW;
+        const int32_t g = (is_col ? T.x0 : T.y0) - kHalo + static_cast<int32_t>(c);
+        const uint32_t n = is_col ? w : h;
+        const bool inside = g >= 0 && g < static_cast<int32_t>(n);
+        const uint32_t cl = prm.border_zero ? (inside ? static_cast<uint32_t>(g) : 0u)
+                                            : tint_clamp(g, n);
+        if (T.l > 0) {
+            const TapEntry tp = load_tap(P.taps, (is_col ? T.t_lc : T.t_lr) + cl);
+            item = make_uint2(tp.i01, __float_as_uint(tp.frac));
+        } else {
+            item = make_uint2(cl | (cl << 16), 0u);
+        }
+    } else if (e < SI::kSrow) {
+        const uint32_t q = e - SI::kTap;
+        const uint32_t jj = q / TH, rr = q - jj * TH;
+        const uint32_t y = static_cast<uint32_t>(T.y0) + rr;
+        const uint32_t sh = h + kSmoothExt;
+        if (y < sh) {  // rows past the extended grid are not smoothed
+            const TapEntry tp = load_tap(P.taps, T.t_sr + jj * sh + y);
+            item = make_uint2(tp.i01, __float_as_uint(tp.frac));
+        }
+    } else if (e < SI::kMask) {
+        const uint32_t rr = e - SI::kSrow;
+        const uint32_t y = static_cast<uint32_t>(T.y0) + rr;
+        if (T.tile_x < T.G.wpr && y < h) {
+            const uint64_t m = P.mask[T.G.word_off + y * T.G.wpr + T.tile_x];
+            item = make_uint2(static_cast<uint32_t>(m), static_cast<uint32_t>(m >> 32));
+        }
+    }
+}
+
+// One workgroup per tile.
 template <int TH>
 __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job0,
                                                             Params prm) {
     constexpr int LH = TH + 2 * kHalo;                     // staged rows
     constexpr int RPT = TH / (kDetectThreads / 64);        // rows per thread
     constexpr uint32_t kSrcWords = stage_words<TH>();
+    constexpr uint32_t kWaves = kDetectThreads / 64;
+    // first source rows per wave loaded ahead (level scales up to 2.5)
+    constexpr uint32_t kPF = (LH * 5 / 2 + 4 + kWaves - 1) / kWaves;
+    using SI = SetupItems<TH>;
+    static_assert(SI::kMask <= kDetectThreads, "one set-up item per thread");
     __shared__ float lut[256];
     __shared__ float tile[LH][kLdsW];
     // u8 source region while staging, then hbuf[3][LH][64], the horizontal
@@ -207,86 +381,61 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     __shared__ uint64_t mask_lds[TH];       // cull-mask word of each tile row
 
     const uint32_t tid = threadIdx.x;
-    uint32_t b = blockIdx.x;
-    uint32_t j = 0;
-    if (bv.n) {
-        j = find_job(bv.tile_prefix, bv.n, b);
-        b -= as_const(bv.tile_prefix + j);
-    }
-    const Job J = bv.n ? copy_const(const_ptr(bv.jobs + j)) : job0;
+    const uint32_t lane = tid & 63;
+    const uint32_t wave = tid >> 6;
+    const float wts[3] = {0.3125f, 0.375f, 0.3125f};
+    for (uint32_t i = tid; i < 256; i += kDetectThreads) lut[i] = static_cast<float>(i) / 255.0f;
+
+    const DetectTile T = decode_tile<TH>(bv, job0, prm, blockIdx.x);
+    uint32_t pf[kPF];
+    uint2 item;
+    prefetch_tile<TH, kPF>(T, prm, tid, pf, item);
+    const Job& J = T.J;
     ConstPlan P = as_const(J.plan);
-    uint32_t l = 0;
-    while (l + 1 < P.L && b >= P.lv[l + 1].tile_base) ++l;
-    const LevelGeom G = copy_const(&P.lv[l]);  // by value: stores below cannot alias it
-    const uint32_t t = b - G.tile_base;
-    const uint32_t tile_y = t / G.tiles_x;
-    const uint32_t tile_x = t - tile_y * G.tiles_x;
-    const int32_t x0 = static_cast<int32_t>(tile_x) * kTileW;
-    const int32_t y0 = static_cast<int32_t>(tile_y) * TH;
+    const LevelGeom& G = T.G;
+    const uint32_t l = T.l, tile_x = T.tile_x;
+    const int32_t x0 = T.x0, y0 = T.y0;
     const uint32_t w = G.w, h = G.h;
     const uint32_t sw = w + kSmoothExt, sh = h + kSmoothExt;
-    const float wts[3] = {0.3125f, 0.375f, 0.3125f};
-
-    lut[tid] = static_cast<float>(tid) / 255.0f;
-    uint32_t t_lc, t_lr, t_sc, t_sr;
-    level_tap_offsets(G.tap_off, w, h, &t_lc, &t_lr, &t_sc, &t_sr);
+    const uint32_t t_sc = T.t_sc;
     const TapEntry* taps = P.taps;
+    const uint32_t c_first = T.c_first, r_first = T.r_first;
+    const uint32_t c_last = T.c_last, r_last = T.r_last;
+    const uint32_t rowmin = T.rowmin, cs = T.cs, nd = T.nd, nrows = T.nrows;
+    const bool use_src = T.use_src;
+    const uint8_t* col0 = J.img + cs;
 
-    // Set-up items, spread over the block: per-column / per-row source
-    // taps of the staged region, row taps of the 9-tap Gaussian for the
-    // TH output rows, the cull-mask words of the tile rows and, on row-0
-    // tiles, the texels of the four row-0 neighbours outside the tile
-    // (x0-2, x0-1, x0+64, x0+65) whose scores the 9-aligned windows of
-    // cull_corners.wgsl:36-56 may need.  Taps come from the plan's tables.
+    // Set-up items and the row-0 neighbour texels into LDS.
     const bool has_row0 = y0 == 0 && tile_x < G.wpr;
-    constexpr uint32_t kTapItems = kLdsW + LH;
-    constexpr uint32_t kSrowItems = kTapItems + 3 * TH;
-    constexpr uint32_t kMaskItems = kSrowItems + TH;
-    const uint32_t n_items = kMaskItems + (has_row0 ? 4 * 17 : 0);
-    for (uint32_t e = tid; e < n_items; e += kDetectThreads) {
-        if (e < kTapItems) {
+    {
+        const uint32_t e = tid;
+        if (e < SI::kTap) {
             const bool is_col = e < kLdsW;
             const uint32_t c = is_col ? e : e - kLdsW;
             const int32_t g = (is_col ? x0 : y0) - kHalo + static_cast<int32_t>(c);
             const uint32_t n = is_col ? w : h;
             const bool inside = g >= 0 && g < static_cast<int32_t>(n);
-            const uint32_t cl = prm.border_zero ? (inside ? static_cast<uint32_t>(g) : 0u)
-                                                : tint_clamp(g, n);
-            uint32_t i0 = cl, i1 = cl;
-            float frac = 0.0f;
-            if (l > 0) {
-                const TapEntry tp = load_tap(taps, (is_col ? t_lc : t_lr) + cl);
-                i0 = tp.i01 & 0xFFFFu;
-                i1 = tp.i01 >> 16;
-                frac = tp.frac;
-            }
+            const uint32_t i0 = item.x & 0xFFFFu, i1 = item.x >> 16;
+            const float frac = __uint_as_float(item.y);
             const uint32_t ok = (!prm.border_zero || inside) ? 1u : 0u;
             if (is_col) {
                 col_i0[c] = i0; col_i1[c] = i1; col_a[c] = frac; col_ok[c] = ok;
             } else {
                 row_i0[c] = i0; row_i1[c] = i1; row_b[c] = frac; row_ok[c] = ok;
             }
-        } else if (e < kSrowItems) {
-            const uint32_t q = e - kTapItems;
+        } else if (e < SI::kSrow) {
+            const uint32_t q = e - SI::kTap;
             const uint32_t jj = q / TH, rr = q - jj * TH;
-            const uint32_t y = static_cast<uint32_t>(y0) + rr;
-            uint32_t i01 = 0;
-            float frac = 0.0f;
-            if (y < sh) {  // rows past the extended grid are not smoothed
-                const TapEntry tp = load_tap(taps, t_sr + jj * sh + y);
-                i01 = tp.i01;
-                frac = tp.frac;
-            }
-            srow_i0[jj][rr] = i01 & 0xFFFFu;
-            srow_i1[jj][rr] = i01 >> 16;
-            srow_b[jj][rr] = frac;
-        } else if (e < kMaskItems) {
-            const uint32_t rr = e - kSrowItems;
-            const uint32_t y = static_cast<uint32_t>(y0) + rr;
-            mask_lds[rr] = (tile_x < G.wpr && y < h) ? P.mask[G.word_off + y * G.wpr + tile_x]
-                                                      : 0ull;
-        } else {
-            const uint32_t q = e - kMaskItems;
+            srow_i0[jj][rr] = item.x & 0xFFFFu;
+            srow_i1[jj][rr] = item.x >> 16;
+            srow_b[jj][rr] = __uint_as_float(item.y);
+        } else if (e < SI::kMask) {
+            mask_lds[e - SI::kSrow] = static_cast<uint64_t>(item.x) |
+                                      (static_cast<uint64_t>(item.y) << 32);
+        }
+    }
+    if (has_row0) {
+        for (uint32_t q = tid; q < 4 * 17; q += kDetectThreads) {
             const uint32_t k = q / 17, i = q - k * 17;
             const int32_t xe = k < 2 ? x0 - 2 + static_cast<int32_t>(k)
                                      : x0 + kTileW + static_cast<int32_t>(k) - 2;
@@ -308,38 +457,20 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
             row0_ring[k][i] = v;
         }
     }
-    __syncthreads();
-
-    // Staged columns/rows with g >= 0 map monotonically onto the frame, so
-    // their source texels form one rectangle; negative g (Tint sends those
-    // to the far edge) and levels whose rectangle does not fit use gathers.
-    const uint32_t c_first = x0 >= kHalo ? 0u : static_cast<uint32_t>(kHalo - x0);
-    const uint32_t r_first = y0 >= kHalo ? 0u : static_cast<uint32_t>(kHalo - y0);
-    uint32_t c_last = kLdsW - 1, r_last = LH - 1;
-    if (prm.border_zero) {
-        const int32_t cl_ = static_cast<int32_t>(w) - 1 - x0 + kHalo;
-        const int32_t rl_ = static_cast<int32_t>(h) - 1 - y0 + kHalo;
-        c_last = cl_ < kLdsW - 1 ? static_cast<uint32_t>(cl_ < 0 ? 0 : cl_) : kLdsW - 1;
-        r_last = rl_ < LH - 1 ? static_cast<uint32_t>(rl_ < 0 ? 0 : rl_) : LH - 1;
+    // Source rows loaded above, then the rows past the first kPF per wave
+    // (taller rectangles): batches of kBatch loads, each issued before its
+    // LDS stores.
+    {
+        const bool src_lane = use_src && lane < nd;
+#pragma unroll
+        for (uint32_t k = 0; k < kPF; ++k) {
+            const uint32_t rr = wave + k * kWaves;
+            if (src_lane && rr < nrows) src[rr * nd + lane] = pf[k];
+        }
     }
-    const uint32_t colmin = col_i0[c_first], colmax = col_i1[c_last];
-    const uint32_t rowmin = row_i0[r_first], rowmax = row_i1[r_last];
-    const uint32_t cs = colmin & ~3u;
-    const uint32_t nd = (colmax - cs) / 4 + 1;  // dwords per source row
-    const uint32_t nrows = rowmax - rowmin + 1;
-    const bool use_src = colmax >= colmin && rowmax >= rowmin && nd <= 64 &&
-                         (J.pitch & 3u) == 0 && (reinterpret_cast<uintptr_t>(J.img) & 3u) == 0 &&
-                         nd * nrows <= kSrcWords;
-    const uint32_t lane = tid & 63;
-    const uint32_t wave = tid >> 6;
-    constexpr uint32_t kWaves = kDetectThreads / 64;
     if (use_src) {
-        // Wave v copies rows v, v+4, ...; lane d copies dword d of a row.
-        // Each group of kBatch loads is issued before any of its LDS
-        // stores, so their latencies overlap.
         constexpr uint32_t kBatch = 8;
-        const uint8_t* col0 = J.img + cs;
-        for (uint32_t r0 = wave; r0 < nrows; r0 += kWaves * kBatch) {
+        for (uint32_t r0 = wave + kPF * kWaves; r0 < nrows; r0 += kWaves * kBatch) {
             uint32_t v[kBatch];
 #pragma unroll
             for (uint32_t k = 0; k < kBatch; ++k) {
@@ -561,7 +692,7 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     }
     __syncthreads();
 
-    if (tile_x >= G.wpr) return;  // extension-only tile: no corners
+    if (tile_x < G.wpr) {  // extension-only tiles have no corners
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
         const uint32_t r = wave * RPT + k;
@@ -585,6 +716,7 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
         }
         const uint64_t bits = __ballot(keep);
         if (lane == 0) J.surv[G.word_off + y * G.wpr + tile_x] = bits;
+    }
     }
 }
 
@@ -739,19 +871,28 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
     // orientation radii up to 3, the moment disc.
     constexpr int32_t kBoxR = 3;
     constexpr uint32_t kBoxSide = 2 * kBoxR + 1, kBoxN = kBoxSide * kBoxSide;
+    // Sample points of one half of the tests (the tests of one parity): point
+    // q is A (q even) or B (q odd) of test 2 * (q >> 1) + half.
+    constexpr uint32_t kHalfPts = 256;
+    constexpr uint32_t PPL = kHalfPts / LANES;         // points per lane and half
+    constexpr uint32_t kSlots = kHalfPts + kHalfPts / 16;  // point q at q + q / 16
     __shared__ float lut[256];
-    __shared__ float4 pat[256];          // BRIEF pattern as floats (ax, ay, bx, by)
+    __shared__ float2 ppt[2][kHalfPts];  // BRIEF pattern point of each half and q
     __shared__ float box[KPB][kBoxN];    // per keypoint: level texels around it
+    __shared__ float samp[KPB][kSlots];  // per keypoint: smoothed value of each point
     const uint32_t tid = threadIdx.x;
     lut[tid] = static_cast<float>(tid) / 255.0f;
-    pat[tid] = make_float4(c_pattern[tid][0], c_pattern[tid][1], c_pattern[tid][2],
-                           c_pattern[tid][3]);
+    {
+        const uint32_t q = tid, i = 2 * (q >> 1), c = (q & 1u) * 2;
+        ppt[0][q] = make_float2(c_pattern[i][c], c_pattern[i][c + 1]);
+        ppt[1][q] = make_float2(c_pattern[i + 1][c], c_pattern[i + 1][c + 1]);
+    }
     __syncthreads();
 
     uint32_t b = blockIdx.x;
     uint32_t j = 0;
     if (bv.n) {
-        j = find_job(bv.desc_prefix, bv.n, b);
+        j = job_of(bv.desc_job, b);
         b -= as_const(bv.desc_prefix + j);
     }
     const Job J = bv.n ? copy_const(const_ptr(bv.jobs + j)) : job0;
@@ -848,38 +989,71 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
     float sn, cs;
     sincos_own(theta, &sn, &cs);
 
-    // Descriptor (generate_features.wgsl:101-144): lane s runs tests
-    // TPL*s .. TPL*s + TPL - 1; G at the wrapped u32 coordinates (:116-129)
-    // is looked up in the smoothed image, where coordinates past the
-    // extension (including wrapped negatives) share the value at w+3 / h+3.
+    // Descriptor (generate_features.wgsl:101-144): G at the wrapped u32
+    // coordinates (:116-129) is looked up in the smoothed image, where
+    // coordinates past the extension (including wrapped negatives) share the
+    // value at w+3 / h+3.  The 512 sample points are gathered in two halves
+    // (the tests of each parity) into LDS, then lane s evaluates tests
+    // TPL*s .. TPL*s + TPL - 1 from there.  The gather order comes from a
+    // table per angle bin that sorts the points by their position rotated to
+    // the bin's centre, so the lanes of one load touch few rows; it only
+    // orders the loads, every value is the one at the point's own exact
+    // rotated coordinates.
     //
     // round() is half-to-even.  For |v| < 2^22, v + 1.5*2^23 lies in
     // [2^23, 2^24), where the float spacing is 1, so the addition rounds v
     // to the nearest integer, ties to even, and leaves that integer in the
     // mantissa: i32(round(v)) = bits(v + 1.5*2^23) - bits(1.5*2^23).  The
     // pattern coordinates are at most 13 in magnitude, so |v| <= 19.  The
-    // subtraction is folded into the keypoint coordinate (mod 2^32).
+    // subtraction is folded into the keypoint coordinate (mod 2^32); columns
+    // are clamped as byte offsets, 4 * (x + ax) staying above 4 * (sw - 1)
+    // for the wrapped negatives.
     constexpr float kRound = 12582912.0f;  // 1.5 * 2^23
     constexpr uint32_t kRoundBits = 0x4B400000u;
     const uint32_t sw = G.w + kSmoothExt, sh = G.h + kSmoothExt;
     const uint8_t* sm = reinterpret_cast<const uint8_t*>(J.smooth + G.sm_off);
-    const uint32_t xb = x - kRoundBits, yb = y - kRoundBits;
-    // Columns are clamped as byte offsets: 4 * (x + ax) stays above
-    // 4 * (sw - 1) for the wrapped negatives (|ax| <= 19).
-    const uint32_t xb4 = 4 * xb, sw4 = 4 * sw, xmax4 = 4 * (sw - 1);
+    const uint32_t yb = y - kRoundBits;
+    const uint32_t xb4 = 4 * (x - kRoundBits), sw4 = 4 * sw, xmax4 = 4 * (sw - 1);
+    const uint32_t bin = umin(static_cast<uint32_t>((theta + fm::kPi) * kAngleBinScale),
+                              kAngleBins - 1);
+    const uint32_t* order = reinterpret_cast<const uint32_t*>(
+        desc_order<LANES>() + (static_cast<size_t>(bin) * 2 * LANES + sub) * PPL);
+    float* ks = samp[tid / LANES];
     uint32_t bits = 0;
 #pragma unroll
-    for (int k = static_cast<int>(TPL) - 1; k >= 0; --k) {
-        const float4 pt = pat[TPL * sub + k];
-        const float pax = pt.x, pay = pt.y, pbx = pt.z, pby = pt.w;
-        const uint32_t ax4 = umin(4 * __float_as_uint((pax * cs - pay * sn) + kRound) + xb4, xmax4);
-        const uint32_t ay = umin(__float_as_uint((pax * sn + pay * cs) + kRound) + yb, sh - 1);
-        const uint32_t bx4 = umin(4 * __float_as_uint((pbx * cs - pby * sn) + kRound) + xb4, xmax4);
-        const uint32_t by = umin(__float_as_uint((pbx * sn + pby * cs) + kRound) + yb, sh - 1);
-        // byte offsets < 4 * sw * sh (24-bit factors)
-        const float sa = load_f32(sm, __umul24(ay, sw4) + ax4);
-        const float sb = load_f32(sm, __umul24(by, sw4) + bx4);
-        bits = (bits << 1) | (sa > sb ? 1u : 0u);
+    for (uint32_t half = 0; half < 2; ++half) {
+        uint32_t ids[PPL / 4];
+#pragma unroll
+        for (uint32_t k = 0; k < PPL / 4; ++k) ids[k] = order[half * LANES * PPL / 4 + k];
+        constexpr uint32_t kB = PPL < 8 ? PPL : 8;  // loads in flight per batch
+#pragma unroll
+        for (uint32_t k0 = 0; k0 < PPL; k0 += kB) {
+            float v[kB];
+            uint32_t qs[kB];
+#pragma unroll
+            for (uint32_t u = 0; u < kB; ++u) {
+                const uint32_t k = k0 + u;
+                const uint32_t q = (ids[k / 4] >> (8 * (k % 4))) & 0xFFu;
+                qs[u] = q;
+                const float2 pp = ppt[half][q];
+                const uint32_t ax4 =
+                    umin(4 * __float_as_uint((pp.x * cs - pp.y * sn) + kRound) + xb4, xmax4);
+                const uint32_t ay =
+                    umin(__float_as_uint((pp.x * sn + pp.y * cs) + kRound) + yb, sh - 1);
+                // byte offsets < 4 * sw * sh (24-bit factors)
+                v[u] = load_f32(sm, __umul24(ay, sw4) + ax4);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kB; ++u) ks[qs[u] + (qs[u] >> 4)] = v[u];
+        }
+        wave_sync();
+#pragma unroll
+        for (uint32_t k = half; k < TPL; k += 2) {
+            const uint32_t qa = 2 * ((TPL * sub + k) >> 1);
+            const float sa = ks[qa + (qa >> 4)], sb = ks[qa + 1 + ((qa + 1) >> 4)];
+            bits |= (sa > sb ? 1u : 0u) << k;
+        }
+        wave_sync();
     }
     // Record word e: x, y, strength, angle, then descriptor word e - 4 =
     // bits of lanes LPW*(e-4) .. LPW*(e-4) + LPW - 1, low lane first.
@@ -1030,11 +1204,6 @@ __device__ inline uint32_t group_inclusive_sum(uint32_t v, uint32_t sub) {
     return v;
 }
 
-// Element i of a global array with a uniform base (32-bit offsets).
-__device__ inline uint32_t load_u32(const uint32_t* base, uint32_t i) {
-    return reinterpret_cast<const ORBFE_GLOBAL uint32_t*>(reinterpret_cast<uintptr_t>(base))[i];
-}
-
 __device__ inline float2 load_f2(const float2* base, uint32_t i) {
     const f32x2 v =
         reinterpret_cast<const ORBFE_GLOBAL f32x2*>(reinterpret_cast<uintptr_t>(base))[i];
@@ -1071,7 +1240,7 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
     uint32_t b = blockIdx.x;
     uint32_t j = 0;
     if (bv.n) {
-        j = find_job(bv.block_prefix, bv.n, b);
+        j = job_of(bv.block_job, b);
         b -= as_const(bv.block_prefix + j);
     }
     const MatchJob M = bv.n ? copy_const(const_ptr(bv.jobs + j)) : job0;
@@ -1317,8 +1486,47 @@ __global__ __launch_bounds__(256) void k_bf_merge(const uint2* __restrict__ part
 // ---------------------------------------------------------------------------
 // Host-side launchers (called from orbfe_api.cpp).
 // ---------------------------------------------------------------------------
+// Gather order of the describe kernel (g_desc_order): for each angle bin,
+// the 256 points of each half sorted by (row, column) of their position
+// rotated to the bin's centre angle; the lanes of a group take consecutive
+// points of that order in each load.  The order only affects which lane
+// loads which point, never a value.
+static void build_desc_order(uint8_t* out, uint32_t lanes) {
+    const uint32_t ppl = 256 / lanes;
+    for (uint32_t bin = 0; bin < kAngleBins; ++bin) {
+        const double th = -3.14159265358979323846 + (bin + 0.5) * (2.0 * 3.14159265358979323846 / kAngleBins);
+        const double c = std::cos(th), sn = std::sin(th);
+        for (uint32_t half = 0; half < 2; ++half) {
+            uint32_t seq[256];
+            long key[256];
+            for (uint32_t q = 0; q < 256; ++q) {
+                const uint32_t i = 2 * (q >> 1) + half, o = (q & 1u) * 2;
+                const double px = kOrbfeBriefPattern[i][o], py = kOrbfeBriefPattern[i][o + 1];
+                const long rx = std::lround(px * c - py * sn), ry = std::lround(px * sn + py * c);
+                key[q] = (ry + 64) * 256 + (rx + 64);
+                seq[q] = q;
+            }
+            std::stable_sort(seq, seq + 256, [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+            uint8_t* dst = out + (static_cast<size_t>(bin) * 2 + half) * 256;
+            for (uint32_t s = 0; s < lanes; ++s)
+                for (uint32_t k = 0; k < ppl; ++k) dst[s * ppl + k] = static_cast<uint8_t>(seq[lanes * k + s]);
+        }
+    }
+}
+
 hipError_t upload_pattern() {
-    return hipMemcpyToSymbol(HIP_SYMBOL(c_pattern), kOrbfeBriefPattern, sizeof(kOrbfeBriefPattern));
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_pattern), kOrbfeBriefPattern,
+                                     sizeof(kOrbfeBriefPattern));
+    if (e != hipSuccess) return e;
+    static uint8_t order[3][kDescOrderBytes];
+    static bool built = false;
+    if (!built) {
+        build_desc_order(order[0], 8);
+        build_desc_order(order[1], 16);
+        build_desc_order(order[2], 32);
+        built = true;
+    }
+    return hipMemcpyToSymbol(HIP_SYMBOL(g_desc_order), order, sizeof(order));
 }
 
 hipError_t launch_detect(uint32_t tile_h, uint32_t blocks, BatchView bv, const Job& job0,
@@ -1331,6 +1539,7 @@ hipError_t launch_detect(uint32_t tile_h, uint32_t blocks, BatchView bv, const J
     }
     return hipGetLastError();
 }
+
 
 hipError_t launch_compact(uint32_t blocks, BatchView bv, const Job& job0, uint32_t levels,
                           hipStream_t s) {
