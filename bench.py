@@ -102,7 +102,11 @@ def cpu_baseline(frames, n_sample):
     return n_sample / dt, dt
 
 
-INT_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 int32 lanes/clk x 2.4 GHz
+# MI355X published dense FP32 vector rate: 157.3 TFLOP/s = 78.6 T FMA lanes/s,
+# i.e. 2 wave64 VALU instructions per CU per clock at 2.4 GHz.  The integer
+# rooflines use the sustained rate measured on the device itself
+# (orbfe_probe_valu_rate), which bench.py reports beside it.
+PUBLISHED_VALU_WAVE_INSTR_PER_S = 256 * 2 * 2.4e9
 
 
 def capture_graph(torch, launch):
@@ -186,7 +190,10 @@ def phase_mixed(torch, orbfe, synth, reps):
     t_plain = timed(torch, plain, reps)
     tune = autotune.tune_detect(frames, 752, 480, reps=max(3, reps), fast_threshold=thr20(),
                                 lowe_ratio=RATIO)
-    ctx = orbfe.Context(752, 480, fast_threshold=thr20(), lowe_ratio=RATIO, **tune.config())
+    mtune = autotune.tune_match(pairs, 752, 480, reps=max(3, reps), fast_threshold=thr20(),
+                                lowe_ratio=RATIO)
+    ctx = orbfe.Context(752, 480, fast_threshold=thr20(), lowe_ratio=RATIO, **tune.config(),
+                        **mtune.config())
     b = ctx.batch_create(frames)
     mb = ctx.match_batch_create(pairs)
 
@@ -201,12 +208,24 @@ def phase_mixed(torch, orbfe, synth, reps):
             "batched_frames_per_s": round(n / t_batch, 1),
             "speedup": round(t_plain / t_batch, 2),
             "plain_ms": round(t_plain * 1e3, 3), "batched_ms": round(t_batch * 1e3, 3),
-            "tuned_shape": {"detect_tile_rows": tune.best[0], "describe_lanes": tune.best[1]},
+            "tuned_shape": {"detect_tile_rows": tune.best[0], "describe_lanes": tune.best[1],
+                            "match_lanes": mtune.best},
             "detect_stage_ms_by_shape": {f"{r}x{d}": round(t * 1e3, 4)
-                                         for (r, d), t in sorted(tune.seconds.items())}}
+                                         for (r, d), t in sorted(tune.seconds.items())},
+            "match_stage_ms_by_lanes": {str(n): round(t * 1e3, 4)
+                                        for n, t in sorted(mtune.seconds.items())}}
 
 
-def phase_matcher(torch, orbfe, synth, n, reps):
+def valu_peak(ctx):
+    """Sustained VALU issue rate (wave64 instructions/s) and the integer lane
+    rate it gives (Tops/s)."""
+    rate = ctx.probe_valu_rate()
+    return {"wave_instr_per_s": float(f"{rate:.4g}"), "int_Tops": round(rate * 64 / 1e12, 2),
+            "published_wave_instr_per_s": PUBLISHED_VALU_WAVE_INSTR_PER_S,
+            "source": "orbfe_probe_valu_rate (8 independent xor/add chains per lane, every CU)"}
+
+
+def phase_matcher(torch, orbfe, synth, n, reps, int_peak_tops):
     """Brute-force matcher stress: best/second-best over n x n 256-bit descriptors."""
     q, t, _, _ = synth.descriptors(n, n, n)
     dev = torch.device("cuda:0")
@@ -231,8 +250,8 @@ def phase_matcher(torch, orbfe, synth, n, reps):
             "pairs_per_s": round(pairs / dt, 1),
             "effective_GBps": round(32 * pairs / dt / 1e9, 1),
             "int_Tops": round(16 * pairs / kern / 1e12, 2),
-            "int_peak_Tops": round(INT_PEAK_TOPS, 2),
-            "int_frac": round(16 * pairs / kern / 1e12 / INT_PEAK_TOPS, 4),
+            "int_peak_Tops": round(int_peak_tops, 2),
+            "int_frac": round(16 * pairs / kern / 1e12 / int_peak_tops, 4),
             "k_bf_partial_us": round(kern * 1e6, 2)}
 
 
@@ -322,7 +341,9 @@ def main():
         elif args.workload == "mixed":
             out = phase_mixed(torch, orbfe, synth, max(2, args.steps))
         elif args.workload == "matcher":
-            out = phase_matcher(torch, orbfe, synth, 65536, max(2, args.steps))
+            peak = valu_peak(ctx)
+            out = phase_matcher(torch, orbfe, synth, 65536, max(2, args.steps), peak["int_Tops"])
+            out["valu_peak"] = peak
         else:
             out = phase_serving(torch, orbfe, synth, args.serve_seconds)
         print(json.dumps({"workload": args.workload, **out}), flush=True)
@@ -345,7 +366,9 @@ def main():
     ms_per_step = 1000.0 * dt / args.steps
     fps = N_FRAMES * args.steps / dt
     n_matches = int(sum(int((m >= 0).sum()) for m in matches))
-    log(f"plain: {ms_per_step:.3f} ms/step, {fps:.1f} frames/s, {n_matches} matches/step")
+    mps = n_matches * args.steps / dt
+    log(f"plain: {ms_per_step:.3f} ms/step, {fps:.1f} frames/s, {n_matches} matches/step, "
+        f"{mps:.0f} matches/s")
 
     # Per-kernel durations over the same work, with HIP events on the stream
     # the kernels run on.
@@ -384,7 +407,9 @@ def main():
         log("batched sequence:", json.dumps(bseq))
         extra["mixed_batch"] = phase_mixed(torch, orbfe, synth, max(3, args.steps // 4))
         log("mixed batch:", json.dumps(extra["mixed_batch"]))
-        extra["matcher"] = phase_matcher(torch, orbfe, synth, 65536, 3)
+        extra["valu_peak"] = valu_peak(ctx)
+        extra["matcher"] = phase_matcher(torch, orbfe, synth, 65536, 3,
+                                         extra["valu_peak"]["int_Tops"])
         log("matcher:", json.dumps(extra["matcher"]))
         extra["serving"] = phase_serving(torch, orbfe, synth, args.serve_seconds)
         log("serving:", json.dumps(extra["serving"]))
@@ -408,6 +433,7 @@ def main():
                    "frames_per_step": N_FRAMES, "resolution": [W, H], "cap": CAP,
                    "parallelism": "single GPU"},
         "matches_per_step": n_matches,
+        "matches_per_s": round(mps, 1),
         "per_kernel": per_kernel,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -64,8 +64,9 @@ typedef struct orbfe_config {
        on them).  0 = default.  detect_tile_rows: rows of a detect tile, 8, 16
        (default) or 32.  describe_lanes: lanes per keypoint in the describe
        kernel, 8, 16 (default) or 32.  match_lanes: lanes per query in the
-       pipeline matcher, 16 (default), 32 or 64.  uni_slam_amd/autotune.py
-       picks them by timing the variants on a workload. */
+       pipeline matcher, 16, 32 or 64 (0: 16 for batches, 64 for a single
+       frame pair).  uni_slam_amd/autotune.py picks them by timing the
+       variants on a workload. */
     uint32_t detect_tile_rows;
     uint32_t describe_lanes;
     uint32_t match_lanes;
@@ -222,6 +223,14 @@ int orbfe_timing_enable(struct orbfe_ctx* ctx, int enable);
 int orbfe_timing_read(struct orbfe_ctx* ctx, const char* name, uint32_t* launches,
                       double* total_ms);
 int orbfe_timing_reset(struct orbfe_ctx* ctx);
+
+/* Sustained VALU issue rate of the context's device, in wave64 instructions
+ * per second: a probe kernel of independent 32-bit integer xor/add chains on
+ * every CU, timed with HIP events on `stream` (synchronizes).  No
+ * counterpart in the reference: the peak that the integer rooflines of the
+ * matchers are quoted against. */
+int orbfe_probe_valu_rate(struct orbfe_ctx* ctx, hipStream_t stream,
+                          double* wave_instructions_per_s);
 
 #ifdef __cplusplus
 }

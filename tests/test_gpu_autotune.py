@@ -34,3 +34,27 @@ def test_tuner_variants_identical_and_best_reproduces(torch_gpu):
                            fast_threshold=thr20(), **res.config()) as c:
             got, _ = c.detect(f["image"].cpu().numpy())
         assert (got == wf.cpu().numpy()).all()
+
+
+def test_match_tuner_variants_identical_and_best_reproduces(torch_gpu):
+    torch = torch_gpu
+    dev = torch.device("cuda:0")
+    items = synth.mixed_batch(22, n_frames=4)
+    pairs, plain = [], []
+    for it in items:
+        h, w = it["a"].shape
+        cap = it["cap"]
+        with orbfe.Context(w, h, features_per_level=cap, fast_threshold=thr20(),
+                           lowe_ratio=0.8) as c:
+            fa, _ = c.detect(it["a"])
+            fb, _ = c.detect(it["b"])
+            plain.append(c.match_pipeline(fa, fb)[0])
+        pairs.append({"prev": torch.from_numpy(fa).to(dev), "curr": torch.from_numpy(fb).to(dev),
+                      "prev_match": torch.zeros(6 * cap, dtype=torch.int32, device=dev),
+                      "width": w, "height": h, "cap": cap})
+    print(f"tuning {len(pairs)} pairs over {len(autotune.MATCH_LANES)} shapes", flush=True)
+    res = autotune.tune_match(pairs, 752, 480, reps=2, fast_threshold=thr20(), lowe_ratio=0.8)
+    assert set(res.seconds) == set(autotune.MATCH_LANES)
+    assert res.best in autotune.MATCH_LANES
+    for p, want in zip(pairs, plain):
+        assert (p["prev_match"].cpu().numpy() == want).all()

@@ -30,6 +30,8 @@ hipError_t launch_match_index(uint32_t blocks, MatchBatchView bv, const MatchJob
 hipError_t launch_match_dir(uint32_t lanes, bool mutual, uint32_t blocks, MatchBatchView bv,
                             const MatchJob& job0, float ratio, hipStream_t s);
 uint32_t bf_queries_per_block();
+hipError_t launch_valu_probe(uint32_t* sink, uint32_t blocks, hipStream_t s,
+                             double* instr_per_launch);
 hipError_t launch_bf(const uint32_t* q, uint32_t nq, const uint32_t* t, uint32_t nt,
                      uint32_t splits, uint32_t split_len, uint2* part, uint32_t* idx,
                      uint32_t* d1, uint32_t* d2, hipStream_t s, int phase);
@@ -172,6 +174,8 @@ struct orbfe_ctx {
     int device = 0;
     orbfe_config cfg{};
     Params prm{};
+    uint32_t match_lanes_batch = orbfe::kMatchLanesBatch;  // matcher lanes per query
+    uint32_t match_lanes_pair = orbfe::kMatchLanesPair;
     std::string err;
     std::map<std::tuple<uint32_t, uint32_t, uint32_t>, std::unique_ptr<PlanEntry>> plans;
     PlanEntry* main = nullptr;
@@ -290,8 +294,6 @@ int get_plan(orbfe_ctx* ctx, uint32_t W0, uint32_t H0, uint32_t cap, PlanEntry**
     P.desc_lanes = ctx->cfg.describe_lanes;
     P.feat_stride = 8 + 48 * cap;
     P.desc_groups = ceil_div(cap, 256 / P.desc_lanes);
-    P.match_lanes = ctx->cfg.match_lanes;
-    P.query_groups = ceil_div(cap, 256 / P.match_lanes);
     P.max_dx = 0.1f * static_cast<float>(W0);
     P.max_dy = 0.1f * static_cast<float>(H0);
     P.cell_inv_x = 2.0f / P.max_dx;
@@ -405,19 +407,24 @@ int launch_detect_stage(orbfe_ctx* ctx, const BatchView& bv, const Job& job0, ui
     return ORBFE_OK;
 }
 
+// Matcher blocks of one frame pair of `plan` with `lanes` lanes per query.
+uint32_t match_blocks(const Plan& plan, uint32_t lanes) {
+    return plan.L * ceil_div(plan.cap, 256 / lanes);
+}
+
 int launch_match_stage(orbfe_ctx* ctx, const MatchBatchView& bv, const MatchJob& job0,
-                       uint32_t blocks, hipStream_t s) {
+                       uint32_t lanes, uint32_t blocks, hipStream_t s) {
     const float ratio = ctx->prm.ratio;
     const uint32_t pairs = bv.n ? bv.n : 1;
     HIP_TRY(ctx, timed(ctx, "k_match_index", s, [&] {
                 return orbfe::launch_match_index(pairs * 2 * ctx->cfg.levels, bv, job0, s);
             }));
     HIP_TRY(ctx, timed(ctx, "k_match_dir", s, [&] {
-                return orbfe::launch_match_dir(ctx->cfg.match_lanes, false, blocks, bv, job0,
+                return orbfe::launch_match_dir(lanes, false, blocks, bv, job0,
                                                ratio, s);
             }));
     HIP_TRY(ctx, timed(ctx, "k_match_mutual", s, [&] {
-                return orbfe::launch_match_dir(ctx->cfg.match_lanes, true, blocks, bv, job0,
+                return orbfe::launch_match_dir(lanes, true, blocks, bv, job0,
                                                ratio, s);
             }));
     return ORBFE_OK;
@@ -541,7 +548,7 @@ int layout_match(orbfe_ctx* ctx, const orbfe_match_pair* pairs, uint32_t n, Matc
         off += pe->index_bytes;
         M.index_curr = reinterpret_cast<uint32_t*>(arena + off);
         off += pe->index_bytes;
-        lay.prefix[i + 1] = lay.prefix[i] + pe->host.L * pe->host.query_groups;
+        lay.prefix[i + 1] = lay.prefix[i] + match_blocks(pe->host, ctx->match_lanes_batch);
     }
     lay.arena_bytes = off;
     fill_block_jobs(lay.prefix, lay.block_job);
@@ -684,16 +691,17 @@ int orbfe_create(const orbfe_config* cfg, int device, orbfe_ctx** out) {
     if (cfg->fast_min_run > 16) return ORBFE_ERANGE;
     const uint32_t th = cfg->detect_tile_rows ? cfg->detect_tile_rows : orbfe::kTileHDefault;
     const uint32_t dl = cfg->describe_lanes ? cfg->describe_lanes : orbfe::kDescLanesDefault;
-    const uint32_t ml = cfg->match_lanes ? cfg->match_lanes : orbfe::kMatchLanesDefault;
+    const uint32_t ml = cfg->match_lanes;
     if ((th != 8 && th != 16 && th != 32) || (dl != 8 && dl != 16 && dl != 32) ||
-        (ml != 16 && ml != 32 && ml != 64))
+        (ml != 0 && ml != 16 && ml != 32 && ml != 64))
         return ORBFE_ERANGE;
     auto ctx = std::make_unique<orbfe_ctx>();
     ctx->device = device;
     ctx->cfg = *cfg;
     ctx->cfg.detect_tile_rows = th;  // every plan of a context shares the launch shapes
     ctx->cfg.describe_lanes = dl;
-    ctx->cfg.match_lanes = ml;
+    ctx->match_lanes_batch = ml ? ml : orbfe::kMatchLanesBatch;
+    ctx->match_lanes_pair = ml ? ml : orbfe::kMatchLanesPair;
     ctx->prm.threshold = cfg->fast_threshold;
     ctx->prm.min_run = cfg->fast_min_run;
     ctx->prm.orient_r = cfg->orientation_radius;
@@ -903,7 +911,8 @@ int orbfe_match_pipeline_async(orbfe_ctx* ctx, const void* prev_features,
     job0.index_prev = ctx->index[0];
     job0.index_curr = ctx->index[1];
     const MatchBatchView bv{nullptr, nullptr, nullptr, 0};
-    return launch_match_stage(ctx, bv, job0, P.L * P.query_groups, stream);
+    return launch_match_stage(ctx, bv, job0, ctx->match_lanes_pair,
+                              match_blocks(P, ctx->match_lanes_pair), stream);
 }
 
 int orbfe_match_pipeline_batch_async(orbfe_ctx* ctx, const orbfe_match_pair* pairs, uint32_t n,
@@ -925,7 +934,7 @@ int orbfe_match_pipeline_batch_async(orbfe_ctx* ctx, const orbfe_match_pair* pai
     rc = commit_table(ctx, slot, lay.table_bytes, stream);
     if (rc) return rc;
     const MatchBatchView bv = match_view(lay, static_cast<uint8_t*>(slot->dev));
-    return launch_match_stage(ctx, bv, MatchJob{}, lay.prefix[n], stream);
+    return launch_match_stage(ctx, bv, MatchJob{}, ctx->match_lanes_batch, lay.prefix[n], stream);
 }
 
 int orbfe_match_batch_create(orbfe_ctx* ctx, const orbfe_match_pair* pairs, uint32_t n,
@@ -963,7 +972,8 @@ int orbfe_match_batch_create(orbfe_ctx* ctx, const orbfe_match_pair* pairs, uint
 
 int orbfe_match_batch_async(orbfe_ctx* ctx, orbfe_match_batch* batch, hipStream_t stream) {
     if (!ctx || !batch || batch->ctx != ctx) return fail(ctx, ORBFE_EINVAL, "bad batch");
-    return launch_match_stage(ctx, batch->view, MatchJob{}, batch->blocks, stream);
+    return launch_match_stage(ctx, batch->view, MatchJob{}, ctx->match_lanes_batch, batch->blocks,
+                              stream);
 }
 
 void orbfe_match_batch_destroy(orbfe_match_batch* batch) {
@@ -1036,6 +1046,34 @@ int orbfe_match_bf_async(orbfe_ctx* ctx, const uint32_t* query, uint32_t nq,
                 return orbfe::launch_bf(query, nq, train, nt, splits, split_len, ctx->bf_part,
                                         idx, d1, d2, stream, 1);
             }));
+    return ORBFE_OK;
+}
+
+int orbfe_probe_valu_rate(orbfe_ctx* ctx, hipStream_t stream, double* rate) {
+    if (!ctx || !rate) return ORBFE_EINVAL;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int cus = 0;
+    HIP_TRY(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    const uint32_t blocks = static_cast<uint32_t>(cus) * 32;  // 8 workgroups per CU, 4 rounds
+    uint32_t* sink = nullptr;
+    int rc = dalloc(ctx, &sink, 4ull * blocks);
+    if (rc) return rc;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    double per = 0.0;
+    hipError_t e = hipEventCreate(&ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+    if (e == hipSuccess) e = orbfe::launch_valu_probe(sink, blocks, stream, &per);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(ev[0], stream);
+    for (int r = 0; r < 5 && e == hipSuccess; ++r) e = orbfe::launch_valu_probe(sink, blocks, stream, &per);
+    if (e == hipSuccess) e = hipEventRecord(ev[1], stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+    for (auto x : ev)
+        if (x) (void)hipEventDestroy(x);
+    dfree(sink);
+    if (e != hipSuccess) return fail(ctx, ORBFE_EHIP, "valu probe: %s", hipGetErrorString(e));
+    *rate = 5.0 * per / (static_cast<double>(ms) * 1e-3);
     return ORBFE_OK;
 }
 

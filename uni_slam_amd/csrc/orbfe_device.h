@@ -34,8 +34,11 @@ constexpr int kDetectThreads = 256;
 // desc_lanes keypoints per 256-thread workgroup.
 constexpr uint32_t kDescLanesDefault = 16;
 // k_match_dir: match_lanes lanes per query (16, 32 or 64), so 256 /
-// match_lanes queries per 256-thread workgroup.
-constexpr uint32_t kMatchLanesDefault = 16;
+// match_lanes queries per 256-thread workgroup.  By default a batch uses 16
+// (many queries keep the device full) and a single frame pair 64 (its
+// queries alone do not).
+constexpr uint32_t kMatchLanesBatch = 16;
+constexpr uint32_t kMatchLanesPair = 64;
 // Matcher cell index: LOD0 plane cut into kCellsPerAxis^2 cells of half the
 // search window (0.05*W0 x 0.05*H0); 24 cells cover 1.2x the frame.
 constexpr uint32_t kCellsPerAxis = 24;
@@ -88,13 +91,11 @@ struct Plan {
     uint32_t W0, H0, L, cap;
     uint32_t tile_h;      // detect tile rows
     uint32_t desc_lanes;  // describe lanes per keypoint
-    uint32_t match_lanes; // matcher lanes per query
     uint32_t tiles;       // detect tiles of one frame
     uint32_t words;       // 64-bit survivor words of one frame
     uint32_t pyr_floats;  // pyramid floats (levels >= 1)
     uint32_t feat_stride; // 8 + 48*cap
     uint32_t desc_groups; // describe blocks per level = ceil(cap / (256 / desc_lanes))
-    uint32_t query_groups; // matcher blocks per level = ceil(cap / (256 / match_lanes))
     float max_dx, max_dy; // 0.1f*W0, 0.1f*H0 (match_features_cpu.cpp:76-77)
     uint32_t sm_floats;   // smoothed-image floats of one frame
     float cell_inv_x, cell_inv_y;  // 1 / cell size (cells of max_dx/2 x max_dy/2)

@@ -721,14 +721,18 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
 }
 
 // ---------------------------------------------------------------------------
-// k_compact: one workgroup of 16 waves per (frame, level).  Each wave owns a
-// contiguous run of survivor words, read 64 at a time (coalesced): pass 1
-// counts, a block scan of the 16 wave totals gives each wave its base rank,
-// pass 2 re-reads its words with a wave prefix sum and writes keypoints
-// until the cap is reached.
+// k_compact: one workgroup of 16 waves per (frame, level).  The level's
+// survivor words are cut into chunks of 64 words (one per lane, read
+// coalesced); in each round of up to kRoundChunks chunks, chunk c goes to
+// wave c % 16.  A round counts its chunks (words kept in registers), a block
+// scan over the chunk counts gives every chunk its base rank, then every
+// wave ranks the survivors of its chunks whose base is below the cap and
+// writes their keypoints, all waves at once.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kCompactThreads = 1024;
 constexpr uint32_t kCompactWaves = kCompactThreads / 64;
+constexpr uint32_t kCompactChunks = 8;                             // per wave and round
+constexpr uint32_t kRoundChunks = kCompactWaves * kCompactChunks;  // 128 (8192 words)
 
 __device__ inline uint32_t wave_inclusive_sum(uint32_t v, uint32_t lane) {
 #pragma unroll
@@ -738,11 +742,6 @@ __device__ inline uint32_t wave_inclusive_sum(uint32_t v, uint32_t lane) {
     }
     return v;
 }
-
-// Survivor words a wave keeps in registers (64 lanes x kCompactChunks):
-// enough for a 1920x1080 level 0 (32,400 words over 16 waves); larger
-// levels take the re-reading path.
-constexpr uint32_t kCompactChunks = 32;
 
 __device__ inline void emit_word(uint64_t v, uint32_t k, uint32_t rank, uint32_t cap,
                                  uint32_t wpr, uint32_t* kp) {
@@ -758,7 +757,8 @@ __device__ inline void emit_word(uint64_t v, uint32_t k, uint32_t rank, uint32_t
 
 __global__ __launch_bounds__(kCompactThreads) void k_compact(BatchView bv, Job job0,
                                                               uint32_t levels) {
-    __shared__ uint32_t wave_total[kCompactWaves];
+    __shared__ uint32_t cbase[kRoundChunks];
+    __shared__ uint32_t round_total;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63;
     const uint32_t wave = tid >> 6;
@@ -768,59 +768,55 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(BatchView bv, Job j
     ConstPlan P = as_const(J.plan);
     const LevelGeom G = copy_const(&P.lv[l]);  // by value: stores below cannot alias it
     const uint32_t words = G.h * G.wpr;
+    const uint32_t nchunks = (words + 63) / 64;
     const uint64_t* surv = J.surv + G.word_off;
-    const uint32_t seg = (words + kCompactWaves - 1) / kCompactWaves;
-    const uint32_t begin = wave * seg < words ? wave * seg : words;
-    const uint32_t end = begin + seg < words ? begin + seg : words;
-    const uint32_t nch = (end - begin + 63) / 64;
-    const bool in_regs = nch <= kCompactChunks;  // block-uniform (seg is)
-
-    uint64_t wv[kCompactChunks];
-    uint32_t cnt = 0;
-    if (in_regs) {
-#pragma unroll
-        for (uint32_t c = 0; c < kCompactChunks; ++c) {
-            const uint32_t k = begin + c * 64 + lane;
-            wv[c] = (c < nch && k < end) ? surv[k] : 0ull;
-            cnt += __popcll(wv[c]);
-        }
-    } else {
-        for (uint32_t k = begin + lane; k < end; k += 64) cnt += __popcll(surv[k]);
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
-    if (lane == 0) wave_total[wave] = cnt;
-    __syncthreads();
-    uint32_t base = 0, total = 0;
-    for (uint32_t v = 0; v < kCompactWaves; ++v) {
-        const uint32_t tv = wave_total[v];
-        base += v < wave ? tv : 0u;
-        total += tv;
-    }
     uint32_t* kp = J.kp + G.kp_off;
-    if (in_regs) {
+    const uint32_t cap = P.cap;
+
+    uint32_t carry = 0;  // survivors of the rounds before (block-uniform)
+    for (uint32_t r0 = 0; r0 < nchunks; r0 += kRoundChunks) {
+        uint64_t wv[kCompactChunks];
+        uint32_t cc[kCompactChunks];
 #pragma unroll
         for (uint32_t c = 0; c < kCompactChunks; ++c) {
-            if (c < nch && base < P.cap) {  // wave-uniform
-                const uint32_t c1 = __popcll(wv[c]);
-                const uint32_t incl = wave_inclusive_sum(c1, lane);
-                emit_word(wv[c], begin + c * 64 + lane, base + incl - c1, P.cap, G.wpr, kp);
-                base += __shfl(incl, 63);
+            const uint32_t chunk = r0 + wave + kCompactWaves * c;
+            const uint32_t k = chunk * 64 + lane;
+            wv[c] = (chunk < nchunks && k < words) ? surv[k] : 0ull;
+            cc[c] = __popcll(wv[c]);
+        }
+#pragma unroll
+        for (uint32_t c = 0; c < kCompactChunks; ++c) {
+            uint32_t t = cc[c];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
+            if (lane == 0) cbase[wave + kCompactWaves * c] = t;
+        }
+        __syncthreads();
+        if (wave == 0) {  // exclusive scan of the round's chunk counts, 2 per lane
+            static_assert(kRoundChunks == 128, "two chunks per lane");
+            const uint32_t a = cbase[2 * lane], b2 = cbase[2 * lane + 1];
+            const uint32_t incl = wave_inclusive_sum(a + b2, lane);
+            const uint32_t ex = carry + incl - (a + b2);
+            cbase[2 * lane] = ex;
+            cbase[2 * lane + 1] = ex + a;
+            if (lane == 63) round_total = incl;
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t c = 0; c < kCompactChunks; ++c) {
+            const uint32_t cl = wave + kCompactWaves * c;
+            const uint32_t base = cbase[cl];
+            if (r0 + cl < nchunks && base < cap) {  // wave-uniform
+                const uint32_t incl = wave_inclusive_sum(cc[c], lane);
+                emit_word(wv[c], (r0 + cl) * 64 + lane, base + incl - cc[c], cap, G.wpr, kp);
             }
         }
-    } else {
-        for (uint32_t k0 = begin; k0 < end && base < P.cap; k0 += 64) {
-            const uint32_t k = k0 + lane;
-            const uint64_t v = k < end ? surv[k] : 0ull;
-            const uint32_t c1 = __popcll(v);
-            const uint32_t incl = wave_inclusive_sum(c1, lane);
-            emit_word(v, k, base + incl - c1, P.cap, G.wpr, kp);
-            base += __shfl(incl, 63);
-        }
+        carry += round_total;
+        __syncthreads();  // cbase and round_total are rewritten by the next round
     }
     if (tid == 0) {
         uint32_t* hdr = reinterpret_cast<uint32_t*>(J.feat + G.feat_off);
-        hdr[0] = total;
+        hdr[0] = carry;
         hdr[1] = 0;
     }
 }
@@ -1144,6 +1140,9 @@ __global__ __launch_bounds__(256) void k_match_index(MatchBatchView bv, MatchJob
     uint32_t* base = side ? J.index_curr : J.index_prev;
     uint32_t* starts = base + l * (kCells + 1);
     const uint32_t n = A.count(l, P.cap);
+    // The mutual pass writes only the matched entries of prev_match.
+    if (side == 0)
+        for (uint32_t i = tid; i < P.cap; i += 256) J.prev_match[l * P.cap + i] = -1;
     const float sc = P.lv[l].lod0_scale;
     for (uint32_t c = tid; c <= kCells; c += 256) hist[c] = 0;
     __syncthreads();
@@ -1218,8 +1217,13 @@ __device__ inline uint4 load_u4(const uint4* base, uint32_t i) {
 }  // namespace
 
 // Pipeline matcher, LANES lanes per query (256 / LANES queries per block).
-// kMutual = false: queries = curr, pool from prev -> b_to_a.
-// kMutual = true:  queries = prev, pool from curr, mutual check -> prev_match.
+// kMutual = false: slot = curr feature b; query b, pool from prev -> b_to_a[b].
+// kMutual = true:  slot = curr feature b with a = b_to_a[b] >= 0; query a,
+//   pool from curr; prev_match[a] = b when a's best match is b.  This is the
+//   reference's mutual check (match_features_cpu.cpp:196-205: a matches
+//   iff its best match b has b_to_a[b] == a) evaluated only for the prev
+//   features some b points at; every other entry keeps the -1 written by
+//   k_match_index.
 //
 // A query's candidates are the records of its pool levels (l, l-1, l+1, the
 // reference's concatenation order) in the cells under its search window:
@@ -1245,33 +1249,40 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
     }
     const MatchJob M = bv.n ? copy_const(const_ptr(bv.jobs + j)) : job0;
     ConstPlan P = as_const(M.plan);
-    const uint32_t l = b / P.query_groups;
-    const uint32_t slot = (b - l * P.query_groups) * QPB + grp;
+    const uint32_t groups = (P.cap + QPB - 1) / QPB;  // blocks per level
+    const uint32_t l = b / groups;
+    const uint32_t slot = (b - l * groups) * QPB + grp;
     if (slot >= P.cap) return;
     const FeatView Q{kMutual ? M.prev : M.curr, P.feat_stride};
     const FeatView C{kMutual ? M.curr : M.prev, P.feat_stride};
     const uint32_t* cidx = kMutual ? M.index_curr : M.index_prev;
-    int32_t* out = kMutual ? M.prev_match : M.b_to_a;
-    const uint32_t nq = Q.count(l, P.cap);
-    if (slot >= nq) {
-        if (sub == 0) out[l * P.cap + slot] = -1;
+    if (slot >= FeatView{M.curr, P.feat_stride}.count(l, P.cap)) {
+        if (!kMutual && sub == 0) M.b_to_a[l * P.cap + slot] = -1;
         return;
     }
+    // Query: feature (ql, qs) of the query side.
+    uint32_t ql = l, qs = slot;
+    if (kMutual) {
+        const int32_t ab = M.b_to_a[l * P.cap + slot];
+        if (ab < 0) return;
+        ql = static_cast<uint32_t>(ab) >> 24;
+        qs = static_cast<uint32_t>(ab) & 0xFFFFFFu;
+    }
 
-    // Pool levels in the reference's concatenation order: l, l-1, l+1.
-    const uint32_t p0 = l;
-    const uint32_t p1 = l > 0 ? l - 1 : l + 1;
-    const uint32_t p2 = l + 1;
-    uint32_t npool = 1, total_pool = C.count(l, P.cap);
-    if (l > 0) { total_pool += C.count(l - 1, P.cap); ++npool; }
-    if (l + 1 < P.L) { total_pool += C.count(l + 1, P.cap); ++npool; }
+    // Pool levels in the reference's concatenation order: ql, ql-1, ql+1.
+    const uint32_t p0 = ql;
+    const uint32_t p1 = ql > 0 ? ql - 1 : ql + 1;
+    const uint32_t p2 = ql + 1;
+    uint32_t npool = 1, total_pool = C.count(ql, P.cap);
+    if (ql > 0) { total_pool += C.count(ql - 1, P.cap); ++npool; }
+    if (ql + 1 < P.L) { total_pool += C.count(ql + 1, P.cap); ++npool; }
     const auto pool_level = [&](uint32_t s) { return s == 0 ? p0 : (s == 1 ? p1 : p2); };
 
     int32_t result = -1;
     if (total_pool >= 2) {
-        const uint32_t* qr = Q.rec(l, slot);
+        const uint32_t* qr = Q.rec(ql, qs);
         float qx, qy;
-        rec_lod0(qr, P.lv[l].lod0_scale, &qx, &qy);
+        rec_lod0(qr, P.lv[ql].lod0_scale, &qx, &qy);
         uint32_t qd[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) qd[k] = qr[4 + k];
@@ -1384,13 +1395,13 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
                 result = static_cast<int32_t>((pool_level(s) << 24) | idx);
             }
         }
-        if (kMutual && result >= 0 && sub == 0) {
-            const uint32_t bl = static_cast<uint32_t>(result) >> 24;
-            const uint32_t bi = static_cast<uint32_t>(result) & 0xFFFFFFu;
-            if (M.b_to_a[bl * P.cap + bi] != static_cast<int32_t>((l << 24) | slot)) result = -1;
-        }
     }
-    if (sub == 0) out[l * P.cap + slot] = result;
+    if (sub == 0) {
+        if (!kMutual)
+            M.b_to_a[l * P.cap + slot] = result;
+        else if (result == static_cast<int32_t>((l << 24) | slot))
+            M.prev_match[ql * P.cap + qs] = result;
+    }
 }
 
 template __global__ void k_match_dir<16, false>(MatchBatchView, MatchJob, float);
@@ -1583,6 +1594,33 @@ hipError_t launch_match_dir(uint32_t lanes, bool mutual, uint32_t blocks, MatchB
         case 64: launch_match_dir_t<64>(mutual, blocks, bv, job0, ratio, s); break;
         default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+// VALU issue-rate probe: 8 independent xor/add chains per lane, kProbeIters
+// iterations (16 VALU instructions each); the result only feeds an
+// impossible comparison so the chains stay live.
+constexpr int kProbeIters = 4096;
+constexpr uint32_t kProbeVaLU = 16 * kProbeIters;  // VALU instructions per wave
+
+__global__ __launch_bounds__(256) void k_valu_probe(uint32_t* sink, uint32_t seed) {
+    uint32_t a[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a[k] = seed + threadIdx.x * 8 + k;
+    for (int i = 0; i < kProbeIters; ++i) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) a[k] = (a[k] ^ 0x9e3779b9u) + 0x7f4a7c15u;
+    }
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s ^= a[k];
+    if (s == 0x12345678u) sink[blockIdx.x] = s;
+}
+
+hipError_t launch_valu_probe(uint32_t* sink, uint32_t blocks, hipStream_t s,
+                             double* instr_per_launch) {
+    hipLaunchKernelGGL(k_valu_probe, dim3(blocks), dim3(256), 0, s, sink, 1u);
+    *instr_per_launch = static_cast<double>(blocks) * 4.0 * kProbeVaLU;
     return hipGetLastError();
 }
 

@@ -89,6 +89,7 @@ _SIGNATURES = [
     ("orbfe_timing_read", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_uint32),
                                     C.POINTER(C.c_double)]),
     ("orbfe_timing_reset", C.c_int, [C.c_void_p]),
+    ("orbfe_probe_valu_rate", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]),
 ]
 
 ENTRY_POINTS = [s[0] for s in _SIGNATURES]
@@ -274,6 +275,12 @@ class Context:
         ms = C.c_double()
         self._check(lib().orbfe_timing_read(self._h, name.encode(), C.byref(n), C.byref(ms)))
         return int(n.value), float(ms.value)
+
+    def probe_valu_rate(self, stream=None) -> float:
+        """Sustained VALU issue rate of the device (wave64 instructions/s)."""
+        r = C.c_double()
+        self._check(lib().orbfe_probe_valu_rate(self._h, _stream(stream), C.byref(r)))
+        return float(r.value)
 
 
 class Batch:
