@@ -52,6 +52,39 @@ constexpr int8_t c_circle_y[16] = {-3, -3, -2, -1, 0, 1, 2, 3, 3, 3, 2, 1, 0, -1
 __device__ inline uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
 __device__ inline uint32_t umax(uint32_t a, uint32_t b) { return a > b ? a : b; }
 
+// acc + popcount(x) as one v_bcnt_u32_b32 (the compiler would otherwise
+// split popcount chains into bcnt(x, 0) and add3 trees).  ALU only.
+__device__ inline uint32_t bcnt_acc(uint32_t x, uint32_t acc) {
+    uint32_t r;
+    asm("v_bcnt_u32_b32 %0, %1, %2" : "=v"(r) : "v"(x), "v"(acc));
+    return r;
+}
+
+// Middle value of three (v_med3_u32).  ALU only.
+__device__ inline uint32_t med3_u32(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t r;
+    asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
+// Hamming distance of two 256-bit descriptors (a: 8 words; b: two halves).
+__device__ inline uint32_t hamming256(const uint32_t* a, const uint4& lo, const uint4& hi) {
+    uint32_t d = bcnt_acc(a[0] ^ lo.x, 0u);
+    d = bcnt_acc(a[1] ^ lo.y, d);
+    d = bcnt_acc(a[2] ^ lo.z, d);
+    d = bcnt_acc(a[3] ^ lo.w, d);
+    d = bcnt_acc(a[4] ^ hi.x, d);
+    d = bcnt_acc(a[5] ^ hi.y, d);
+    d = bcnt_acc(a[6] ^ hi.z, d);
+    return bcnt_acc(a[7] ^ hi.w, d);
+}
+
+// (min1, min2) <- the two smallest of (min1, min2, key), given min1 <= min2.
+__device__ inline void keep_two_smallest(uint32_t& min1, uint32_t& min2, uint32_t key) {
+    min2 = med3_u32(min1, min2, key);
+    min1 = umin(min1, key);
+}
+
 // Orders the LDS accesses of the lanes of one wave around this point.  The
 // LDS executes one wave's DS instructions in issue order, so only the
 // compiler must be kept from moving memory operations across; global
@@ -1367,14 +1400,9 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
                 if (!ok[u]) continue;
                 const float cx = xy[u].x, cy = xy[u].y;
                 if (cx < lo_x || cx > hi_x || __builtin_fabsf(qy - cy) > dy) continue;
-                const uint32_t d = __popc(qd[0] ^ dlo[u].x) + __popc(qd[1] ^ dlo[u].y) +
-                                   __popc(qd[2] ^ dlo[u].z) + __popc(qd[3] ^ dlo[u].w) +
-                                   __popc(qd[4] ^ dhi[u].x) + __popc(qd[5] ^ dhi[u].y) +
-                                   __popc(qd[6] ^ dhi[u].z) + __popc(qd[7] ^ dhi[u].w);
+                const uint32_t d = hamming256(qd, dlo[u], dhi[u]);
                 const uint32_t key = (d << kMatchKeyShift) | (sl[u] << kMatchIdxBits) | pos[u];
-                // (min1, min2) <- the two smallest of (min1, min2, key)
-                min2 = umax(min1, umin(min2, key));
-                min1 = umin(min1, key);
+                keep_two_smallest(min1, min2, key);
             }
         }
 #pragma unroll
@@ -1415,7 +1443,7 @@ template __global__ void k_match_dir<64, true>(MatchBatchView, MatchJob, float);
 // Brute-force matcher.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kBfThreads = 256;
-constexpr uint32_t kBfQPT = 2;          // queries per thread
+constexpr uint32_t kBfQPT = 4;          // queries per thread
 constexpr uint32_t kBfTile = 256;       // train descriptors per LDS tile
 
 __global__ __launch_bounds__(kBfThreads) void k_bf_partial(const uint32_t* __restrict__ query,
@@ -1453,17 +1481,8 @@ __global__ __launch_bounds__(kBfThreads) void k_bf_partial(const uint32_t* __res
             const uint32_t jt = base + t;
 #pragma unroll
             for (uint32_t k = 0; k < kBfQPT; ++k) {
-                uint32_t d = __popc(q[k][0] ^ a.x);
-                d += __popc(q[k][1] ^ a.y);
-                d += __popc(q[k][2] ^ a.z);
-                d += __popc(q[k][3] ^ a.w);
-                d += __popc(q[k][4] ^ c.x);
-                d += __popc(q[k][5] ^ c.y);
-                d += __popc(q[k][6] ^ c.z);
-                d += __popc(q[k][7] ^ c.w);
-                const uint32_t key = (d << kBfKeyShift) | jt;
-                min2[k] = umax(min1[k], umin(min2[k], key));
-                min1[k] = umin(min1[k], key);
+                const uint32_t key = (hamming256(q[k], a, c) << kBfKeyShift) | jt;
+                keep_two_smallest(min1[k], min2[k], key);
             }
         }
     }
