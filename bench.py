@@ -70,30 +70,35 @@ def measured_traffic(kernel):
         return None
 
 
-def measured_valu_issue(kernel):
-    """VALU issue rate of `kernel` as a fraction of the MI355X peak (one
-    wave64 VALU instruction per CU per clock) on this workload, plain and
-    batched, from the committed rocprofv3 SQ_INSTS_VALU summary; None when
-    no summary covers it."""
+def valu_roofline(kernel, avg_us, peak_rate):
+    """VALU roofline of `kernel` on this workload: its VALU instructions per
+    launch (rocprofv3 SQ_INSTS_VALU, committed summary; the count is fixed by
+    the workload) over the average launch duration measured in this run,
+    against the sustained VALU issue rate probed on this device.  None when
+    no summary covers the kernel."""
     path = os.path.join(ROOT, "profiles", "valu_1280x720.json")
     try:
         with open(path) as f:
             v = json.load(f)
-        return {"plain": v["plain"][kernel]["util"], "batched": v["batched"][kernel]["util"],
-                "source": "profiles/valu_1280x720.json"}
+        n = v["plain"][kernel]["valu_per_launch"]
+        batched = v["batched"][kernel]["util"]
     except (OSError, KeyError, ValueError):
         return None
+    ach = n / (avg_us * 1e-6)
+    return {"valu_per_launch": n, "achieved_wave_instr_per_s": float(f"{ach:.4g}"),
+            "peak_wave_instr_per_s": float(f"{peak_rate:.4g}"), "frac": round(ach / peak_rate, 4),
+            "frac_in_32_frame_batch": batched, "source": "profiles/valu_1280x720.json"}
 
 
 def cpu_baseline(frames, n_sample):
-    """The oracle (scalar C++, one thread) on the first n_sample frames of
-    the sequence: detect each and match consecutive pairs."""
+    """The oracle (scalar C++, one thread) on n_sample frames of the sequence
+    (cycling through it): detect each and match consecutive pairs."""
     from oracle import oracle as O
     p = O.params(W, H, threshold=thr20(), cap=CAP)
     t0 = time.perf_counter()
     prev = None
     for t in range(n_sample):
-        f, _ = O.detect_frame(frames[t], p)
+        f, _ = O.detect_frame(frames[t % len(frames)], p)
         cur = O.load_features(f, LEVELS, CAP)
         if prev is not None:
             O.find_matches(prev[0], prev[1], cur[0], cur[1], W, H, RATIO)
@@ -293,8 +298,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-sample", type=int, default=8,
-                    help="frames of the sequence the CPU baseline processes")
+    ap.add_argument("--cpu-sample", type=int, default=2 * N_FRAMES,
+                    help="frames the CPU baseline processes (cycling through the sequence)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the batched / mixed / matcher / serving phases")
@@ -390,12 +395,16 @@ def main():
             per_kernel[n] = {"launches": launches, "avg_us": round(avg_us, 3),
                              "bytes": kb[n], "GBps": round(kb[n] / (avg_us * 1e-6) / 1e9, 2)}
     log("per-kernel:", json.dumps(per_kernel))
+    peak = valu_peak(ctx)
+    log("valu peak:", json.dumps(peak))
     dom = max(per_kernel, key=lambda n: per_kernel[n]["avg_us"] * per_kernel[n]["launches"])
     ach = per_kernel[dom]["GBps"]
     roofline = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
                 "traffic": measured_traffic(dom),
-                "valu_issue_frac": measured_valu_issue(dom),
+                "valu": valu_roofline(dom, per_kernel[dom]["avg_us"], peak["wave_instr_per_s"]),
+                "limiter": "per-workgroup latency at LDS-limited occupancy (neither HBM- nor "
+                           "VALU-bound; profiles/valu_1280x720.md)",
                 "avg_us": per_kernel[dom]["avg_us"], "bytes_per_launch": kb[dom]}
 
     extra = {}
@@ -407,9 +416,8 @@ def main():
         log("batched sequence:", json.dumps(bseq))
         extra["mixed_batch"] = phase_mixed(torch, orbfe, synth, max(3, args.steps // 4))
         log("mixed batch:", json.dumps(extra["mixed_batch"]))
-        extra["valu_peak"] = valu_peak(ctx)
-        extra["matcher"] = phase_matcher(torch, orbfe, synth, 65536, 3,
-                                         extra["valu_peak"]["int_Tops"])
+        extra["valu_peak"] = peak
+        extra["matcher"] = phase_matcher(torch, orbfe, synth, 65536, 3, peak["int_Tops"])
         log("matcher:", json.dumps(extra["matcher"]))
         extra["serving"] = phase_serving(torch, orbfe, synth, args.serve_seconds)
         log("serving:", json.dumps(extra["serving"]))
@@ -418,8 +426,8 @@ def main():
     if not args.no_cpu:
         cfps, cdt = cpu_baseline(frames, args.cpu_sample)
         cpu = {"value": round(cfps, 4), "unit": "frames/s", "cores": 1, "kind": "port",
-               "sample": f"first {args.cpu_sample} frames of the sequence: detect each, "
-                         f"match {args.cpu_sample - 1} consecutive pairs ({cdt:.1f} s)"}
+               "sample": f"{args.cpu_sample} frames of the 1280x720 sequence (cycled): detect "
+                         f"each, match {args.cpu_sample - 1} consecutive pairs ({cdt:.1f} s)"}
         log(f"cpu baseline: {cfps:.3f} frames/s")
 
     out = {
