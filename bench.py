@@ -290,6 +290,10 @@ def phase_serving(torch, orbfe, synth, seconds, seed=4):
     arr2 = serve.poisson_arrivals(rates, 2.0, seed + 1)
     wall2, done2 = serve.run_schedule(srv2, arr2, pool, time_scale=0.0)
     out["saturated_frames_per_s"] = round(len(done2) / wall2, 1)
+    # the plain per-frame path over the same backlog (upload + per-frame launches)
+    wall_p = serve.plain_backlog(specs, arr2, pool, fast_threshold=thr20(), lowe_ratio=RATIO)
+    out["plain_frames_per_s"] = round(len(arr2) / wall_p, 1)
+    out["saturated_vs_plain"] = round(wall_p / wall2, 2)
     return out
 
 

@@ -321,7 +321,11 @@ class MatchBatch:
             pass
 
 
-def _frames(frames: Sequence[dict]):
+def _frames(frames):
+    """ctypes array of Frame from dicts (image, width, height, pitch, cap,
+    pyramid, features); a ctypes Frame array is passed through as is."""
+    if isinstance(frames, C.Array):
+        return frames
     arr = (Frame * max(len(frames), 1))()
     for i, f in enumerate(frames):
         arr[i].image = _ptr(f["image"])
@@ -334,7 +338,11 @@ def _frames(frames: Sequence[dict]):
     return arr
 
 
-def _pairs(pairs: Sequence[dict]):
+def _pairs(pairs):
+    """ctypes array of MatchPair from dicts (prev, curr, prev_match, width,
+    height, cap); a ctypes MatchPair array is passed through as is."""
+    if isinstance(pairs, C.Array):
+        return pairs
     arr = (MatchPair * max(len(pairs), 1))()
     for i, p in enumerate(pairs):
         arr[i].prev_features = _ptr(p["prev"])

@@ -3,12 +3,13 @@
 Frames arrive from many streams at their own times.  The server keeps, per
 stream, the feature arrays of the previous and current frame, a pyramid
 buffer and two upload slots resident in HBM.  Each scheduling tick takes the
-frames that have arrived, admits as many as the HBM budget and the in-flight
-slots allow, uploads them and launches one batched detect (all admitted
-frames, any mix of sizes) and one batched match (each frame against its
-stream's previous frame) on one of two HIP streams, so a new batch overlaps
-the one already on the device.  A stream's next frame waits (on the device)
-for the batch holding its previous frame.  Latency = completion - arrival.
+frames that have arrived, admits the oldest ones (one per stream) as far as
+the HBM budget and the in-flight slots allow, uploads them and launches one
+batched detect (all admitted frames, any mix of sizes) and one batched match
+(each frame against its stream's previous frame) on one of two HIP streams.
+A batch's uploads overlap the kernels of the batch before it; its kernels
+wait (on the device) for the batches holding its streams' previous frames.
+Latency = completion - arrival.
 
 Results are bit-identical to the plain per-frame path: the batched kernels
 compute each frame exactly as the per-frame launch does.
@@ -17,6 +18,7 @@ from __future__ import annotations
 
 import collections
 import gc
+import heapq
 import time
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence, Tuple
@@ -57,23 +59,29 @@ class StreamSpec:
 
 
 @dataclass
+class _Request:
+    stream: int
+    seq: int            # frame number within its stream
+    t_arrival: float
+    host: torch.Tensor  # pinned u8 (H, W)
+    order: int = 0      # submission number (FIFO admission key)
+
+
+@dataclass
 class _Stream:
     spec: StreamSpec
     feats: List[torch.Tensor]
     pyramid: torch.Tensor
     match: torch.Tensor
     upload: List[torch.Tensor]
-    frames_done: int = 0
-    last_event: Optional[torch.cuda.Event] = None
-    scratch: int = 0
-
-
-@dataclass
-class _Request:
-    stream: int
-    seq: int            # frame number within its stream
-    t_arrival: float
-    host: torch.Tensor  # pinned u8 (H, W)
+    scratch: int
+    # prebuilt C descriptors of this stream's frame / pair for frame parity 0, 1
+    frame_c: List[orbfe.Frame] = field(default_factory=list)
+    pair_c: List[orbfe.MatchPair] = field(default_factory=list)
+    pending: collections.deque = field(default_factory=collections.deque)
+    last_event: Optional[torch.cuda.Event] = None       # batch of the latest admitted frame
+    upload_event: List[Optional[torch.cuda.Event]] = field(
+        default_factory=lambda: [None, None])           # batch that last read upload[p]
 
 
 @dataclass
@@ -82,6 +90,28 @@ class Completed:
     seq: int
     t_arrival: float
     t_done: float
+
+
+def _frame_struct(st: _Stream, p: int) -> orbfe.Frame:
+    sp = st.spec
+    f = orbfe.Frame()
+    f.image = st.upload[p].data_ptr()
+    f.pitch = sp.width
+    f.width, f.height, f.cap = sp.width, sp.height, sp.cap
+    f.pyramid = st.pyramid.data_ptr()
+    f.features = st.feats[p].data_ptr()
+    return f
+
+
+def _pair_struct(st: _Stream, p: int) -> orbfe.MatchPair:
+    """Pair of the frame with parity p against the stream's previous frame."""
+    sp = st.spec
+    m = orbfe.MatchPair()
+    m.prev_features = st.feats[1 - p].data_ptr()
+    m.curr_features = st.feats[p].data_ptr()
+    m.prev_match = st.match.data_ptr()
+    m.width, m.height, m.cap = sp.width, sp.height, sp.cap
+    return m
 
 
 class Server:
@@ -106,15 +136,22 @@ class Server:
         for sp in self.specs:
             fb = orbfe.feature_array_bytes(LEVELS, sp.cap)
             npyr = sum(w * h for w, h in orbfe.level_dims(sp.width, sp.height)[1:])
-            self.streams.append(_Stream(
+            st = _Stream(
                 spec=sp,
                 feats=[torch.zeros(fb, dtype=torch.uint8, device=self.dev) for _ in range(2)],
                 pyramid=torch.zeros(npyr, dtype=torch.float32, device=self.dev),
                 match=torch.full((LEVELS * sp.cap,), -1, dtype=torch.int32, device=self.dev),
                 upload=[torch.empty((sp.height, sp.width), dtype=torch.uint8, device=self.dev)
                         for _ in range(2)],
-                scratch=frame_scratch_bytes(sp.width, sp.height, sp.cap)))
-        self.pending: collections.deque = collections.deque()
+                scratch=frame_scratch_bytes(sp.width, sp.height, sp.cap))
+            st.frame_c = [_frame_struct(st, p) for p in range(2)]
+            st.pair_c = [_pair_struct(st, p) for p in range(2)]
+            self.streams.append(st)
+        # streams with pending frames, keyed by their oldest frame's submission
+        # number: popping in key order is FIFO admission over all frames with
+        # at most one frame per stream per batch
+        self._ready: List[Tuple[int, int]] = []
+        self._order = 0
         self.completed: List[Completed] = []
         self.in_flight_bytes = 0
         self.max_batch_seen = 0
@@ -127,15 +164,10 @@ class Server:
         the contexts' scratch arenas and pinned staging slots to the largest
         batch admission can form (one frame of every stream), so serving
         never allocates device or pinned memory.  Stream state is reset."""
+        n = len(self.streams)
+        frames = (orbfe.Frame * n)(*[st.frame_c[0] for st in self.streams])
+        pairs = (orbfe.MatchPair * n)(*[st.pair_c[0] for st in self.streams])
         for ctx, hs in zip(self.ctxs, self.hip_streams):
-            frames, pairs = [], []
-            for st in self.streams:
-                sp = st.spec
-                frames.append({"image": st.upload[0], "width": sp.width, "height": sp.height,
-                               "pitch": sp.width, "cap": sp.cap, "pyramid": st.pyramid,
-                               "features": st.feats[0]})
-                pairs.append({"prev": st.feats[0], "curr": st.feats[1], "prev_match": st.match,
-                              "width": sp.width, "height": sp.height, "cap": sp.cap})
             with torch.cuda.stream(hs):
                 for _ in range(4):  # each slot of the staging ring
                     ctx.detect_batch_async(frames, stream=hs)
@@ -149,10 +181,15 @@ class Server:
 
     # -- requests --------------------------------------------------------------
     def submit(self, stream: int, seq: int, host_frame: torch.Tensor, t_arrival: float):
-        self.pending.append(_Request(stream, seq, t_arrival, host_frame))
+        st = self.streams[stream]
+        r = _Request(stream, seq, t_arrival, host_frame, self._order)
+        self._order += 1
+        st.pending.append(r)
+        if len(st.pending) == 1:
+            heapq.heappush(self._ready, (r.order, stream))
 
     def idle(self) -> bool:
-        return not self.pending and all(b is None for b in self.busy)
+        return not self._ready and all(b is None for b in self.busy)
 
     # -- scheduling -------------------------------------------------------------
     def poll(self, now_fn=time.perf_counter):
@@ -172,68 +209,81 @@ class Server:
                 self.in_flight_bytes -= nbytes
                 self.busy[i] = None
 
+    def _admit(self) -> Tuple[List[_Request], int]:
+        """FIFO admission: the oldest pending frames, one per stream (a
+        stream's frames stay in order), up to max_batch, within the HBM
+        budget (a frame is always admitted when nothing else is in flight)."""
+        chosen: List[_Request] = []
+        nbytes = 0
+        ready = self._ready
+        while ready and len(chosen) < self.max_batch:
+            s = ready[0][1]
+            st = self.streams[s]
+            if (self.in_flight_bytes + nbytes + st.scratch > self.budget
+                    and (chosen or self.in_flight_bytes > 0)):
+                break
+            heapq.heappop(ready)
+            chosen.append(st.pending.popleft())
+            nbytes += st.scratch
+        for r in chosen:  # back in line with the next frame, for later batches
+            st = self.streams[r.stream]
+            if st.pending:
+                heapq.heappush(ready, (st.pending[0].order, r.stream))
+        return chosen, nbytes
+
     def launch(self) -> bool:
-        """Admit pending frames into a free slot; returns True if launched."""
-        if not self.pending:
+        """Admit pending frames into a free slot; returns True if launched.
+
+        Device order on the slot's HIP stream: each frame's upload (after
+        the batch that last read that upload buffer), then a wait for the
+        batches holding the admitted streams' previous frames (their
+        features, pyramid and match table are rewritten here), then one
+        batched detect and one batched match.  The uploads of a batch thus
+        overlap the kernels of the batch before it."""
+        if not self._ready:
             return False
         try:
             slot = self.busy.index(None)
         except ValueError:
             return False
-        # FIFO admission: one frame per stream per batch (a stream's frames
-        # stay in order), up to max_batch, within the HBM budget (a frame is
-        # always admitted when nothing else is in flight).
-        chosen: List[_Request] = []
-        seen = set()
-        nbytes = 0
-        full = False
-        keep: collections.deque = collections.deque()
-        for r in self.pending:
-            st = self.streams[r.stream]
-            if full or len(chosen) >= self.max_batch or r.stream in seen:
-                keep.append(r)
-                continue
-            if (self.in_flight_bytes + nbytes + st.scratch > self.budget
-                    and (chosen or self.in_flight_bytes > 0)):
-                full = True
-                keep.append(r)
-                continue
-            seen.add(r.stream)
-            chosen.append(r)
-            nbytes += st.scratch
-        self.pending = keep
+        chosen, nbytes = self._admit()
         if not chosen:
             return False
         ctx, hs = self.ctxs[slot], self.hip_streams[slot]
         frames, pairs = [], []
+        waited = set()
         with torch.cuda.stream(hs):
             for r in chosen:
                 st = self.streams[r.stream]
-                if st.last_event is not None:
-                    hs.wait_event(st.last_event)  # previous frame of this stream
-                up = st.upload[r.seq % 2]
-                up.copy_(r.host, non_blocking=True)
-                sp = st.spec
-                frames.append({"image": up, "width": sp.width, "height": sp.height,
-                               "pitch": sp.width, "cap": sp.cap, "pyramid": st.pyramid,
-                               "features": st.feats[r.seq % 2]})
+                p = r.seq & 1
+                e = st.upload_event[p]
+                if e is not None and id(e) not in waited:
+                    hs.wait_event(e)
+                    waited.add(id(e))
+                st.upload[p].copy_(r.host, non_blocking=True)
+                frames.append(st.frame_c[p])
                 if r.seq > 0:
-                    pairs.append({"prev": st.feats[(r.seq - 1) % 2],
-                                  "curr": st.feats[r.seq % 2], "prev_match": st.match,
-                                  "width": sp.width, "height": sp.height, "cap": sp.cap})
-            ctx.detect_batch_async(frames, stream=hs)
+                    pairs.append(st.pair_c[p])
+            for r in chosen:
+                e = self.streams[r.stream].last_event
+                if e is not None and id(e) not in waited:
+                    hs.wait_event(e)
+                    waited.add(id(e))
+            ctx.detect_batch_async((orbfe.Frame * len(frames))(*frames), stream=hs)
             if pairs:
-                ctx.match_batch_async(pairs, stream=hs)
+                ctx.match_batch_async((orbfe.MatchPair * len(pairs))(*pairs), stream=hs)
             if self.keep_outputs:  # stream-ordered snapshots of the results
                 for r in chosen:
                     st = self.streams[r.stream]
                     self._snap[(r.stream, r.seq)] = (
-                        st.feats[r.seq % 2].clone(),
+                        st.feats[r.seq & 1].clone(),
                         st.match.clone() if r.seq > 0 else None)
             ev = torch.cuda.Event()
             ev.record(hs)
         for r in chosen:
-            self.streams[r.stream].last_event = ev
+            st = self.streams[r.stream]
+            st.last_event = ev
+            st.upload_event[r.seq & 1] = ev
         self.busy[slot] = (ev, chosen, nbytes)
         self.in_flight_bytes += nbytes
         self.max_batch_seen = max(self.max_batch_seen, len(chosen))
@@ -292,6 +342,49 @@ def _run(server, arrivals, frame_pool, time_scale, spin_s):
             time.sleep(spin_s)
     wall = time.perf_counter() - t0
     return wall, server.completed
+
+
+def plain_backlog(specs: Sequence[StreamSpec], arrivals, frame_pool, device: int = 0,
+                  fast_threshold: float = 0.15, lowe_ratio: float = 0.85) -> float:
+    """The plain per-frame path over the same frames: every frame of the
+    schedule, in arrival order, uploaded from pinned memory and run through
+    its stream's per-frame detect and match launches on one HIP stream, with
+    no batching.  Returns the wall seconds to drain the whole backlog."""
+    dev = torch.device(f"cuda:{device}")
+    ctxs: Dict[Tuple[int, int, int], orbfe.Context] = {}
+    state = []
+    for sp in specs:
+        key = (sp.width, sp.height, sp.cap)
+        if key not in ctxs:
+            ctxs[key] = orbfe.Context(sp.width, sp.height, device=device,
+                                      features_per_level=sp.cap,
+                                      fast_threshold=fast_threshold, lowe_ratio=lowe_ratio)
+        fb = orbfe.feature_array_bytes(LEVELS, sp.cap)
+        state.append((ctxs[key], torch.empty((sp.height, sp.width), dtype=torch.uint8,
+                                             device=dev),
+                      [torch.zeros(fb, dtype=torch.uint8, device=dev) for _ in range(2)],
+                      torch.full((LEVELS * sp.cap,), -1, dtype=torch.int32, device=dev)))
+    hs = torch.cuda.current_stream(dev)
+
+    def one(t, s, k):
+        c, up, feats, match = state[s]
+        pool = frame_pool[s]
+        up.copy_(pool[k % len(pool)], non_blocking=True)
+        c.detect_async(up, specs[s].width, feats[k & 1], stream=hs)
+        if k > 0:
+            c.match_pipeline_async(feats[1 - (k & 1)], feats[k & 1], match, stream=hs)
+
+    for a in arrivals[:2 * len(specs)]:  # warm every context
+        one(*a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for a in arrivals:
+        one(*a)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    for c in ctxs.values():
+        c.close()
+    return wall
 
 
 def latency_stats(completed: List[Completed]):
