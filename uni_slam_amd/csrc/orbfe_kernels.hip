@@ -190,14 +190,15 @@ __device__ inline float level_texel_direct(const Job& J, ConstPlan P, const Leve
 // ---------------------------------------------------------------------------
 // k_detect
 // ---------------------------------------------------------------------------
-// One LDS buffer holds first the u8 source region of a level >= 1 tile
-// ((TH+6)*s + 3 rows of 70*s + 3 bytes at level scale s: 6.4 KB, 10.4 KB,
-// 17.1 KB for TH = 8, 16, 32 at s = 1.2^5) and then, once the level tile is
-// staged, the horizontal lerps of the smoothing (3 x (TH+6) x 64 floats:
-// 10.5 KB, 16.5 KB, 28.5 KB).  Source regions that do not fit (levels past
-// the fifth) fall back to per-texel gathers.
+// One LDS buffer holds first the u8 source region of a level >= 1 tile and
+// then, once the level tile is staged, the corner scores and candidate lists
+// of the FAST pass.  At level scales up to 2.5 (six levels of 1.2) the region
+// spans at most (TH+6)*2.5 + 4 rows of 45 dwords (70 texels * 2.5 plus the
+// bilinear neighbour and the dword alignment): 7.0 KB, 10.6 KB, 17.8 KB for
+// TH = 8, 16, 32.  Regions that do not fit (levels past the sixth) fall back
+// to per-texel gathers.
 template <int TH>
-constexpr uint32_t stage_words() { return 3u * (TH + 6) * 64; }
+constexpr uint32_t stage_words() { return ((TH + 6) * 5u / 2u + 4u) * 45u; }
 
 // Byte i of the staged source region, read as a dword and shifted, so the
 // load stays a 32-bit LDS read.
@@ -396,18 +397,18 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     static_assert(SI::kMask <= kDetectThreads, "one set-up item per thread");
     __shared__ float lut[256];
     __shared__ float tile[LH][kLdsW];
-    // u8 source region while staging, then hbuf[3][LH][64], the horizontal
-    // lerps of the 9-tap Gaussian, then the corner scores and candidate
+    // u8 source region while staging, then the corner scores and candidate
     // lists of the FAST pass (per wave RPT x 64 bytes and RPT x 64 u16).
     __shared__ uint32_t stage[kSrcWords];
     static_assert(kDetectThreads / 64 * RPT * 64 * 3 <= kSrcWords * 4, "FAST scratch fits");
     uint32_t* const src = stage;
-    __shared__ uint32_t col_i0[kLdsW], col_i1[kLdsW];
+    // taps as i0 | i1 << 16 and fraction
+    __shared__ uint32_t col_i01[kLdsW];
     __shared__ float col_a[kLdsW];
-    __shared__ uint32_t row_i0[LH], row_i1[LH];
+    __shared__ uint32_t row_i01[LH];
     __shared__ float row_b[LH];
     __shared__ uint32_t col_ok[kLdsW], row_ok[LH];
-    __shared__ uint32_t srow_i0[3][TH], srow_i1[3][TH];
+    __shared__ uint32_t srow_i01[3][TH];
     __shared__ float srow_b[3][TH];
     __shared__ uint32_t row0_score[kTileW + 4];
     __shared__ float row0_ring[4][17];      // row-0 neighbours: 16 ring texels + centre
@@ -448,19 +449,17 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
             const int32_t g = (is_col ? x0 : y0) - kHalo + static_cast<int32_t>(c);
             const uint32_t n = is_col ? w : h;
             const bool inside = g >= 0 && g < static_cast<int32_t>(n);
-            const uint32_t i0 = item.x & 0xFFFFu, i1 = item.x >> 16;
             const float frac = __uint_as_float(item.y);
             const uint32_t ok = (!prm.border_zero || inside) ? 1u : 0u;
             if (is_col) {
-                col_i0[c] = i0; col_i1[c] = i1; col_a[c] = frac; col_ok[c] = ok;
+                col_i01[c] = item.x; col_a[c] = frac; col_ok[c] = ok;
             } else {
-                row_i0[c] = i0; row_i1[c] = i1; row_b[c] = frac; row_ok[c] = ok;
+                row_i01[c] = item.x; row_b[c] = frac; row_ok[c] = ok;
             }
         } else if (e < SI::kSrow) {
             const uint32_t q = e - SI::kTap;
             const uint32_t jj = q / TH, rr = q - jj * TH;
-            srow_i0[jj][rr] = item.x & 0xFFFFu;
-            srow_i1[jj][rr] = item.x >> 16;
+            srow_i01[jj][rr] = item.x;
             srow_b[jj][rr] = __uint_as_float(item.y);
         } else if (e < SI::kMask) {
             mask_lds[e - SI::kSrow] = static_cast<uint64_t>(item.x) |
@@ -546,7 +545,8 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     // row taps wave-uniform; columns 64..69 follow, one texel per thread.
     {
         const uint32_t c = lane;
-        const uint32_t ci0 = col_i0[c], ci1 = col_i1[c];
+        const uint32_t ci01 = col_i01[c];
+        const uint32_t ci0 = ci01 & 0xFFFFu, ci1 = ci01 >> 16;
         const float ca = col_a[c];
         const bool cok = col_ok[c] != 0;
         const bool cin = use_src && c >= c_first && c <= c_last;
@@ -554,8 +554,8 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
         for (uint32_t k = 0; k < (LH + kWaves - 1) / kWaves; ++k) {
             const uint32_t r = wave + k * kWaves;
             if (r < LH) {
-                const uint32_t ri0 = __builtin_amdgcn_readfirstlane(row_i0[r]);
-                const uint32_t ri1 = __builtin_amdgcn_readfirstlane(row_i1[r]);
+                const uint32_t ri01 = __builtin_amdgcn_readfirstlane(row_i01[r]);
+                const uint32_t ri0 = ri01 & 0xFFFFu, ri1 = ri01 >> 16;
                 const float rb = __uint_as_float(
                     __builtin_amdgcn_readfirstlane(__float_as_uint(row_b[r])));
                 const bool rok = __builtin_amdgcn_readfirstlane(row_ok[r]) != 0;
@@ -570,9 +570,10 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
             const uint32_t ce = kTileW + (tid - r * (kLdsW - kTileW));
             const bool in = use_src && ce >= c_first && ce <= c_last && r >= r_first &&
                             r <= r_last;
+            const uint32_t ri01 = row_i01[r], ci01 = col_i01[ce];
             tile[r][ce] = (col_ok[ce] & row_ok[r])
-                              ? texel(row_i0[r], row_i1[r], row_b[r], in, col_i0[ce], col_i1[ce],
-                                      col_a[ce], in)
+                              ? texel(ri01 & 0xFFFFu, ri01 >> 16, row_b[r], in, ci01 & 0xFFFFu,
+                                      ci01 >> 16, col_a[ce], in)
                               : 0.0f;
         }
     }
@@ -581,58 +582,128 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     const uint32_t x = static_cast<uint32_t>(x0) + lane;
 
     // Smoothed image over the extended grid (generate_features.wgsl:149-172
-    // at every integer point), separably: the horizontal lerp of each
-    // (column tap, staged row) once, then 3x3 vertical lerps and weights per
-    // point, in the shader's order.  Sampler taps are clamped into the
-    // level, so on all but the far-corner tiles they lie in the staged tile.
+    // at every integer point): per point, 3 x 3 bilinear samples of the
+    // level at offsets -1.2, 0, 1.2 texels, weighted, in the shader's order.
+    // A sample is the vertical lerp of two horizontal lerps of staged texels.
+    //
+    // Staged column (row) g - x0 + kHalo holds the level texel at
+    // tint_clamp(g) (0 outside the level when border_zero), so a tap whose
+    // table entry (i0, i1) equals (tint_clamp(f), tint_clamp(f + 1)) for a
+    // sample floor f (border_zero: (f, f + 1), inside the level) reads
+    // exactly the staged texels f, f + 1.  The floors of the three taps are
+    // x - 2, x or x - 1, and x + 1 wherever the table confirms it (all but
+    // the first two columns and rows of a level at the sizes in use): the
+    // horizontal lerps of a wave's rows y - 2 .. y + RPT + 1 then sit in a
+    // register window shared by its RPT outputs.  Other lanes and waves
+    // evaluate each sample from the staged texels the table names.
     const bool slow = x0 > static_cast<int32_t>(w) + 2 || y0 > static_cast<int32_t>(h) + 2;
     if (!slow) {
         if (x < sw) {
             TapEntry tp[3];
 #pragma unroll
             for (int i = 0; i < 3; ++i) tp[i] = load_tap(taps, t_sc + i * sw + x);
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                const uint32_t c0 = (tp[i].i01 & 0xFFFFu) - x0 + kHalo;
-                const uint32_t c1 = (tp[i].i01 >> 16) - x0 + kHalo;
-                const float fr = tp[i].frac;
-#pragma unroll
-                for (uint32_t k = 0; k < (LH + kWaves - 1) / kWaves; ++k) {
-                    const uint32_t r = wave + k * kWaves;
-                    if (r < LH) {
-                        const float t0 = tile[r][c0];
-                        stage[(i * LH + r) * kTileW + lane] =
-                            __float_as_uint(t0 + fr * (tile[r][c1] - t0));
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        if (x < sw) {
+            const bool bz = prm.border_zero != 0;
+            // the table entry under which a tap with sample floor f on an
+            // axis of n texels reads the staged texels f, f + 1
+            const auto expect = [bz](int32_t f, uint32_t n) -> uint32_t {
+                if (bz)
+                    return f >= 0 && f + 1 < static_cast<int32_t>(n)
+                               ? static_cast<uint32_t>(f) | (static_cast<uint32_t>(f + 1) << 16)
+                               : 0xFFFFFFFFu;  // never a table entry
+                return tint_clamp(f, n) | (tint_clamp(f + 1, n) << 16);
+            };
+            const int32_t xi = static_cast<int32_t>(x);
+            const bool col_a1 = tp[1].i01 == expect(xi, w);
+            const bool col_fast = tp[0].i01 == expect(xi - 2, w) &&
+                                  tp[2].i01 == expect(xi + 1, w) &&
+                                  (col_a1 || tp[1].i01 == expect(xi - 1, w));
+            // wave-uniform: rows of this wave's outputs
+            bool row_fast = true;
+            uint32_t row_b1 = 0;  // bit k: middle tap of output k has floor y - 1
 #pragma unroll
             for (int k = 0; k < RPT; ++k) {
                 const uint32_t rr = wave * RPT + k;
-                const uint32_t y = static_cast<uint32_t>(y0) + rr;
-                if (y >= sh) continue;
-                float sum = 0.0f;
+                const int32_t yi = y0 + static_cast<int32_t>(rr);
+                if (static_cast<uint32_t>(yi) >= sh) continue;
+                const uint32_t e0 = __builtin_amdgcn_readfirstlane(srow_i01[0][rr]);
+                const uint32_t e1 = __builtin_amdgcn_readfirstlane(srow_i01[1][rr]);
+                const uint32_t e2 = __builtin_amdgcn_readfirstlane(srow_i01[2][rr]);
+                const bool a1 = e1 == expect(yi, h);
+                row_fast = row_fast && e0 == expect(yi - 2, h) && e2 == expect(yi + 1, h) &&
+                           (a1 || e1 == expect(yi - 1, h));
+                if (!a1) row_b1 |= 1u << k;
+            }
+            const float fr0 = tp[0].frac, fr1 = tp[1].frac, fr2 = tp[2].frac;
+            if (row_fast && col_fast) {
+                float hv[3][RPT + 4];
 #pragma unroll
-                for (int jj = 0; jj < 3; ++jj) {
-                    // wave-uniform row taps (rr is)
-                    const uint32_t ra =
-                        __builtin_amdgcn_readfirstlane(srow_i0[jj][rr]) - y0 + kHalo;
-                    const uint32_t rb =
-                        __builtin_amdgcn_readfirstlane(srow_i1[jj][rr]) - y0 + kHalo;
-                    const float bb = __uint_as_float(
-                        __builtin_amdgcn_readfirstlane(__float_as_uint(srow_b[jj][rr])));
-#pragma unroll
-                    for (int i = 0; i < 3; ++i) {
-                        const float top = __uint_as_float(stage[(i * LH + ra) * kTileW + lane]);
-                        const float sv =
-                            top + bb * (__uint_as_float(stage[(i * LH + rb) * kTileW + lane]) - top);
-                        sum = sum + sv * wts[i] * wts[jj];
-                    }
+                for (int m = 0; m < RPT + 4; ++m) {
+                    const float* tr = &tile[wave * RPT + 1 + m][lane + 1];  // row y - 2 + m
+                    const float v0 = tr[0], v1 = tr[1], v2 = tr[2], v3 = tr[3], v4 = tr[4];
+                    hv[0][m] = v0 + fr0 * (v1 - v0);
+                    const float a = col_a1 ? v2 : v1, b = col_a1 ? v3 : v2;
+                    hv[1][m] = a + fr1 * (b - a);
+                    hv[2][m] = v3 + fr2 * (v4 - v3);
                 }
-                J.smooth[G.sm_off + y * sw + x] = sum;
+#pragma unroll
+                for (int k = 0; k < RPT; ++k) {
+                    const uint32_t rr = wave * RPT + k;
+                    const uint32_t y = static_cast<uint32_t>(y0) + rr;
+                    if (y >= sh) continue;
+                    const bool b1 = (row_b1 >> k) & 1u;
+                    float sum = 0.0f;
+#pragma unroll
+                    for (int jj = 0; jj < 3; ++jj) {
+                        const float bb = __uint_as_float(
+                            __builtin_amdgcn_readfirstlane(__float_as_uint(srow_b[jj][rr])));
+#pragma unroll
+                        for (int i = 0; i < 3; ++i) {
+                            float top, bot;
+                            if (jj == 0) {
+                                top = hv[i][k];
+                                bot = hv[i][k + 1];
+                            } else if (jj == 1) {
+                                top = b1 ? hv[i][k + 1] : hv[i][k + 2];
+                                bot = b1 ? hv[i][k + 2] : hv[i][k + 3];
+                            } else {
+                                top = hv[i][k + 3];
+                                bot = hv[i][k + 4];
+                            }
+                            const float sv = top + bb * (bot - top);
+                            sum = sum + sv * wts[i] * wts[jj];
+                        }
+                    }
+                    J.smooth[G.sm_off + y * sw + x] = sum;
+                }
+            } else {
+                const float frs[3] = {fr0, fr1, fr2};
+#pragma unroll
+                for (int k = 0; k < RPT; ++k) {
+                    const uint32_t rr = wave * RPT + k;
+                    const uint32_t y = static_cast<uint32_t>(y0) + rr;
+                    if (y >= sh) continue;
+                    float sum = 0.0f;
+#pragma unroll
+                    for (int jj = 0; jj < 3; ++jj) {
+                        const uint32_t e = __builtin_amdgcn_readfirstlane(srow_i01[jj][rr]);
+                        const uint32_t ra = (e & 0xFFFFu) - y0 + kHalo;
+                        const uint32_t rb = (e >> 16) - y0 + kHalo;
+                        const float bb = __uint_as_float(
+                            __builtin_amdgcn_readfirstlane(__float_as_uint(srow_b[jj][rr])));
+#pragma unroll
+                        for (int i = 0; i < 3; ++i) {
+                            const uint32_t c0 = (tp[i].i01 & 0xFFFFu) - x0 + kHalo;
+                            const uint32_t c1 = (tp[i].i01 >> 16) - x0 + kHalo;
+                            const float t0 = tile[ra][c0];
+                            const float top = t0 + frs[i] * (tile[ra][c1] - t0);
+                            const float t1 = tile[rb][c0];
+                            const float bot = t1 + frs[i] * (tile[rb][c1] - t1);
+                            const float sv = top + bb * (bot - top);
+                            sum = sum + sv * wts[i] * wts[jj];
+                        }
+                    }
+                    J.smooth[G.sm_off + y * sw + x] = sum;
+                }
             }
         }
     } else if (x < sw) {
@@ -648,7 +719,8 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     // and 8, so for min_run >= 9 a pixel failing either of those two tests
     // scores 0; the rest are gathered into a per-wave list and get the full
     // ring.  Scores land in score_lds (0 elsewhere).
-    __syncthreads();  // hbuf no longer read: its words become the FAST scratch
+    // The source region is dead since the tile staging: its words become
+    // the per-wave FAST scratch.
     uint8_t* wscore = reinterpret_cast<uint8_t*>(stage) + wave * RPT * 64;
     uint16_t* wlist = reinterpret_cast<uint16_t*>(reinterpret_cast<uint8_t*>(stage) +
                                                   kDetectThreads * RPT) + wave * RPT * 64;
@@ -767,13 +839,63 @@ constexpr uint32_t kCompactWaves = kCompactThreads / 64;
 constexpr uint32_t kCompactChunks = 8;                             // per wave and round
 constexpr uint32_t kRoundChunks = kCompactWaves * kCompactChunks;  // 128 (8192 words)
 
-__device__ inline uint32_t wave_inclusive_sum(uint32_t v, uint32_t lane) {
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t o = __shfl_up(v, off);
-        if (lane >= static_cast<uint32_t>(off)) v += o;
-    }
+// v plus the value of the lane the DPP control names (0 where it names none
+// or its row is masked off): one VALU with a DPP source, no LDS round trip.
+template <int kCtrl, int kRowMask = 0xf>
+__device__ inline uint32_t dpp_add(uint32_t v) {
+    return v + static_cast<uint32_t>(
+                   __builtin_amdgcn_update_dpp(0, static_cast<int>(v), kCtrl, kRowMask, 0xf, false));
+}
+
+// Inclusive sum over the aligned group of W (16, 32 or 64) lanes: shifts
+// 1, 2, 4, 8 within each 16-lane row, then lane 15 of a row into the next
+// row (W >= 32) and lane 31 into the upper half (W = 64).  Every lane of a
+// group must be active.
+template <uint32_t W>
+__device__ inline uint32_t group_scan_dpp(uint32_t v) {
+    static_assert(W == 16 || W == 32 || W == 64, "group of 16, 32 or 64 lanes");
+    v = dpp_add<0x111>(v);  // row_shr:1
+    v = dpp_add<0x112>(v);  // row_shr:2
+    v = dpp_add<0x114>(v);  // row_shr:4
+    v = dpp_add<0x118>(v);  // row_shr:8
+    if (W >= 32) v = dpp_add<0x142, 0xa>(v);  // row_bcast:15 into rows 1, 3
+    if (W == 64) v = dpp_add<0x143, 0xc>(v);  // row_bcast:31 into rows 2, 3
     return v;
+}
+
+__device__ inline uint32_t wave_inclusive_sum(uint32_t v) { return group_scan_dpp<64>(v); }
+
+// The value of the lane the DPP control names (every lane of the row valid).
+template <int kCtrl>
+__device__ inline uint32_t dpp_mov(uint32_t v) {
+    return static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(v), kCtrl, 0xf, 0xf, false));
+}
+
+// Butterfly partners within a 16-lane row: lane ^ 1, lane ^ 2 (quad_perm),
+// the mirrored quad of the 8-lane half (row_half_mirror) and the mirrored
+// half of the row (row_mirror); after the four steps every lane has met
+// every other lane of its row.
+constexpr int kDppQuadXor1 = 0xB1;  // quad_perm [1,0,3,2]
+constexpr int kDppQuadXor2 = 0x4E;  // quad_perm [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141;
+constexpr int kDppRowMirror = 0x140;
+
+// Sum over the aligned group of W (16, 32 or 64) lanes, in every lane.
+template <uint32_t W>
+__device__ inline uint32_t group_sum_dpp(uint32_t v) {
+    v += dpp_mov<kDppQuadXor1>(v);
+    v += dpp_mov<kDppQuadXor2>(v);
+    v += dpp_mov<kDppHalfMirror>(v);
+    v += dpp_mov<kDppRowMirror>(v);
+#pragma unroll
+    for (uint32_t off = 16; off < W; off <<= 1) v += __shfl_xor(v, static_cast<int>(off), W);
+    return v;
+}
+
+// Sum over the wave (every lane active), wave-uniform.
+__device__ inline uint32_t wave_total(uint32_t v) {
+    return __builtin_amdgcn_readlane(group_scan_dpp<64>(v), 63);
 }
 
 __device__ inline void emit_word(uint64_t v, uint32_t k, uint32_t rank, uint32_t cap,
@@ -819,16 +941,14 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(BatchView bv, Job j
         }
 #pragma unroll
         for (uint32_t c = 0; c < kCompactChunks; ++c) {
-            uint32_t t = cc[c];
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
+            const uint32_t t = wave_total(cc[c]);
             if (lane == 0) cbase[wave + kCompactWaves * c] = t;
         }
         __syncthreads();
         if (wave == 0) {  // exclusive scan of the round's chunk counts, 2 per lane
             static_assert(kRoundChunks == 128, "two chunks per lane");
             const uint32_t a = cbase[2 * lane], b2 = cbase[2 * lane + 1];
-            const uint32_t incl = wave_inclusive_sum(a + b2, lane);
+            const uint32_t incl = wave_inclusive_sum(a + b2);
             const uint32_t ex = carry + incl - (a + b2);
             cbase[2 * lane] = ex;
             cbase[2 * lane + 1] = ex + a;
@@ -840,7 +960,7 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(BatchView bv, Job j
             const uint32_t cl = wave + kCompactWaves * c;
             const uint32_t base = cbase[cl];
             if (r0 + cl < nchunks && base < cap) {  // wave-uniform
-                const uint32_t incl = wave_inclusive_sum(cc[c], lane);
+                const uint32_t incl = wave_inclusive_sum(cc[c]);
                 emit_word(wv[c], (r0 + cl) * 64 + lane, base + incl - cc[c], cap, G.wpr, kp);
             }
         }
@@ -1194,7 +1314,7 @@ __global__ __launch_bounds__(256) void k_match_index(MatchBatchView bv, MatchJob
             v[k] = hist[tid * kPer + k];
             sum += v[k];
         }
-        const uint32_t incl = wave_inclusive_sum(sum, tid);
+        const uint32_t incl = wave_inclusive_sum(sum);
         uint32_t run = incl - sum;
 #pragma unroll
         for (uint32_t k = 0; k < kPer; ++k) {
@@ -1224,17 +1344,6 @@ __global__ __launch_bounds__(256) void k_match_index(MatchBatchView bv, MatchJob
 }
 
 namespace {
-
-// Inclusive sum over the aligned group of W lanes that holds `sub`.
-template <uint32_t W>
-__device__ inline uint32_t group_inclusive_sum(uint32_t v, uint32_t sub) {
-#pragma unroll
-    for (uint32_t off = 1; off < W; off <<= 1) {
-        const uint32_t o = __shfl_up(v, off, W);
-        if (sub >= off) v += o;
-    }
-    return v;
-}
 
 __device__ inline float2 load_f2(const float2* base, uint32_t i) {
     const f32x2 v =
@@ -1346,12 +1455,12 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
                 len = st[cx1 + 1] - a;
                 gstart = lv * P.cap + a;
             }
-            const uint32_t incl = total + group_inclusive_sum<LANES>(len, sub);
+            const uint32_t incl = total + group_scan_dpp<LANES>(len);
             if (k < nranges) {
                 r_end[grp][k] = incl;
                 r_gb[grp][k] = gstart - (incl - len);
             }
-            total = __shfl(incl, static_cast<int>(LANES - 1), LANES);
+            total += group_sum_dpp<LANES>(len);
         }
         wave_sync();
 
@@ -1405,15 +1514,22 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
                 keep_two_smallest(min1, min2, key);
             }
         }
-#pragma unroll
-        for (uint32_t off = LANES / 2; off > 0; off >>= 1) {
-            const uint32_t o1 = __shfl_xor(min1, static_cast<int>(off), LANES);
-            const uint32_t o2 = __shfl_xor(min2, static_cast<int>(off), LANES);
+        // The two smallest keys of the group (each lane holds the two
+        // smallest of a disjoint candidate subset).
+        const auto merge = [&](uint32_t o1, uint32_t o2) {
             const uint32_t n1 = umin(min1, o1);
             const uint32_t n2 = umin(umax(min1, o1), umin(min2, o2));
             min1 = n1;
             min2 = n2;
-        }
+        };
+        merge(dpp_mov<kDppQuadXor1>(min1), dpp_mov<kDppQuadXor1>(min2));
+        merge(dpp_mov<kDppQuadXor2>(min1), dpp_mov<kDppQuadXor2>(min2));
+        merge(dpp_mov<kDppHalfMirror>(min1), dpp_mov<kDppHalfMirror>(min2));
+        merge(dpp_mov<kDppRowMirror>(min1), dpp_mov<kDppRowMirror>(min2));
+#pragma unroll
+        for (uint32_t off = 16; off < LANES; off <<= 1)
+            merge(__shfl_xor(min1, static_cast<int>(off), LANES),
+                  __shfl_xor(min2, static_cast<int>(off), LANES));
         if (min1 != 0xFFFFFFFFu) {
             const uint32_t d1 = min1 >> kMatchKeyShift;
             const uint32_t d2 = min2 == 0xFFFFFFFFu ? 0xFFFFFFFFu : (min2 >> kMatchKeyShift);
