@@ -836,8 +836,8 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
 // ---------------------------------------------------------------------------
 constexpr uint32_t kCompactThreads = 1024;
 constexpr uint32_t kCompactWaves = kCompactThreads / 64;
-constexpr uint32_t kCompactChunks = 8;                             // per wave and round
-constexpr uint32_t kRoundChunks = kCompactWaves * kCompactChunks;  // 128 (8192 words)
+constexpr uint32_t kCompactChunks = 16;                            // per wave and round
+constexpr uint32_t kRoundChunks = kCompactWaves * kCompactChunks;  // 256 (16384 words)
 
 // v plus the value of the lane the DPP control names (0 where it names none
 // or its row is masked off): one VALU with a DPP source, no LDS round trip.
@@ -945,13 +945,22 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(BatchView bv, Job j
             if (lane == 0) cbase[wave + kCompactWaves * c] = t;
         }
         __syncthreads();
-        if (wave == 0) {  // exclusive scan of the round's chunk counts, 2 per lane
-            static_assert(kRoundChunks == 128, "two chunks per lane");
-            const uint32_t a = cbase[2 * lane], b2 = cbase[2 * lane + 1];
-            const uint32_t incl = wave_inclusive_sum(a + b2);
-            const uint32_t ex = carry + incl - (a + b2);
-            cbase[2 * lane] = ex;
-            cbase[2 * lane + 1] = ex + a;
+        if (wave == 0) {  // exclusive scan of the round's chunk counts, kPer per lane
+            constexpr uint32_t kPer = kRoundChunks / 64;
+            uint32_t v[kPer];
+            uint32_t sum = 0;
+#pragma unroll
+            for (uint32_t k = 0; k < kPer; ++k) {
+                v[k] = cbase[kPer * lane + k];
+                sum += v[k];
+            }
+            const uint32_t incl = wave_inclusive_sum(sum);
+            uint32_t run = carry + incl - sum;
+#pragma unroll
+            for (uint32_t k = 0; k < kPer; ++k) {
+                cbase[kPer * lane + k] = run;
+                run += v[k];
+            }
             if (lane == 63) round_total = incl;
         }
         __syncthreads();
@@ -1279,7 +1288,14 @@ __device__ inline uint32_t cell_of(float v, float inv) {
 // cell range of candidates is read with contiguous loads.  Cell order is
 // monotone in the coordinates, so every record inside a search window lies
 // in the cells the window's (margin-widened) corners map to.
-__global__ __launch_bounds__(256) void k_match_index(MatchBatchView bv, MatchJob job0) {
+// 1024 threads; each thread loads its first kIndexHeld records once (all
+// loads issued together) and keeps them in registers across the histogram
+// and its scan; records past 1024 * kIndexHeld (caps above 4096) are
+// reloaded in the scatter pass.
+constexpr uint32_t kIndexThreads = 1024;
+constexpr uint32_t kIndexHeld = 4;
+
+__global__ __launch_bounds__(kIndexThreads) void k_match_index(MatchBatchView bv, MatchJob job0) {
     __shared__ uint32_t hist[kCells + 1];
     const uint32_t tid = threadIdx.x;
     const MatchJob M = bv.n ? copy_const(const_ptr(bv.jobs)) : job0;  // layout constants only
@@ -1293,16 +1309,59 @@ __global__ __launch_bounds__(256) void k_match_index(MatchBatchView bv, MatchJob
     uint32_t* base = side ? J.index_curr : J.index_prev;
     uint32_t* starts = base + l * (kCells + 1);
     const uint32_t n = A.count(l, P.cap);
+    const float sc = P.lv[l].lod0_scale;
+    const float inv_x = P.cell_inv_x, inv_y = P.cell_inv_y;
+    // Held records as named variables (an array indexed in a loop would be
+    // placed in scratch or LDS before the loop is unrolled).
+    struct Held {
+        uint4 head, lo, hi;
+        uint32_t cell;
+        float x, y;
+    };
+    static_assert(kIndexHeld == 4, "four held records");
+    const auto load = [&](uint32_t i) -> Held {
+        Held h{};
+        if (i < n) {
+            const uint32_t* r = A.rec(l, i);
+            h.head = *reinterpret_cast<const uint4*>(r);
+            h.lo = *reinterpret_cast<const uint4*>(r + 4);
+            h.hi = *reinterpret_cast<const uint4*>(r + 8);
+        }
+        return h;
+    };
+    Held h0 = load(tid);
+    Held h1 = load(tid + kIndexThreads);
+    Held h2 = load(tid + 2 * kIndexThreads);
+    Held h3 = load(tid + 3 * kIndexThreads);
     // The mutual pass writes only the matched entries of prev_match.
     if (side == 0)
-        for (uint32_t i = tid; i < P.cap; i += 256) J.prev_match[l * P.cap + i] = -1;
-    const float sc = P.lv[l].lod0_scale;
-    for (uint32_t c = tid; c <= kCells; c += 256) hist[c] = 0;
+        for (uint32_t i = tid; i < P.cap; i += kIndexThreads) J.prev_match[l * P.cap + i] = -1;
+    for (uint32_t c = tid; c <= kCells; c += kIndexThreads) hist[c] = 0;
+    // LOD0 position (toLod0, match_features_cpu.cpp:58-64) and cell
+    const auto pos_of = [&](uint32_t rx, uint32_t ry, float* x, float* y) -> uint32_t {
+        *x = static_cast<float>(static_cast<uint32_t>(__uint_as_float(rx))) * sc;
+        *y = static_cast<float>(static_cast<uint32_t>(__uint_as_float(ry))) * sc;
+        return cell_of(*y, inv_y) * kCellsPerAxis + cell_of(*x, inv_x);
+    };
     __syncthreads();
-    for (uint32_t i = tid; i < n; i += 256) {
+    const auto count = [&](Held h, uint32_t i) -> Held {
+        if (i < n) {
+            float x, y;
+            h.cell = pos_of(h.head.x, h.head.y, &x, &y);
+            h.x = x;
+            h.y = y;
+            atomicAdd(&hist[h.cell], 1u);
+        }
+        return h;
+    };
+    h0 = count(h0, tid);
+    h1 = count(h1, tid + kIndexThreads);
+    h2 = count(h2, tid + 2 * kIndexThreads);
+    h3 = count(h3, tid + 3 * kIndexThreads);
+    for (uint32_t i = tid + kIndexHeld * kIndexThreads; i < n; i += kIndexThreads) {
+        const uint32_t* r = A.rec(l, i);
         float x, y;
-        rec_lod0(A.rec(l, i), sc, &x, &y);
-        atomicAdd(&hist[cell_of(y, P.cell_inv_y) * kCellsPerAxis + cell_of(x, P.cell_inv_x)], 1u);
+        atomicAdd(&hist[pos_of(r[0], r[1], &x, &y)], 1u);
     }
     __syncthreads();
     if (tid < 64) {  // exclusive scan of kCells counts by one wave, 9 per lane
@@ -1330,12 +1389,23 @@ __global__ __launch_bounds__(256) void k_match_index(MatchBatchView bv, MatchJob
     uint4* sdlo = reinterpret_cast<uint4*>(base + P.idx_dlo) + g0;
     uint4* sdhi = reinterpret_cast<uint4*>(base + P.idx_dhi) + g0;
     uint32_t* spos = base + P.idx_pos + g0;
-    for (uint32_t i = tid; i < n; i += 256) {
+    const auto scatter = [&](Held h, uint32_t i) {
+        if (i < n) {
+            const uint32_t k = atomicAdd(&hist[h.cell], 1u);
+            sxy[k] = make_float2(h.x, h.y);
+            sdlo[k] = h.lo;
+            sdhi[k] = h.hi;
+            spos[k] = i;
+        }
+    };
+    scatter(h0, tid);
+    scatter(h1, tid + kIndexThreads);
+    scatter(h2, tid + 2 * kIndexThreads);
+    scatter(h3, tid + 3 * kIndexThreads);
+    for (uint32_t i = tid + kIndexHeld * kIndexThreads; i < n; i += kIndexThreads) {
         const uint32_t* r = A.rec(l, i);
         float x, y;
-        rec_lod0(r, sc, &x, &y);
-        const uint32_t c = cell_of(y, P.cell_inv_y) * kCellsPerAxis + cell_of(x, P.cell_inv_x);
-        const uint32_t k = atomicAdd(&hist[c], 1u);
+        const uint32_t k = atomicAdd(&hist[pos_of(r[0], r[1], &x, &y)], 1u);
         sxy[k] = make_float2(x, y);
         sdlo[k] = *reinterpret_cast<const uint4*>(r + 4);
         sdhi[k] = *reinterpret_cast<const uint4*>(r + 8);
@@ -1706,7 +1776,7 @@ hipError_t launch_describe(uint32_t lanes, uint32_t blocks, BatchView bv, const 
 
 hipError_t launch_match_index(uint32_t blocks, MatchBatchView bv, const MatchJob& job0,
                               hipStream_t s) {
-    hipLaunchKernelGGL(k_match_index, dim3(blocks), dim3(256), 0, s, bv, job0);
+    hipLaunchKernelGGL(k_match_index, dim3(blocks), dim3(kIndexThreads), 0, s, bv, job0);
     return hipGetLastError();
 }
 
