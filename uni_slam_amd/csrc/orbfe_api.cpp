@@ -301,11 +301,10 @@ int get_plan(orbfe_ctx* ctx, uint32_t W0, uint32_t H0, uint32_t cap, PlanEntry**
     {
         const auto al4 = [](uint32_t v) { return (v + 3u) & ~3u; };  // 16-byte aligned
         const uint32_t recs = L * cap;
-        P.idx_xy = al4(L * (orbfe::kCells + 1));
-        P.idx_dlo = al4(P.idx_xy + 2 * recs);
+        P.idx_head = al4(L * (orbfe::kCells + 1));
+        P.idx_dlo = P.idx_head + 4 * recs;
         P.idx_dhi = P.idx_dlo + 4 * recs;
-        P.idx_pos = P.idx_dhi + 4 * recs;
-        P.idx_words = al4(P.idx_pos + recs);
+        P.idx_words = P.idx_dhi + 4 * recs;
     }
     uint32_t tiles = 0, words = 0, pyr = 0, sm = 0, ntaps = 0;
     float scale = 1.0f;

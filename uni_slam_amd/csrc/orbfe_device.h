@@ -101,10 +101,11 @@ struct Plan {
     float cell_inv_x, cell_inv_y;  // 1 / cell size (cells of max_dx/2 x max_dy/2)
     // Cell index of one side (u32 words): cell starts [L][kCells + 1], then
     // the records of every level in cell order as arrays over the global
-    // record number g = level * cap + k: LOD0 positions (float2), the low and
-    // high descriptor halves (uint4 each) and record indices (u32).
+    // record number g = level * cap + k: heads (uint4: LOD0 x and y as float
+    // bits, record index, 0) and the low and high descriptor halves (uint4
+    // each), so a candidate is three 16-byte loads.
     uint32_t idx_words;   // words of one side's index
-    uint32_t idx_xy, idx_dlo, idx_dhi, idx_pos;  // word offsets of the arrays
+    uint32_t idx_head, idx_dlo, idx_dhi;  // word offsets of the arrays
     const uint64_t* mask; // static cull mask, `words` words
     const TapEntry* taps; // sampler tap tables of all levels
     LevelGeom lv[kMaxLevels];

@@ -1266,7 +1266,6 @@ constexpr uint32_t kMatchThreads = 256;
 constexpr uint32_t kCandPerLane = 2;
 // Native vector types for loads through the global address space.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 // LOD0 position of a record (toLod0, match_features_cpu.cpp:58-64).
 __device__ inline void rec_lod0(const uint32_t* r, float scale, float* x, float* y) {
@@ -1283,8 +1282,8 @@ __device__ inline uint32_t cell_of(float v, float inv) {
 // k_match_index: per (pair, side, level) a counting sort of the level's
 // records by LOD0 cell.  The index of one side holds the cell starts of
 // every level ([L][kCells + 1]) and, for every level's records in cell
-// order, the LOD0 positions, the two descriptor halves and the record
-// indices as arrays over a global record number g = level * cap + k, so a
+// order, heads (LOD0 position, record index) and the two descriptor halves
+// as 16-byte arrays over a global record number g = level * cap + k, so a
 // cell range of candidates is read with contiguous loads.  Cell order is
 // monotone in the coordinates, so every record inside a search window lies
 // in the cells the window's (margin-widened) corners map to.
@@ -1385,17 +1384,15 @@ __global__ __launch_bounds__(kIndexThreads) void k_match_index(MatchBatchView bv
     }
     __syncthreads();
     const size_t g0 = static_cast<size_t>(l) * P.cap;
-    float2* sxy = reinterpret_cast<float2*>(base + P.idx_xy) + g0;
+    uint4* shead = reinterpret_cast<uint4*>(base + P.idx_head) + g0;
     uint4* sdlo = reinterpret_cast<uint4*>(base + P.idx_dlo) + g0;
     uint4* sdhi = reinterpret_cast<uint4*>(base + P.idx_dhi) + g0;
-    uint32_t* spos = base + P.idx_pos + g0;
     const auto scatter = [&](Held h, uint32_t i) {
         if (i < n) {
             const uint32_t k = atomicAdd(&hist[h.cell], 1u);
-            sxy[k] = make_float2(h.x, h.y);
+            shead[k] = make_uint4(__float_as_uint(h.x), __float_as_uint(h.y), i, 0u);
             sdlo[k] = h.lo;
             sdhi[k] = h.hi;
-            spos[k] = i;
         }
     };
     scatter(h0, tid);
@@ -1406,20 +1403,13 @@ __global__ __launch_bounds__(kIndexThreads) void k_match_index(MatchBatchView bv
         const uint32_t* r = A.rec(l, i);
         float x, y;
         const uint32_t k = atomicAdd(&hist[pos_of(r[0], r[1], &x, &y)], 1u);
-        sxy[k] = make_float2(x, y);
+        shead[k] = make_uint4(__float_as_uint(x), __float_as_uint(y), i, 0u);
         sdlo[k] = *reinterpret_cast<const uint4*>(r + 4);
         sdhi[k] = *reinterpret_cast<const uint4*>(r + 8);
-        spos[k] = i;
     }
 }
 
 namespace {
-
-__device__ inline float2 load_f2(const float2* base, uint32_t i) {
-    const f32x2 v =
-        reinterpret_cast<const ORBFE_GLOBAL f32x2*>(reinterpret_cast<uintptr_t>(base))[i];
-    return make_float2(v.x, v.y);
-}
 
 __device__ inline uint4 load_u4(const uint4* base, uint32_t i) {
     const u32x4 v = reinterpret_cast<const ORBFE_GLOBAL u32x4*>(reinterpret_cast<uintptr_t>(base))[i];
@@ -1534,10 +1524,9 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
         }
         wave_sync();
 
-        const float2* cxy = reinterpret_cast<const float2*>(cidx + P.idx_xy);
+        const uint4* chead = reinterpret_cast<const uint4*>(cidx + P.idx_head);
         const uint4* cdlo = reinterpret_cast<const uint4*>(cidx + P.idx_dlo);
         const uint4* cdhi = reinterpret_cast<const uint4*>(cidx + P.idx_dhi);
-        const uint32_t* cpos = cidx + P.idx_pos;
         uint32_t min1 = 0xFFFFFFFFu, min2 = 0xFFFFFFFFu;
         uint32_t k = 0;
         uint32_t kend = total ? r_end[grp][0] : 0u, kgb = total ? r_gb[grp][0] : 0u;
@@ -1562,14 +1551,11 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
                     sl[u] = kslot;
                 }
             }
-            float2 xy[kCandPerLane];
-            uint32_t pos[kCandPerLane];
-            uint4 dlo[kCandPerLane], dhi[kCandPerLane];
+            uint4 head[kCandPerLane], dlo[kCandPerLane], dhi[kCandPerLane];
 #pragma unroll
             for (uint32_t u = 0; u < kCandPerLane; ++u) {
                 if (ok[u]) {
-                    xy[u] = load_f2(cxy, g[u]);
-                    pos[u] = load_u32(cpos, g[u]);
+                    head[u] = load_u4(chead, g[u]);
                     dlo[u] = load_u4(cdlo, g[u]);
                     dhi[u] = load_u4(cdhi, g[u]);
                 }
@@ -1577,10 +1563,10 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
 #pragma unroll
             for (uint32_t u = 0; u < kCandPerLane; ++u) {
                 if (!ok[u]) continue;
-                const float cx = xy[u].x, cy = xy[u].y;
+                const float cx = __uint_as_float(head[u].x), cy = __uint_as_float(head[u].y);
                 if (cx < lo_x || cx > hi_x || __builtin_fabsf(qy - cy) > dy) continue;
                 const uint32_t d = hamming256(qd, dlo[u], dhi[u]);
-                const uint32_t key = (d << kMatchKeyShift) | (sl[u] << kMatchIdxBits) | pos[u];
+                const uint32_t key = (d << kMatchKeyShift) | (sl[u] << kMatchIdxBits) | head[u].z;
                 keep_two_smallest(min1, min2, key);
             }
         }

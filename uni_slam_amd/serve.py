@@ -45,7 +45,7 @@ def frame_scratch_bytes(width: int, height: int, cap: int) -> int:
 
     def al4(v):
         return (v + 3) // 4 * 4
-    idx_words = al4(al4(al4(24 * 24 + 1) + 2 * cap) + 8 * cap + cap)
+    idx_words = al4(24 * 24 + 1) + 12 * cap
     detect = al(8 * words) + al(4 * LEVELS * cap) + al(4 * smooth)
     match = al(4 * LEVELS * cap) + 2 * al(4 * LEVELS * idx_words)
     return detect + match
