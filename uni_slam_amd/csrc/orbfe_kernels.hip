@@ -543,7 +543,66 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     // Stage the level tile ((TH + 6) x 70 texels).  Lane c stages column c
     // of rows wave, wave + 4, ... with its column taps in registers and the
     // row taps wave-uniform; columns 64..69 follow, one texel per thread.
-    {
+    // Tiles whose every texel lies in the staged source region (no
+    // border_zero, not at the level's top or left edge) take a branch-free
+    // form: the source bytes of all the lane's rows are read first, then
+    // their LUT values, then the lerps.
+    constexpr uint32_t kStageRows = (LH + kWaves - 1) / kWaves;
+    const bool stage_fast = use_src && !prm.border_zero && c_first == 0 && r_first == 0 &&
+                            c_last == kLdsW - 1 && r_last == LH - 1;
+    if (stage_fast) {
+        const uint32_t c = lane;
+        const uint32_t ci01 = col_i01[c];
+        const uint32_t a0 = (ci01 & 0xFFFFu) - cs, a1 = (ci01 >> 16) - cs;
+        const float ca = col_a[c];
+        uint32_t o0[kStageRows], o1[kStageRows];
+        float rb[kStageRows];
+#pragma unroll
+        for (uint32_t k = 0; k < kStageRows; ++k) {
+            const uint32_t r = umin(wave + k * kWaves, LH - 1);  // rows past LH: recomputed, not stored
+            const uint32_t e = __builtin_amdgcn_readfirstlane(row_i01[r]);
+            o0[k] = ((e & 0xFFFFu) - rowmin) * spitch;
+            o1[k] = ((e >> 16) - rowmin) * spitch;
+            rb[k] = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(row_b[r])));
+        }
+        if (l == 0) {
+            uint32_t b[kStageRows];
+#pragma unroll
+            for (uint32_t k = 0; k < kStageRows; ++k) b[k] = srcb[o0[k] + a0];
+#pragma unroll
+            for (uint32_t k = 0; k < kStageRows; ++k) {
+                const uint32_t r = wave + k * kWaves;
+                if (r < LH) tile[r][c] = lut[b[k]];
+            }
+        } else {
+            uint32_t b00[kStageRows], b10[kStageRows], b01[kStageRows], b11[kStageRows];
+#pragma unroll
+            for (uint32_t k = 0; k < kStageRows; ++k) {
+                b00[k] = srcb[o0[k] + a0];
+                b10[k] = srcb[o0[k] + a1];
+                b01[k] = srcb[o1[k] + a0];
+                b11[k] = srcb[o1[k] + a1];
+            }
+            float t[kStageRows];
+#pragma unroll
+            for (uint32_t k = 0; k < kStageRows; ++k)
+                t[k] = lerp2(lut[b00[k]], lut[b10[k]], lut[b01[k]], lut[b11[k]], ca, rb[k]);
+#pragma unroll
+            for (uint32_t k = 0; k < kStageRows; ++k) {
+                const uint32_t r = wave + k * kWaves;
+                if (r < LH) tile[r][c] = t[k];
+            }
+        }
+        constexpr uint32_t kExtra = (kLdsW - kTileW) * LH;
+        static_assert(kExtra <= kDetectThreads, "one extra texel per thread");
+        if (tid < kExtra) {
+            const uint32_t r = tid / (kLdsW - kTileW);
+            const uint32_t ce = kTileW + (tid - r * (kLdsW - kTileW));
+            const uint32_t ri01 = row_i01[r], ce01 = col_i01[ce];
+            tile[r][ce] = texel(ri01 & 0xFFFFu, ri01 >> 16, row_b[r], true, ce01 & 0xFFFFu,
+                                ce01 >> 16, col_a[ce], true);
+        }
+    } else {
         const uint32_t c = lane;
         const uint32_t ci01 = col_i01[c];
         const uint32_t ci0 = ci01 & 0xFFFFu, ci1 = ci01 >> 16;
