@@ -39,6 +39,19 @@ constexpr uint32_t kDescLanesDefault = 16;
 // queries alone do not).
 constexpr uint32_t kMatchLanesBatch = 16;
 constexpr uint32_t kMatchLanesPair = 64;
+// Query list of the mutual matcher pass: kListSegs segments, each with its
+// count (kListStride words apart: no two counters share a 128-byte line) and
+// room for the queries of every direction block b with b % kListSegs == s
+// (blocks of at most 16 queries; 256 / QPB queries per block for QPB >= 4).
+constexpr uint32_t kListSegs = 64;
+constexpr uint32_t kListStride = 32;
+__host__ __device__ constexpr uint32_t query_list_words(uint32_t L, uint32_t cap) {
+    return kListSegs * kListStride + L * cap + 64u * L + 64u * kListSegs;
+}
+__host__ __device__ inline uint32_t query_segment_capacity(uint32_t blocks, uint32_t qpb) {
+    return (blocks + kListSegs - 1) / kListSegs * qpb;
+}
+
 // Matcher cell index: LOD0 plane cut into kCellsPerAxis^2 cells of half the
 // search window (0.05*W0 x 0.05*H0); 24 cells cover 1.2x the frame.
 constexpr uint32_t kCellsPerAxis = 24;
@@ -128,6 +141,8 @@ struct MatchJob {
     const uint8_t* curr;
     int32_t* prev_match;  // L*cap
     int32_t* b_to_a;      // scratch L*cap
+    uint32_t* q_list;     // scratch, query_list_words(L, cap): the curr
+                          // features (level << 24 | slot) with b_to_a >= 0
     uint32_t* index_prev; // scratch: cell index of prev, plan->idx_words
     uint32_t* index_curr; // scratch: cell index of curr
 };

@@ -1391,9 +1391,11 @@ __global__ __launch_bounds__(kIndexThreads) void k_match_index(MatchBatchView bv
     Held h1 = load(tid + kIndexThreads);
     Held h2 = load(tid + 2 * kIndexThreads);
     Held h3 = load(tid + 3 * kIndexThreads);
-    // The mutual pass writes only the matched entries of prev_match.
+    // The mutual pass writes only the matched entries of prev_match; the
+    // direction pass appends to the query list.
     if (side == 0)
         for (uint32_t i = tid; i < P.cap; i += kIndexThreads) J.prev_match[l * P.cap + i] = -1;
+    if (side == 0 && l == 0 && tid < kListSegs) J.q_list[tid * kListStride] = 0;
     for (uint32_t c = tid; c <= kCells; c += kIndexThreads) hist[c] = 0;
     // LOD0 position (toLod0, match_features_cpu.cpp:58-64) and cell
     const auto pos_of = [&](uint32_t rx, uint32_t ry, float* x, float* y) -> uint32_t {
@@ -1478,13 +1480,17 @@ __device__ inline uint4 load_u4(const uint4* base, uint32_t i) {
 }  // namespace
 
 // Pipeline matcher, LANES lanes per query (256 / LANES queries per block).
-// kMutual = false: slot = curr feature b; query b, pool from prev -> b_to_a[b].
-// kMutual = true:  slot = curr feature b with a = b_to_a[b] >= 0; query a,
+// kMutual = false: slot = curr feature b; query b, pool from prev -> b_to_a[b];
+//   every b with a match is appended to segment (block % kListSegs) of the
+//   query list (one atomic per wave on that segment's counter).
+// kMutual = true:  block b takes entries (b / kListSegs) * QPB + group of
+//   segment b % kListSegs; an entry names b, a = b_to_a[b]; query a,
 //   pool from curr; prev_match[a] = b when a's best match is b.  This is the
 //   reference's mutual check (match_features_cpu.cpp:196-205: a matches
 //   iff its best match b has b_to_a[b] == a) evaluated only for the prev
 //   features some b points at; every other entry keeps the -1 written by
-//   k_match_index.
+//   k_match_index.  The list's order varies from run to run; each entry's
+//   result does not depend on it.
 //
 // A query's candidates are the records of its pool levels (l, l-1, l+1, the
 // reference's concatenation order) in the cells under its search window:
@@ -1510,22 +1516,33 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
     }
     const MatchJob M = bv.n ? copy_const(const_ptr(bv.jobs + j)) : job0;
     ConstPlan P = as_const(M.plan);
-    const uint32_t groups = (P.cap + QPB - 1) / QPB;  // blocks per level
-    const uint32_t l = b / groups;
-    const uint32_t slot = (b - l * groups) * QPB + grp;
-    if (slot >= P.cap) return;
     const FeatView Q{kMutual ? M.prev : M.curr, P.feat_stride};
     const FeatView C{kMutual ? M.curr : M.prev, P.feat_stride};
     const uint32_t* cidx = kMutual ? M.index_curr : M.index_prev;
-    if (slot >= FeatView{M.curr, P.feat_stride}.count(l, P.cap)) {
-        if (!kMutual && sub == 0) M.b_to_a[l * P.cap + slot] = -1;
-        return;
+    uint32_t l, slot;  // the curr feature b
+    if (!kMutual) {
+        const uint32_t groups = (P.cap + QPB - 1) / QPB;  // blocks per level
+        l = b / groups;
+        slot = (b - l * groups) * QPB + grp;
+        if (slot >= P.cap) return;
+        if (slot >= FeatView{M.curr, P.feat_stride}.count(l, P.cap)) {
+            if (sub == 0) M.b_to_a[l * P.cap + slot] = -1;
+            return;
+        }
+    } else {
+        const uint32_t seg = b % kListSegs;
+        const uint32_t t = (b / kListSegs) * QPB + grp;
+        if (t >= M.q_list[seg * kListStride]) return;  // written by the direction pass
+        const uint32_t blocks = P.L * ((P.cap + QPB - 1) / QPB);
+        const uint32_t e = M.q_list[kListSegs * kListStride +
+                                    seg * query_segment_capacity(blocks, QPB) + t];
+        l = e >> 24;
+        slot = e & 0xFFFFFFu;
     }
     // Query: feature (ql, qs) of the query side.
     uint32_t ql = l, qs = slot;
     if (kMutual) {
         const int32_t ab = M.b_to_a[l * P.cap + slot];
-        if (ab < 0) return;
         ql = static_cast<uint32_t>(ab) >> 24;
         qs = static_cast<uint32_t>(ab) & 0xFFFFFFu;
     }
@@ -1655,11 +1672,26 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
             }
         }
     }
-    if (sub == 0) {
-        if (!kMutual)
-            M.b_to_a[l * P.cap + slot] = result;
-        else if (result == static_cast<int32_t>((l << 24) | slot))
-            M.prev_match[ql * P.cap + qs] = result;
+    if (!kMutual) {
+        if (sub == 0) M.b_to_a[l * P.cap + slot] = result;
+        // append b to the query list: one atomic per wave
+        const uint32_t lane = tid & 63;
+        const bool claim = sub == 0 && result >= 0;
+        const uint64_t m = __ballot(claim);
+        if (m) {
+            const uint32_t seg = b % kListSegs;
+            const uint32_t blocks = P.L * ((P.cap + QPB - 1) / QPB);
+            const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(m))) - 1;
+            uint32_t base = 0;
+            if (lane == leader)
+                base = atomicAdd(M.q_list + seg * kListStride, static_cast<uint32_t>(__popcll(m)));
+            base = __builtin_amdgcn_readlane(base, leader);
+            if (claim)
+                M.q_list[kListSegs * kListStride + seg * query_segment_capacity(blocks, QPB) +
+                         base + __popcll(m & ((1ull << lane) - 1ull))] = (l << 24) | slot;
+        }
+    } else if (sub == 0 && result == static_cast<int32_t>((l << 24) | slot)) {
+        M.prev_match[ql * P.cap + qs] = result;
     }
 }
 

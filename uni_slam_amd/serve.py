@@ -34,8 +34,8 @@ LEVELS = 6
 def frame_scratch_bytes(width: int, height: int, cap: int) -> int:
     """Device scratch one in-flight frame uses in a batch, as laid out by
     liborbfe: survivor bitmap, keypoint list and smoothed image of the
-    detect stage; match table and the two sides' cell indices (cell starts
-    and the records sorted by cell) of the match stage."""
+    detect stage; match table, query list and the two sides' cell indices
+    (cell starts and the records sorted by cell) of the match stage."""
     dims = orbfe.level_dims(width, height)
     words = sum(h * ((w + 63) // 64) for w, h in dims)
     smooth = sum((w + 4) * (h + 4) for w, h in dims)
@@ -47,7 +47,8 @@ def frame_scratch_bytes(width: int, height: int, cap: int) -> int:
         return (v + 3) // 4 * 4
     idx_words = al4(24 * 24 + 1) + 12 * cap
     detect = al(8 * words) + al(4 * LEVELS * cap) + al(4 * smooth)
-    match = al(4 * LEVELS * cap) + 2 * al(4 * LEVELS * idx_words)
+    qlist_words = 64 * 32 + LEVELS * cap + 64 * LEVELS + 64 * 64
+    match = al(4 * LEVELS * cap) + al(4 * qlist_words) + 2 * al(4 * LEVELS * idx_words)
     return detect + match
 
 
