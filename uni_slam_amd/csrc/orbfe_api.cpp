@@ -21,7 +21,7 @@ namespace orbfe {
 hipError_t upload_pattern();
 hipError_t launch_detect(uint32_t tile_h, uint32_t blocks, BatchView bv, const Job& job0,
                          const Params& prm, hipStream_t s);
-hipError_t launch_compact(uint32_t blocks, BatchView bv, const Job& job0, uint32_t levels,
+hipError_t launch_compact(uint32_t frames, BatchView bv, const Job& job0, uint32_t levels,
                           hipStream_t s);
 hipError_t launch_describe(uint32_t lanes, uint32_t blocks, BatchView bv, const Job& job0,
                            const Params& prm, hipStream_t s);
@@ -393,13 +393,13 @@ hipError_t timed(orbfe_ctx* ctx, const char* name, hipStream_t s, F&& launch) {
 }
 
 int launch_detect_stage(orbfe_ctx* ctx, const BatchView& bv, const Job& job0, uint32_t tiles,
-                        uint32_t blocks_compact, uint32_t desc_blocks, hipStream_t s) {
+                        uint32_t frames, uint32_t desc_blocks, hipStream_t s) {
     const Params prm = ctx->prm;
     HIP_TRY(ctx, timed(ctx, "k_detect", s, [&] {
                 return orbfe::launch_detect(ctx->cfg.detect_tile_rows, tiles, bv, job0, prm, s);
             }));
     HIP_TRY(ctx, timed(ctx, "k_compact", s, [&] {
-                return orbfe::launch_compact(blocks_compact, bv, job0, ctx->cfg.levels, s);
+                return orbfe::launch_compact(frames, bv, job0, ctx->cfg.levels, s);
             }));
     HIP_TRY(ctx, timed(ctx, "k_describe", s, [&] {
                 return orbfe::launch_describe(ctx->cfg.describe_lanes, desc_blocks, bv, job0, prm,
@@ -772,7 +772,7 @@ int orbfe_detect_async(orbfe_ctx* ctx, const uint8_t* image, size_t pitch, float
     job0.kp = ctx->kp;
     job0.smooth = ctx->smooth;
     const BatchView bv{nullptr, nullptr, nullptr, nullptr, nullptr, 0};
-    return launch_detect_stage(ctx, bv, job0, P.tiles, P.L, P.L * P.desc_groups, stream);
+    return launch_detect_stage(ctx, bv, job0, P.tiles, 1, P.L * P.desc_groups, stream);
 }
 
 int orbfe_detect(orbfe_ctx* ctx, const uint8_t* image, size_t pitch, float* pyramid,
@@ -817,7 +817,7 @@ int orbfe_detect_batch_async(orbfe_ctx* ctx, const orbfe_frame* frames, uint32_t
     rc = commit_table(ctx, slot, lay.table_bytes, stream);
     if (rc) return rc;
     const BatchView bv = detect_view(lay, static_cast<uint8_t*>(slot->dev));
-    return launch_detect_stage(ctx, bv, Job{}, lay.tile_prefix[n], n * ctx->cfg.levels,
+    return launch_detect_stage(ctx, bv, Job{}, lay.tile_prefix[n], n,
                                lay.desc_prefix[n], stream);
 }
 
@@ -857,7 +857,7 @@ int orbfe_batch_create(orbfe_ctx* ctx, const orbfe_frame* frames, uint32_t n,
 
 int orbfe_batch_detect_async(orbfe_ctx* ctx, orbfe_batch* batch, hipStream_t stream) {
     if (!ctx || !batch || batch->ctx != ctx) return fail(ctx, ORBFE_EINVAL, "bad batch");
-    return launch_detect_stage(ctx, batch->view, Job{}, batch->tiles, batch->n * ctx->cfg.levels,
+    return launch_detect_stage(ctx, batch->view, Job{}, batch->tiles, batch->n,
                                batch->desc_blocks, stream);
 }
 

@@ -885,13 +885,18 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
 }
 
 // ---------------------------------------------------------------------------
-// k_compact: one workgroup of 16 waves per (frame, level).  The level's
-// survivor words are cut into chunks of 64 words (one per lane, read
-// coalesced); in each round of up to kRoundChunks chunks, chunk c goes to
-// wave c % 16.  A round counts its chunks (words kept in registers), a block
-// scan over the chunk counts gives every chunk its base rank, then every
-// wave ranks the survivors of its chunks whose base is below the cap and
-// writes their keypoints, all waves at once.
+// k_compact: `parts` workgroups of 16 waves per (frame, level), part p
+// taking the p-th slice of the level's survivor words (whole 64-word
+// chunks): 8 for a single frame (the level's work spread over 8 CUs), 1 for
+// batches (enough workgroups already; no recount).  A part first counts the survivors of the words before its slice
+// (a popcount pass over words the detect kernel just wrote) for its base
+// rank, so the parts need nothing from each other.  Its slice is cut into
+// chunks of 64 words (one per lane, read coalesced); in each round of up to
+// kRoundChunks chunks, chunk c goes to wave c % 16.  A round counts its
+// chunks (words kept in registers), a block scan over the chunk counts gives
+// every chunk its base rank, then every wave ranks the survivors of its
+// chunks whose base is below the cap and writes their keypoints, all waves
+// at once.  The last part writes the level's survivor count.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kCompactThreads = 1024;
 constexpr uint32_t kCompactWaves = kCompactThreads / 64;
@@ -970,32 +975,63 @@ __device__ inline void emit_word(uint64_t v, uint32_t k, uint32_t rank, uint32_t
 }
 
 __global__ __launch_bounds__(kCompactThreads) void k_compact(BatchView bv, Job job0,
-                                                              uint32_t levels) {
+                                                              uint32_t levels, uint32_t parts) {
     __shared__ uint32_t cbase[kRoundChunks];
+    __shared__ uint32_t wsum[kCompactWaves];
     __shared__ uint32_t round_total;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63;
     const uint32_t wave = tid >> 6;
-    const uint32_t j = blockIdx.x / levels;
-    const uint32_t l = blockIdx.x - j * levels;
+    const uint32_t per_frame = levels * parts;
+    const uint32_t j = blockIdx.x / per_frame;
+    const uint32_t rem = blockIdx.x - j * per_frame;
+    const uint32_t l = rem / parts;
+    const uint32_t part = rem - l * parts;
     const Job J = bv.n ? copy_const(const_ptr(bv.jobs + j)) : job0;
     ConstPlan P = as_const(J.plan);
     const LevelGeom G = copy_const(&P.lv[l]);  // by value: stores below cannot alias it
     const uint32_t words = G.h * G.wpr;
     const uint32_t nchunks = (words + 63) / 64;
+    const uint32_t part_chunks = (nchunks + parts - 1) / parts;
+    const uint32_t c0 = umin(part * part_chunks, nchunks);
+    const uint32_t c1 = umin(c0 + part_chunks, nchunks);
     const uint64_t* surv = J.surv + G.word_off;
     uint32_t* kp = J.kp + G.kp_off;
     const uint32_t cap = P.cap;
+    const bool last = part == parts - 1;
 
-    uint32_t carry = 0;  // survivors of the rounds before (block-uniform)
-    for (uint32_t r0 = 0; r0 < nchunks; r0 += kRoundChunks) {
+    // Survivors before the slice: words [0, 64 * c0).
+    uint32_t carry = 0;  // block-uniform
+    if (c0 > 0) {
+        const uint32_t w0 = umin(64 * c0, words);
+        uint32_t pc = 0;
+        constexpr uint32_t kU = 4;  // loads in flight per thread
+        for (uint32_t k0 = 0; k0 < w0; k0 += kU * kCompactThreads) {
+            uint64_t v[kU];
+#pragma unroll
+            for (uint32_t u = 0; u < kU; ++u) {
+                const uint32_t k = k0 + u * kCompactThreads + tid;
+                v[u] = k < w0 ? surv[k] : 0ull;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kU; ++u) pc += __popcll(v[u]);
+        }
+        const uint32_t t = wave_total(pc);
+        if (lane == 0) wsum[wave] = t;
+        __syncthreads();
+#pragma unroll
+        for (uint32_t w = 0; w < kCompactWaves; ++w) carry += wsum[w];
+        if (carry >= cap && !last) return;  // block-uniform: nothing of the slice is kept
+    }
+
+    for (uint32_t r0 = c0; r0 < c1; r0 += kRoundChunks) {
         uint64_t wv[kCompactChunks];
         uint32_t cc[kCompactChunks];
 #pragma unroll
         for (uint32_t c = 0; c < kCompactChunks; ++c) {
             const uint32_t chunk = r0 + wave + kCompactWaves * c;
             const uint32_t k = chunk * 64 + lane;
-            wv[c] = (chunk < nchunks && k < words) ? surv[k] : 0ull;
+            wv[c] = (chunk < c1 && k < words) ? surv[k] : 0ull;
             cc[c] = __popcll(wv[c]);
         }
 #pragma unroll
@@ -1027,7 +1063,7 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(BatchView bv, Job j
         for (uint32_t c = 0; c < kCompactChunks; ++c) {
             const uint32_t cl = wave + kCompactWaves * c;
             const uint32_t base = cbase[cl];
-            if (r0 + cl < nchunks && base < cap) {  // wave-uniform
+            if (r0 + cl < c1 && base < cap) {  // wave-uniform
                 const uint32_t incl = wave_inclusive_sum(cc[c]);
                 emit_word(wv[c], (r0 + cl) * 64 + lane, base + incl - cc[c], cap, G.wpr, kp);
             }
@@ -1035,7 +1071,7 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(BatchView bv, Job j
         carry += round_total;
         __syncthreads();  // cbase and round_total are rewritten by the next round
     }
-    if (tid == 0) {
+    if (last && tid == 0) {
         uint32_t* hdr = reinterpret_cast<uint32_t*>(J.feat + G.feat_off);
         hdr[0] = carry;
         hdr[1] = 0;
@@ -1834,9 +1870,11 @@ hipError_t launch_detect(uint32_t tile_h, uint32_t blocks, BatchView bv, const J
 }
 
 
-hipError_t launch_compact(uint32_t blocks, BatchView bv, const Job& job0, uint32_t levels,
+hipError_t launch_compact(uint32_t frames, BatchView bv, const Job& job0, uint32_t levels,
                           hipStream_t s) {
-    hipLaunchKernelGGL(k_compact, dim3(blocks), dim3(kCompactThreads), 0, s, bv, job0, levels);
+    const uint32_t parts = frames == 1 ? 8u : 1u;
+    hipLaunchKernelGGL(k_compact, dim3(frames * levels * parts), dim3(kCompactThreads), 0, s, bv,
+                       job0, levels, parts);
     return hipGetLastError();
 }
 
