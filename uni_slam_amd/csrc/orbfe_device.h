@@ -95,7 +95,8 @@ struct LevelGeom {
     // Entry offset of the level's tap tables in Plan::taps: level texel
     // column cx (w entries) and row cy (h entries) sampled from level 0 at
     // (c + 0.5) / n, then the 9-tap Gaussian's column taps [3][w + 4] and
-    // row taps [3][h + 4] at offsets -1.2, 0, 1.2 texels.
+    // row taps [3][h + 4] at offsets -1.2, 0, 1.2 texels, then the smoothing
+    // tap codes (smooth_tap_code) of every column [w + 4] and row [h + 4].
     uint32_t tap_off;
     uint32_t pad_;
 };
@@ -286,8 +287,17 @@ __host__ __device__ inline void level_tap_offsets(uint32_t base, uint32_t w, uin
 }
 
 __host__ __device__ inline uint32_t level_tap_entries(uint32_t w, uint32_t h) {
-    return w + h + 3 * (w + kSmoothExt) + 3 * (h + kSmoothExt);
+    return w + h + 4 * (w + kSmoothExt) + 4 * (h + kSmoothExt);
 }
+
+// Entry offsets of the smoothing tap codes of a level (after its four
+// tables): one per column of the extended grid, then one per row.
+__host__ __device__ inline void level_code_offsets(uint32_t base, uint32_t w, uint32_t h,
+                                                   uint32_t* col_code, uint32_t* row_code) {
+    *col_code = base + w + h + 3 * (w + kSmoothExt) + 3 * (h + kSmoothExt);
+    *row_code = *col_code + w + kSmoothExt;
+}
+
 
 __host__ __device__ inline float lerp2(float t00, float t10, float t01, float t11,
                                        float a, float b) {
@@ -300,6 +310,33 @@ __host__ __device__ inline float lerp2(float t00, float t10, float t01, float t1
 __host__ __device__ inline uint32_t tint_clamp(int32_t c, uint32_t n) {
     const uint32_t u = static_cast<uint32_t>(c);
     return u < n - 1 ? u : n - 1;
+}
+
+// Smoothing tap code of point p along an axis of n texels, from the table
+// entries (i0 | i1 << 16) of its three taps at offsets -1.2, 0, 1.2: the
+// kernels' tile staging holds the texel of tint_clamp(g) at staged
+// coordinate g (0 outside the level with border_zero), so a tap whose entry
+// equals the pattern for sample floor f reads exactly the staged texels f,
+// f + 1.  Bit 0: the floors are p - 2, p or p - 1, p + 1 under tint_clamp;
+// bit 1: the middle tap's floor is p - 1 (bit 0 set); bits 2, 3: the same
+// with border_zero (floors inside the level).
+enum : uint32_t { kCodeFast = 1u, kCodeMidLow = 2u, kCodeFastBz = 4u, kCodeMidLowBz = 8u };
+
+__host__ __device__ inline uint32_t smooth_tap_code(uint32_t e0, uint32_t e1, uint32_t e2,
+                                                    int32_t p, uint32_t n) {
+    const auto pat = [n](int32_t f) { return tint_clamp(f, n) | (tint_clamp(f + 1, n) << 16); };
+    const auto pat_bz = [n](int32_t f) -> uint32_t {
+        return f >= 0 && f + 1 < static_cast<int32_t>(n)
+                   ? static_cast<uint32_t>(f) | (static_cast<uint32_t>(f + 1) << 16)
+                   : 0xFFFFFFFFu;  // never a table entry
+    };
+    uint32_t code = 0;
+    const bool a = e1 == pat(p), b = e1 == pat(p - 1);
+    if (e0 == pat(p - 2) && e2 == pat(p + 1) && (a || b)) code |= kCodeFast | (a ? 0u : kCodeMidLow);
+    const bool az = e1 == pat_bz(p), bzz = e1 == pat_bz(p - 1);
+    if (e0 == pat_bz(p - 2) && e2 == pat_bz(p + 1) && (az || bzz))
+        code |= kCodeFastBz | (az ? 0u : kCodeMidLowBz);
+    return code;
 }
 
 }  // namespace orbfe

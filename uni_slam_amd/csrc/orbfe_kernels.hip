@@ -649,48 +649,38 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     // tint_clamp(g) (0 outside the level when border_zero), so a tap whose
     // table entry (i0, i1) equals (tint_clamp(f), tint_clamp(f + 1)) for a
     // sample floor f (border_zero: (f, f + 1), inside the level) reads
-    // exactly the staged texels f, f + 1.  The floors of the three taps are
-    // x - 2, x or x - 1, and x + 1 wherever the table confirms it (all but
-    // the first two columns and rows of a level at the sizes in use): the
-    // horizontal lerps of a wave's rows y - 2 .. y + RPT + 1 then sit in a
+    // exactly the staged texels f, f + 1.  Where the plan's tap codes
+    // (smooth_tap_code) confirm the floors x - 2, x or x - 1, and x + 1 (all
+    // but the first two columns and rows of a level at the sizes in use),
+    // the horizontal lerps of a wave's rows y - 2 .. y + RPT + 1 sit in a
     // register window shared by its RPT outputs.  Other lanes and waves
-    // evaluate each sample from the staged texels the table names.
+    // evaluate each sample from the staged texels the table names.  A
+    // middle row tap with fraction 0 (most rows) yields its first operand
+    // exactly (no value here is -0, infinite or NaN), so its lerp is skipped.
     const bool slow = x0 > static_cast<int32_t>(w) + 2 || y0 > static_cast<int32_t>(h) + 2;
     if (!slow) {
         if (x < sw) {
+            uint32_t t_cc, t_rc;
+            level_code_offsets(G.tap_off, w, h, &t_cc, &t_rc);
             TapEntry tp[3];
 #pragma unroll
             for (int i = 0; i < 3; ++i) tp[i] = load_tap(taps, t_sc + i * sw + x);
+            const uint32_t ccode = load_tap(taps, t_cc + x).i01;
             const bool bz = prm.border_zero != 0;
-            // the table entry under which a tap with sample floor f on an
-            // axis of n texels reads the staged texels f, f + 1
-            const auto expect = [bz](int32_t f, uint32_t n) -> uint32_t {
-                if (bz)
-                    return f >= 0 && f + 1 < static_cast<int32_t>(n)
-                               ? static_cast<uint32_t>(f) | (static_cast<uint32_t>(f + 1) << 16)
-                               : 0xFFFFFFFFu;  // never a table entry
-                return tint_clamp(f, n) | (tint_clamp(f + 1, n) << 16);
-            };
-            const int32_t xi = static_cast<int32_t>(x);
-            const bool col_a1 = tp[1].i01 == expect(xi, w);
-            const bool col_fast = tp[0].i01 == expect(xi - 2, w) &&
-                                  tp[2].i01 == expect(xi + 1, w) &&
-                                  (col_a1 || tp[1].i01 == expect(xi - 1, w));
-            // wave-uniform: rows of this wave's outputs
+            const uint32_t fast_bit = bz ? kCodeFastBz : kCodeFast;
+            const uint32_t low_bit = bz ? kCodeMidLowBz : kCodeMidLow;
+            const bool col_fast = (ccode & fast_bit) != 0;
+            const bool col_a1 = (ccode & low_bit) == 0;
+            // wave-uniform (scalar loads): rows of this wave's outputs
             bool row_fast = true;
             uint32_t row_b1 = 0;  // bit k: middle tap of output k has floor y - 1
 #pragma unroll
             for (int k = 0; k < RPT; ++k) {
-                const uint32_t rr = wave * RPT + k;
-                const int32_t yi = y0 + static_cast<int32_t>(rr);
-                if (static_cast<uint32_t>(yi) >= sh) continue;
-                const uint32_t e0 = __builtin_amdgcn_readfirstlane(srow_i01[0][rr]);
-                const uint32_t e1 = __builtin_amdgcn_readfirstlane(srow_i01[1][rr]);
-                const uint32_t e2 = __builtin_amdgcn_readfirstlane(srow_i01[2][rr]);
-                const bool a1 = e1 == expect(yi, h);
-                row_fast = row_fast && e0 == expect(yi - 2, h) && e2 == expect(yi + 1, h) &&
-                           (a1 || e1 == expect(yi - 1, h));
-                if (!a1) row_b1 |= 1u << k;
+                const uint32_t y = static_cast<uint32_t>(y0) + wave * RPT + k;
+                const uint32_t rcode = copy_const(const_ptr(taps + t_rc + umin(y, sh - 1))).i01;
+                if (y >= sh) continue;
+                row_fast = row_fast && (rcode & fast_bit) != 0;
+                if (rcode & low_bit) row_b1 |= 1u << k;
             }
             const float fr0 = tp[0].frac, fr1 = tp[1].frac, fr2 = tp[2].frac;
             if (row_fast && col_fast) {
@@ -715,6 +705,14 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
                     for (int jj = 0; jj < 3; ++jj) {
                         const float bb = __uint_as_float(
                             __builtin_amdgcn_readfirstlane(__float_as_uint(srow_b[jj][rr])));
+                        if (jj == 1 && bb == 0.0f) {  // wave-uniform
+#pragma unroll
+                            for (int i = 0; i < 3; ++i) {
+                                const float top = b1 ? hv[i][k + 1] : hv[i][k + 2];
+                                sum = sum + top * wts[i] * wts[jj];
+                            }
+                            continue;
+                        }
 #pragma unroll
                         for (int i = 0; i < 3; ++i) {
                             float top, bot;
