@@ -157,11 +157,66 @@ void build_level_taps(const Plan& P, const LevelGeom& G, orbfe::TapEntry* out) {
                        0.0f};
 }
 
+// Tile descriptors of one level (TileDesc): the source rectangle of level 0
+// that a tile's staged texels (70 columns x TH + 6 rows from (x0 - 3, y0 -
+// 3), coordinates clamped by the textureLoad rule) sample.  Staged columns
+// and rows with g >= 0 map monotonically onto the frame, so the rectangle's
+// corners are the taps of the first and last staged column and row;
+// negative g, which Tint sends to the far edge, lie outside it and are
+// gathered by the kernel.
+void build_level_tiles(const Plan& P, const LevelGeom& G, uint32_t l,
+                       const orbfe::TapEntry* taps, bool border_zero, orbfe::TileDesc* out) {
+    const int32_t th = static_cast<int32_t>(P.tile_h);
+    const int32_t lh = th + 2 * orbfe::kHalo;
+    const int32_t w = static_cast<int32_t>(G.w), h = static_cast<int32_t>(G.h);
+    uint32_t lc, lr, sc, sr;
+    orbfe::level_tap_offsets(G.tap_off, G.w, G.h, &lc, &lr, &sc, &sr);
+    const auto clamp_coord = [&](int32_t g, uint32_t n) -> uint32_t {
+        const bool inside = g >= 0 && g < static_cast<int32_t>(n);
+        return border_zero ? (inside ? static_cast<uint32_t>(g) : 0u) : orbfe::tint_clamp(g, n);
+    };
+    for (uint32_t ty = 0; ty < G.tiles_y; ++ty) {
+        for (uint32_t tx = 0; tx < G.tiles_x; ++tx) {
+            const int32_t x0 = static_cast<int32_t>(tx) * orbfe::kTileW;
+            const int32_t y0 = static_cast<int32_t>(ty) * th;
+            const int32_t c_first = x0 >= orbfe::kHalo ? 0 : orbfe::kHalo - x0;
+            const int32_t r_first = y0 >= orbfe::kHalo ? 0 : orbfe::kHalo - y0;
+            int32_t c_last = orbfe::kLdsW - 1, r_last = lh - 1;
+            if (border_zero) {
+                const int32_t cl = w - 1 - x0 + orbfe::kHalo, rl = h - 1 - y0 + orbfe::kHalo;
+                c_last = cl < orbfe::kLdsW - 1 ? (cl < 0 ? 0 : cl) : orbfe::kLdsW - 1;
+                r_last = rl < lh - 1 ? (rl < 0 ? 0 : rl) : lh - 1;
+            }
+            uint32_t colmin = clamp_coord(x0 - orbfe::kHalo + c_first, G.w);
+            uint32_t colmax = clamp_coord(x0 - orbfe::kHalo + c_last, G.w);
+            uint32_t rowmin = clamp_coord(y0 - orbfe::kHalo + r_first, G.h);
+            uint32_t rowmax = clamp_coord(y0 - orbfe::kHalo + r_last, G.h);
+            if (l > 0) {
+                colmin = taps[lc + colmin].i01 & 0xFFFFu;
+                colmax = taps[lc + colmax].i01 >> 16;
+                rowmin = taps[lr + rowmin].i01 & 0xFFFFu;
+                rowmax = taps[lr + rowmax].i01 >> 16;
+            }
+            const uint32_t cs = colmin & ~3u;
+            const uint32_t nd = colmax >= cs ? (colmax - cs) / 4 + 1 : 0;
+            const uint32_t nrows = rowmax >= rowmin ? rowmax - rowmin + 1 : 0;
+            const bool fits = colmax >= colmin && rowmax >= rowmin && nd <= 64 &&
+                              nd * nrows <= orbfe::detect_src_words(P.tile_h);
+            orbfe::TileDesc& d = out[ty * G.tiles_x + tx];
+            d.lxy = l | (tx << 4) | (ty << 16);
+            d.cs = cs;
+            d.rowmin = rowmin;
+            d.nd_rows = fits ? (nd | (nrows << 16) | (1u << 31)) : 0u;
+        }
+    }
+}
+
 struct PlanEntry {
     Plan host{};
     Plan* dev = nullptr;
     uint64_t* mask = nullptr;
     orbfe::TapEntry* taps = nullptr;
+    orbfe::TileDesc* tiles = nullptr;
     // per-frame scratch sizes (bytes), 256-aligned
     size_t surv_bytes = 0, kp_bytes = 0, pyr_bytes = 0, feat_bytes = 0, match_bytes = 0;
     size_t sm_bytes = 0, index_bytes = 0;
@@ -366,6 +421,14 @@ int get_plan(orbfe_ctx* ctx, uint32_t W0, uint32_t H0, uint32_t cap, PlanEntry**
     HIP_TRY(ctx, hipMemcpy(pe->taps, taps.data(), taps.size() * sizeof(orbfe::TapEntry),
                            hipMemcpyHostToDevice));
     P.taps = pe->taps;
+    std::vector<orbfe::TileDesc> td(tiles ? tiles : 1);
+    for (uint32_t l = 0; l < L; ++l)
+        build_level_tiles(P, P.lv[l], l, taps.data(), ctx->prm.border_zero != 0,
+                          td.data() + P.lv[l].tile_base);
+    rc = dalloc(ctx, &pe->tiles, td.size() * sizeof(orbfe::TileDesc));
+    if (rc) return rc;
+    HIP_TRY(ctx, hipMemcpy(pe->tiles, td.data(), td.size() * sizeof(orbfe::TileDesc),
+                           hipMemcpyHostToDevice));
     rc = dalloc(ctx, &pe->dev, sizeof(Plan));
     if (rc) return rc;
     HIP_TRY(ctx, hipMemcpy(pe->dev, &P, sizeof(Plan), hipMemcpyHostToDevice));
@@ -482,6 +545,7 @@ int layout_detect(orbfe_ctx* ctx, const orbfe_frame* frames, uint32_t n, DetectL
         if (rc) return rc;
         Job& J = lay.jobs[i];
         J.plan = pe->dev;
+        J.tiles = pe->tiles;
         J.img = f.image;
         J.pitch = f.pitch;
         J.feat = static_cast<uint8_t*>(f.features);
@@ -666,6 +730,7 @@ void orbfe_destroy(orbfe_ctx* ctx) {
     for (auto& kv : ctx->plans) {
         dfree(kv.second->mask);
         dfree(kv.second->taps);
+        dfree(kv.second->tiles);
         dfree(kv.second->dev);
     }
     dfree(ctx->surv);
@@ -776,6 +841,7 @@ int orbfe_detect_async(orbfe_ctx* ctx, const uint8_t* image, size_t pitch, float
     if (pitch < P.W0) return fail(ctx, ORBFE_EINVAL, "pitch %zu < width %u", pitch, P.W0);
     Job job0{};
     job0.plan = ctx->main->dev;
+    job0.tiles = ctx->main->tiles;
     job0.img = image;
     job0.pitch = pitch;
     job0.pyr = pyramid ? pyramid : ctx->pyr_scratch;

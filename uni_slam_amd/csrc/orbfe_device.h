@@ -125,8 +125,29 @@ struct Plan {
     LevelGeom lv[kMaxLevels];
 };
 
+// One detect tile of a plan, filled on the host with the tile decode the
+// kernel would otherwise repeat: level | tile_x << 4 | tile_y << 16, then the
+// source rectangle of level 0 its staged texels sample (first dword column
+// cs, first row rowmin, nd dwords per row, nrows rows) and whether it fits
+// the LDS source buffer (bit 31 of nd_rows).
+struct TileDesc {
+    uint32_t lxy;
+    uint32_t cs;
+    uint32_t rowmin;
+    uint32_t nd_rows;  // nd | nrows << 16 | fits << 31
+};
+
+// Dwords of the detect kernel's LDS source buffer for tiles of th rows: at
+// level scales up to 2.5 (six levels of 1.2) a tile's source rectangle spans
+// at most (th+6)*2.5 + 4 rows of 45 dwords (70 texels * 2.5 plus the
+// bilinear neighbour and the dword alignment).
+__host__ __device__ constexpr uint32_t detect_src_words(uint32_t th) {
+    return ((th + 6) * 5u / 2u + 4u) * 45u;
+}
+
 struct Job {
     const Plan* plan;
+    const TileDesc* tiles;  // the plan's tile descriptors
     const uint8_t* img;
     uint64_t pitch;
     float* pyr;        // nullable

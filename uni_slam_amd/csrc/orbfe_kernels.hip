@@ -198,7 +198,7 @@ __device__ inline float level_texel_direct(const Job& J, ConstPlan P, const Leve
 // TH = 8, 16, 32.  Regions that do not fit (levels past the sixth) fall back
 // to per-texel gathers.
 template <int TH>
-constexpr uint32_t stage_words() { return ((TH + 6) * 5u / 2u + 4u) * 45u; }
+constexpr uint32_t stage_words() { return detect_src_words(TH); }
 
 // Byte i of the staged source region, read as a dword and shifted, so the
 // load stays a 32-bit LDS read.
@@ -234,11 +234,11 @@ __device__ float smooth_direct(const Job& J, ConstPlan P, const LevelGeom& G, ui
 }
 
 // Everything a workgroup needs to know about one detect tile, from scalar
-// loads: the frame's job, the level geometry, the tile origin, the tap
-// table offsets and the source rectangle of level 0 that the tile's texels
-// sample (staged columns/rows with g >= 0 map monotonically onto the
-// frame, so their source texels form one rectangle; negative g, which Tint
-// sends to the far edge, and rectangles that do not fit use gathers).
+// loads: the frame's job, the tile's descriptor (level, position and the
+// source rectangle of level 0 its staged texels sample, precomputed with the
+// plan), the level geometry and the tap table offsets.  The source row loads
+// need only the job and the descriptor, so they are issued while the level
+// geometry is still loading.
 struct DetectTile {
     Job J;
     LevelGeom G;
@@ -254,7 +254,6 @@ template <int TH>
 __device__ inline DetectTile decode_tile(const BatchView& bv, const Job& job0, const Params& prm,
                                          uint32_t b) {
     constexpr int LH = TH + 2 * kHalo;
-    constexpr uint32_t kSrcWords = stage_words<TH>();
     DetectTile T;
     uint32_t j = 0;
     if (bv.n) {
@@ -262,15 +261,20 @@ __device__ inline DetectTile decode_tile(const BatchView& bv, const Job& job0, c
         b -= as_const(bv.tile_prefix + j);
     }
     T.J = bv.n ? copy_const(const_ptr(bv.jobs + j)) : job0;
-    ConstPlan P = as_const(T.J.plan);
-    uint32_t l = 0;
-    while (l + 1 < P.L && b >= P.lv[l + 1].tile_base) ++l;
+    const TileDesc td = copy_const(const_ptr(T.J.tiles + b));
+    const uint32_t l = td.lxy & 0xFu;
     T.l = l;
+    T.tile_x = (td.lxy >> 4) & 0xFFFu;
+    T.tile_y = td.lxy >> 16;
+    T.rowmin = td.rowmin;
+    T.cs = td.cs;
+    T.nd = td.nd_rows & 0xFFFFu;
+    T.nrows = (td.nd_rows >> 16) & 0x7FFFu;
+    T.use_src = (td.nd_rows >> 31) != 0 && (T.J.pitch & 3u) == 0 &&
+                (reinterpret_cast<uintptr_t>(T.J.img) & 3u) == 0;
+    ConstPlan P = as_const(T.J.plan);
     T.G = copy_const(&P.lv[l]);  // by value: stores cannot alias it
     const LevelGeom& G = T.G;
-    const uint32_t t = b - G.tile_base;
-    T.tile_y = t / G.tiles_x;
-    T.tile_x = t - T.tile_y * G.tiles_x;
     const int32_t x0 = static_cast<int32_t>(T.tile_x) * kTileW;
     const int32_t y0 = static_cast<int32_t>(T.tile_y) * TH;
     T.x0 = x0;
@@ -287,32 +291,6 @@ __device__ inline DetectTile decode_tile(const BatchView& bv, const Job& job0, c
         T.c_last = cl_ < kLdsW - 1 ? static_cast<uint32_t>(cl_ < 0 ? 0 : cl_) : kLdsW - 1;
         T.r_last = rl_ < LH - 1 ? static_cast<uint32_t>(rl_ < 0 ? 0 : rl_) : LH - 1;
     }
-    // Level coordinate of staged column/row g (textureLoad robustness rule)
-    // and the rectangle's corners: the taps of the first and last staged
-    // column and row.
-    const auto clamp_coord = [&](int32_t g, uint32_t n) -> uint32_t {
-        const bool inside = g >= 0 && g < static_cast<int32_t>(n);
-        return prm.border_zero ? (inside ? static_cast<uint32_t>(g) : 0u) : tint_clamp(g, n);
-    };
-    const TapEntry* taps = P.taps;
-    const auto tap_i01 = [&](uint32_t idx) { return copy_const(const_ptr(taps + idx)).i01; };
-    uint32_t colmin = clamp_coord(x0 - kHalo + static_cast<int32_t>(T.c_first), w);
-    uint32_t colmax = clamp_coord(x0 - kHalo + static_cast<int32_t>(T.c_last), w);
-    uint32_t rowmin = clamp_coord(y0 - kHalo + static_cast<int32_t>(T.r_first), h);
-    uint32_t rowmax = clamp_coord(y0 - kHalo + static_cast<int32_t>(T.r_last), h);
-    if (l > 0) {
-        colmin = tap_i01(T.t_lc + colmin) & 0xFFFFu;
-        colmax = tap_i01(T.t_lc + colmax) >> 16;
-        rowmin = tap_i01(T.t_lr + rowmin) & 0xFFFFu;
-        rowmax = tap_i01(T.t_lr + rowmax) >> 16;
-    }
-    T.rowmin = rowmin;
-    T.cs = colmin & ~3u;
-    T.nd = (colmax - T.cs) / 4 + 1;  // dwords per source row
-    T.nrows = rowmax - rowmin + 1;
-    T.use_src = colmax >= colmin && rowmax >= rowmin && T.nd <= 64 &&
-                (T.J.pitch & 3u) == 0 && (reinterpret_cast<uintptr_t>(T.J.img) & 3u) == 0 &&
-                T.nd * T.nrows <= kSrcWords;
     return T;
 }
 
