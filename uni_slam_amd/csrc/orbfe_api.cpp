@@ -546,6 +546,9 @@ int layout_detect(orbfe_ctx* ctx, const orbfe_frame* frames, uint32_t n, DetectL
         Job& J = lay.jobs[i];
         J.plan = pe->dev;
         J.tiles = pe->tiles;
+        J.desc_groups = pe->host.desc_groups;
+        J.cap = pe->host.cap;
+        J.feat_stride = pe->host.feat_stride;
         J.img = f.image;
         J.pitch = f.pitch;
         J.feat = static_cast<uint8_t*>(f.features);
@@ -842,6 +845,9 @@ int orbfe_detect_async(orbfe_ctx* ctx, const uint8_t* image, size_t pitch, float
     Job job0{};
     job0.plan = ctx->main->dev;
     job0.tiles = ctx->main->tiles;
+    job0.desc_groups = P.desc_groups;
+    job0.cap = P.cap;
+    job0.feat_stride = P.feat_stride;
     job0.img = image;
     job0.pitch = pitch;
     job0.pyr = pyramid ? pyramid : ctx->pyr_scratch;

@@ -155,6 +155,9 @@ struct Job {
     uint64_t* surv;    // survivor bitmap scratch, plan->words
     uint32_t* kp;      // keypoint list scratch, L*cap (x | y << 16)
     float* smooth;     // smoothed-image scratch, plan->sm_floats
+    // copies of plan fields, so a describe block finds its level's records
+    // without waiting for the plan
+    uint32_t desc_groups, cap, feat_stride, pad_;
 };
 
 struct MatchJob {

@@ -1110,6 +1110,28 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
     __shared__ float box[KPB][kBoxN];    // per keypoint: level texels around it
     __shared__ float samp[KPB][kSlots];  // per keypoint: smoothed value of each point
     const uint32_t tid = threadIdx.x;
+    // The block's level, its survivor count and keypoint entries are loaded
+    // first (from the job alone), then the tables are filled while they
+    // arrive.
+    uint32_t b = blockIdx.x;
+    uint32_t j = 0;
+    if (bv.n) {
+        j = job_of(bv.desc_job, b);
+        b -= as_const(bv.desc_prefix + j);
+    }
+    const Job J = bv.n ? copy_const(const_ptr(bv.jobs + j)) : job0;
+    const uint32_t l = b / J.desc_groups;
+    const uint32_t g = b - l * J.desc_groups;
+    const uint32_t lane = tid & 63;
+    const uint32_t sub = lane & (LANES - 1);   // lane within the keypoint group
+    const uint32_t gbase = lane & ~(LANES - 1);  // first lane of the group
+    const uint32_t slot = g * KPB + tid / LANES;
+    const bool in_cap = slot < J.cap;
+    uint8_t* block = J.feat + static_cast<size_t>(l) * J.feat_stride;  // the level's FeatureArray
+    const uint32_t count = reinterpret_cast<const uint32_t*>(block)[0];
+    const uint32_t kp_entry = in_cap ? J.kp[l * J.cap + slot] : 0u;
+    ConstPlan P = as_const(J.plan);
+    const LevelGeom G = copy_const(&P.lv[l]);  // by value: stores below cannot alias it
     lut[tid] = static_cast<float>(tid) / 255.0f;
     {
         const uint32_t q = tid, i = 2 * (q >> 1), c = (q & 1u) * 2;
@@ -1118,25 +1140,7 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
     }
     __syncthreads();
 
-    uint32_t b = blockIdx.x;
-    uint32_t j = 0;
-    if (bv.n) {
-        j = job_of(bv.desc_job, b);
-        b -= as_const(bv.desc_prefix + j);
-    }
-    const Job J = bv.n ? copy_const(const_ptr(bv.jobs + j)) : job0;
-    ConstPlan P = as_const(J.plan);
-    const uint32_t l = b / P.desc_groups;
-    const uint32_t g = b - l * P.desc_groups;
-    const uint32_t lane = tid & 63;
-    const uint32_t sub = lane & (LANES - 1);   // lane within the keypoint group
-    const uint32_t gbase = lane & ~(LANES - 1);  // first lane of the group
-    const uint32_t slot = g * KPB + tid / LANES;
-    const LevelGeom G = copy_const(&P.lv[l]);  // by value: stores below cannot alias it
-    uint8_t* block = J.feat + G.feat_off;
-    const uint32_t count = reinterpret_cast<const uint32_t*>(block)[0];
-    const uint32_t kept = count < P.cap ? count : P.cap;
-    const bool in_cap = slot < P.cap;
+    const uint32_t kept = count < J.cap ? count : J.cap;
     const bool active = slot < kept;
     uint32_t* out = reinterpret_cast<uint32_t*>(block + 8 + 48ull * slot);
     if (__ballot(active) == 0) {  // whole wave empty: zero its slots
@@ -1147,7 +1151,7 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
         }
         return;
     }
-    const uint32_t kpv = active ? J.kp[G.kp_off + slot] : 0u;
+    const uint32_t kpv = active ? kp_entry : 0u;
     const uint32_t x = kpv & 0xFFFFu;
     const uint32_t y = kpv >> 16;
 
