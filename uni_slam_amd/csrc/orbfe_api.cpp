@@ -624,6 +624,9 @@ int layout_match(orbfe_ctx* ctx, const orbfe_match_pair* pairs, uint32_t n, Matc
         M.prev_match = p.prev_match;
         M.b_to_a = reinterpret_cast<int32_t*>(arena + off);
         M.q_list = reinterpret_cast<uint32_t*>(arena + off + pe->match_bytes);
+        M.cap = pe->host.cap;
+        M.feat_stride = pe->host.feat_stride;
+        M.levels = pe->host.L;
         off += pe->pair_bytes;
         M.index_prev = reinterpret_cast<uint32_t*>(arena + off);
         off += pe->index_bytes;
@@ -996,6 +999,9 @@ int orbfe_match_pipeline_async(orbfe_ctx* ctx, const void* prev_features,
     job0.b_to_a = ctx->b_to_a;
     job0.q_list = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(ctx->b_to_a) +
                                               ctx->main->match_bytes);
+    job0.cap = P.cap;
+    job0.feat_stride = P.feat_stride;
+    job0.levels = P.L;
     job0.index_prev = ctx->index[0];
     job0.index_curr = ctx->index[1];
     const MatchBatchView bv{nullptr, nullptr, nullptr, 0};

@@ -168,6 +168,9 @@ struct MatchJob {
     int32_t* b_to_a;      // scratch L*cap
     uint32_t* q_list;     // scratch, query_list_words(L, cap): the curr
                           // features (level << 24 | slot) with b_to_a >= 0
+    // copies of plan fields, so a matcher block reaches its feature counts
+    // and query record without waiting for the plan
+    uint32_t cap, feat_stride, levels, pad_;
     uint32_t* index_prev; // scratch: cell index of prev, plan->idx_words
     uint32_t* index_curr; // scratch: cell index of curr
 };

@@ -829,8 +829,8 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
             const bool valid = xe >= 0 && xe < static_cast<int32_t>(w);
             row0_score[k < 2 ? k : kTileW + k] = valid && streak > prm.min_run ? streak : 0u;
         }
+        __syncthreads();  // row0_score is read across waves (block-uniform branch)
     }
-    __syncthreads();
 
     if (tile_x < G.wpr) {  // extension-only tiles have no corners
 #pragma unroll
@@ -1532,24 +1532,24 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
     }
     const MatchJob M = bv.n ? copy_const(const_ptr(bv.jobs + j)) : job0;
     ConstPlan P = as_const(M.plan);
-    const FeatView Q{kMutual ? M.prev : M.curr, P.feat_stride};
-    const FeatView C{kMutual ? M.curr : M.prev, P.feat_stride};
+    const FeatView Q{kMutual ? M.prev : M.curr, M.feat_stride};
+    const FeatView C{kMutual ? M.curr : M.prev, M.feat_stride};
     const uint32_t* cidx = kMutual ? M.index_curr : M.index_prev;
     uint32_t l, slot;  // the curr feature b
     if (!kMutual) {
-        const uint32_t groups = (P.cap + QPB - 1) / QPB;  // blocks per level
+        const uint32_t groups = (M.cap + QPB - 1) / QPB;  // blocks per level
         l = b / groups;
         slot = (b - l * groups) * QPB + grp;
-        if (slot >= P.cap) return;
-        if (slot >= FeatView{M.curr, P.feat_stride}.count(l, P.cap)) {
-            if (sub == 0) M.b_to_a[l * P.cap + slot] = -1;
+        if (slot >= M.cap) return;
+        if (slot >= FeatView{M.curr, M.feat_stride}.count(l, M.cap)) {
+            if (sub == 0) M.b_to_a[l * M.cap + slot] = -1;
             return;
         }
     } else {
         const uint32_t seg = b % kListSegs;
         const uint32_t t = (b / kListSegs) * QPB + grp;
         if (t >= M.q_list[seg * kListStride]) return;  // written by the direction pass
-        const uint32_t blocks = P.L * ((P.cap + QPB - 1) / QPB);
+        const uint32_t blocks = M.levels * ((M.cap + QPB - 1) / QPB);
         const uint32_t e = M.q_list[kListSegs * kListStride +
                                     seg * query_segment_capacity(blocks, QPB) + t];
         l = e >> 24;
@@ -1558,7 +1558,7 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
     // Query: feature (ql, qs) of the query side.
     uint32_t ql = l, qs = slot;
     if (kMutual) {
-        const int32_t ab = M.b_to_a[l * P.cap + slot];
+        const int32_t ab = M.b_to_a[l * M.cap + slot];
         ql = static_cast<uint32_t>(ab) >> 24;
         qs = static_cast<uint32_t>(ab) & 0xFFFFFFu;
     }
@@ -1567,9 +1567,9 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
     const uint32_t p0 = ql;
     const uint32_t p1 = ql > 0 ? ql - 1 : ql + 1;
     const uint32_t p2 = ql + 1;
-    uint32_t npool = 1, total_pool = C.count(ql, P.cap);
-    if (ql > 0) { total_pool += C.count(ql - 1, P.cap); ++npool; }
-    if (ql + 1 < P.L) { total_pool += C.count(ql + 1, P.cap); ++npool; }
+    uint32_t npool = 1, total_pool = C.count(ql, M.cap);
+    if (ql > 0) { total_pool += C.count(ql - 1, M.cap); ++npool; }
+    if (ql + 1 < M.levels) { total_pool += C.count(ql + 1, M.cap); ++npool; }
     const auto pool_level = [&](uint32_t s) { return s == 0 ? p0 : (s == 1 ? p1 : p2); };
 
     int32_t result = -1;
@@ -1605,7 +1605,7 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
                     cidx + lv * (kCells + 1) + (cy0 + (k - sk * rows)) * kCellsPerAxis;
                 const uint32_t a = st[cx0];
                 len = st[cx1 + 1] - a;
-                gstart = lv * P.cap + a;
+                gstart = lv * M.cap + a;
             }
             const uint32_t incl = total + group_scan_dpp<LANES>(len);
             if (k < nranges) {
@@ -1689,14 +1689,14 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
         }
     }
     if (!kMutual) {
-        if (sub == 0) M.b_to_a[l * P.cap + slot] = result;
+        if (sub == 0) M.b_to_a[l * M.cap + slot] = result;
         // append b to the query list: one atomic per wave
         const uint32_t lane = tid & 63;
         const bool claim = sub == 0 && result >= 0;
         const uint64_t m = __ballot(claim);
         if (m) {
             const uint32_t seg = b % kListSegs;
-            const uint32_t blocks = P.L * ((P.cap + QPB - 1) / QPB);
+            const uint32_t blocks = M.levels * ((M.cap + QPB - 1) / QPB);
             const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(m))) - 1;
             uint32_t base = 0;
             if (lane == leader)
@@ -1707,7 +1707,7 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
                          base + __popcll(m & ((1ull << lane) - 1ull))] = (l << 24) | slot;
         }
     } else if (sub == 0 && result == static_cast<int32_t>((l << 24) | slot)) {
-        M.prev_match[ql * P.cap + qs] = result;
+        M.prev_match[ql * M.cap + qs] = result;
     }
 }
 
