@@ -364,7 +364,7 @@ int get_plan(orbfe_ctx* ctx, uint32_t W0, uint32_t H0, uint32_t cap, PlanEntry**
     P.desc_groups = ceil_div(cap, 256 / P.desc_lanes);
     P.max_dx = 0.1f * static_cast<float>(W0);
     P.max_dy = 0.1f * static_cast<float>(H0);
-    P.cell_inv_x = 2.0f / P.max_dx;
+    P.cell_inv_x = 4.0f / P.max_dx;
     P.cell_inv_y = 2.0f / P.max_dy;
     {
         const auto al4 = [](uint32_t v) { return (v + 3u) & ~3u; };  // 16-byte aligned

@@ -52,12 +52,15 @@ __host__ __device__ inline uint32_t query_segment_capacity(uint32_t blocks, uint
     return (blocks + kListSegs - 1) / kListSegs * qpb;
 }
 
-// Matcher cell index: LOD0 plane cut into kCellsPerAxis^2 cells of half the
-// search window (0.05*W0 x 0.05*H0); 24 cells cover 1.2x the frame.
-constexpr uint32_t kCellsPerAxis = 24;
-constexpr uint32_t kCells = kCellsPerAxis * kCellsPerAxis;
-// Candidate runs of one query: up to 3 pool levels x kCellsPerAxis cell rows.
-constexpr uint32_t kMaxRanges = 3 * kCellsPerAxis;
+// Matcher cell index: LOD0 plane cut into kCellsX x kCellsY cells of a
+// quarter of the search window's width and half its height (0.025*W0 x
+// 0.05*H0), covering 1.2x the frame; a query's candidates in one cell row
+// span 1.125x the window's width.
+constexpr uint32_t kCellsX = 48;
+constexpr uint32_t kCellsY = 24;
+constexpr uint32_t kCells = kCellsX * kCellsY;
+// Candidate runs of one query: up to 3 pool levels x kCellsY cell rows.
+constexpr uint32_t kMaxRanges = 3 * kCellsY;
 
 // Key packing of the pipeline matcher: distance (<= 256) above a 22-bit
 // candidate id (2 bits pool level slot, 20 bits index).
@@ -112,7 +115,7 @@ struct Plan {
     uint32_t desc_groups; // describe blocks per level = ceil(cap / (256 / desc_lanes))
     float max_dx, max_dy; // 0.1f*W0, 0.1f*H0 (match_features_cpu.cpp:76-77)
     uint32_t sm_floats;   // smoothed-image floats of one frame
-    float cell_inv_x, cell_inv_y;  // 1 / cell size (cells of max_dx/2 x max_dy/2)
+    float cell_inv_x, cell_inv_y;  // 1 / cell size (cells of max_dx/4 x max_dy/2)
     // Cell index of one side (u32 words): cell starts [L][kCells + 1], then
     // the records of every level in cell order as arrays over the global
     // record number g = level * cap + k: heads (uint4: LOD0 x and y as float

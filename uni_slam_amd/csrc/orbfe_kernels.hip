@@ -1348,10 +1348,10 @@ __device__ inline void rec_lod0(const uint32_t* r, float scale, float* x, float*
     *y = static_cast<float>(static_cast<uint32_t>(__uint_as_float(r[1]))) * scale;
 }
 
-__device__ inline uint32_t cell_of(float v, float inv) {
+// Cell coordinate of v on an axis of n cells of width 1 / inv (clamped).
+__device__ inline uint32_t cell_of(float v, float inv, uint32_t n) {
     const float c = __builtin_floorf((v > 0.0f ? v : 0.0f) * inv);
-    return c < static_cast<float>(kCellsPerAxis - 1) ? static_cast<uint32_t>(c)
-                                                     : kCellsPerAxis - 1;
+    return c < static_cast<float>(n - 1) ? static_cast<uint32_t>(c) : n - 1;
 }
 
 // k_match_index: per (pair, side, level) a counting sort of the level's
@@ -1417,7 +1417,7 @@ __global__ __launch_bounds__(kIndexThreads) void k_match_index(MatchBatchView bv
     const auto pos_of = [&](uint32_t rx, uint32_t ry, float* x, float* y) -> uint32_t {
         *x = static_cast<float>(static_cast<uint32_t>(__uint_as_float(rx))) * sc;
         *y = static_cast<float>(static_cast<uint32_t>(__uint_as_float(ry))) * sc;
-        return cell_of(*y, inv_y) * kCellsPerAxis + cell_of(*x, inv_x);
+        return cell_of(*y, inv_y, kCellsY) * kCellsX + cell_of(*x, inv_x, kCellsX);
     };
     __syncthreads();
     const auto count = [&](Held h, uint32_t i) -> Held {
@@ -1585,10 +1585,10 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
         const float dy = P.max_dy;
         // Cells of the window widened by one pixel (covers the rounding of
         // the exact gates below).
-        const uint32_t cx0 = cell_of(lo_x - 1.0f, P.cell_inv_x);
-        const uint32_t cx1 = cell_of(hi_x + 1.0f, P.cell_inv_x);
-        const uint32_t cy0 = cell_of(qy - dy - 1.0f, P.cell_inv_y);
-        const uint32_t cy1 = cell_of(qy + dy + 1.0f, P.cell_inv_y);
+        const uint32_t cx0 = cell_of(lo_x - 1.0f, P.cell_inv_x, kCellsX);
+        const uint32_t cx1 = cell_of(hi_x + 1.0f, P.cell_inv_x, kCellsX);
+        const uint32_t cy0 = cell_of(qy - dy - 1.0f, P.cell_inv_y, kCellsY);
+        const uint32_t cy1 = cell_of(qy + dy + 1.0f, P.cell_inv_y, kCellsY);
         const uint32_t rows = cy1 - cy0 + 1;
         const uint32_t nranges = npool * rows;  // <= kMaxRanges
 
@@ -1602,7 +1602,7 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
                 const uint32_t sk = k / rows;
                 const uint32_t lv = pool_level(sk);
                 const uint32_t* st =
-                    cidx + lv * (kCells + 1) + (cy0 + (k - sk * rows)) * kCellsPerAxis;
+                    cidx + lv * (kCells + 1) + (cy0 + (k - sk * rows)) * kCellsX;
                 const uint32_t a = st[cx0];
                 len = st[cx1 + 1] - a;
                 gstart = lv * M.cap + a;

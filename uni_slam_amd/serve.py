@@ -45,7 +45,7 @@ def frame_scratch_bytes(width: int, height: int, cap: int) -> int:
 
     def al4(v):
         return (v + 3) // 4 * 4
-    idx_words = al4(24 * 24 + 1) + 12 * cap
+    idx_words = al4(48 * 24 + 1) + 12 * cap
     detect = al(8 * words) + al(4 * LEVELS * cap) + al(4 * smooth)
     qlist_words = 64 * 32 + LEVELS * cap + 64 * LEVELS + 64 * 64
     match = al(4 * LEVELS * cap) + al(4 * qlist_words) + 2 * al(4 * LEVELS * idx_words)
