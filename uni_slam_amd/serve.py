@@ -119,12 +119,15 @@ class Server:
     def __init__(self, specs: Sequence[StreamSpec], device: int = 0,
                  hbm_budget: int = 1 << 30, max_batch: int = 64, inflight: int = 2,
                  fast_threshold: float = 0.15, lowe_ratio: float = 0.85,
-                 keep_outputs: bool = False, warmup: bool = True, **launch_shape):
+                 keep_outputs: bool = False, warmup: bool = True, log_batches: bool = False,
+                 **launch_shape):
         self.dev = torch.device(f"cuda:{device}")
         self.specs = list(specs)
         self.budget = hbm_budget
         self.max_batch = max_batch
         self.keep_outputs = keep_outputs
+        # with log_batches: per launched batch, its (stream, seq, submission number)
+        self.batch_log: Optional[List[List[Tuple[int, int, int]]]] = [] if log_batches else None
         self.outputs: Dict[Tuple[int, int], Tuple[np.ndarray, Optional[np.ndarray]]] = {}
         self._snap: Dict[Tuple[int, int], Tuple[torch.Tensor, Optional[torch.Tensor]]] = {}
         w0, h0 = self.specs[0].width, self.specs[0].height
@@ -286,6 +289,8 @@ class Server:
             st.last_event = ev
             st.upload_event[r.seq & 1] = ev
         self.busy[slot] = (ev, chosen, nbytes)
+        if self.batch_log is not None:
+            self.batch_log.append([(r.stream, r.seq, r.order) for r in chosen])
         self.in_flight_bytes += nbytes
         self.max_batch_seen = max(self.max_batch_seen, len(chosen))
         self.batches += 1
