@@ -1,14 +1,16 @@
 """Continuous-batching front-end server for many camera streams on one GPU.
 
 Frames arrive from many streams at their own times.  The server keeps, per
-stream, the feature arrays of the previous and current frame, a pyramid
-buffer and two upload slots resident in HBM.  Each scheduling tick takes the
-frames that have arrived, admits the oldest ones (one per stream) as far as
-the HBM budget and the in-flight slots allow, uploads them and launches one
-batched detect (all admitted frames, any mix of sizes) and one batched match
-(each frame against its stream's previous frame) on one of two HIP streams.
-A batch's uploads overlap the kernels of the batch before it; its kernels
-wait (on the device) for the batches holding its streams' previous frames.
+stream, a ring of slots resident in HBM (upload buffer, feature array,
+pyramid and match table; frame k uses slot k mod slots), so a stream's
+previous-frame descriptors stay on the device.  Each scheduling tick takes
+the frames that have arrived, admits the oldest ones (a few consecutive
+frames per stream at most, each stream in its own order) as far as the HBM
+budget and the in-flight slots allow, uploads them and launches one batched
+detect (all admitted frames, any mix of sizes) and one batched match (each
+frame against its stream's previous frame) on one of two HIP streams.  A
+batch's uploads overlap the kernels of the batch before it; its kernels wait
+(on the device) for the batches holding its streams' previous frames.
 Latency = completion - arrival.
 
 Results are bit-identical to the plain per-frame path: the batched kernels
@@ -71,18 +73,16 @@ class _Request:
 @dataclass
 class _Stream:
     spec: StreamSpec
-    feats: List[torch.Tensor]
-    pyramid: torch.Tensor
-    match: torch.Tensor
-    upload: List[torch.Tensor]
+    feats: List[torch.Tensor]     # per slot
+    pyramid: List[torch.Tensor]   # per slot
+    match: List[torch.Tensor]     # per slot: the slot's frame against the frame before
+    upload: List[torch.Tensor]    # per slot
     scratch: int
-    # prebuilt C descriptors of this stream's frame / pair for frame parity 0, 1
+    # prebuilt C descriptors of the frame / pair of each slot
     frame_c: List[orbfe.Frame] = field(default_factory=list)
     pair_c: List[orbfe.MatchPair] = field(default_factory=list)
     pending: collections.deque = field(default_factory=collections.deque)
-    last_event: Optional[torch.cuda.Event] = None       # batch of the latest admitted frame
-    upload_event: List[Optional[torch.cuda.Event]] = field(
-        default_factory=lambda: [None, None])           # batch that last read upload[p]
+    events: Dict[int, torch.cuda.Event] = field(default_factory=dict)  # seq -> its batch
 
 
 @dataclass
@@ -99,32 +99,40 @@ def _frame_struct(st: _Stream, p: int) -> orbfe.Frame:
     f.image = st.upload[p].data_ptr()
     f.pitch = sp.width
     f.width, f.height, f.cap = sp.width, sp.height, sp.cap
-    f.pyramid = st.pyramid.data_ptr()
+    f.pyramid = st.pyramid[p].data_ptr()
     f.features = st.feats[p].data_ptr()
     return f
 
 
 def _pair_struct(st: _Stream, p: int) -> orbfe.MatchPair:
-    """Pair of the frame with parity p against the stream's previous frame."""
+    """Pair of the frame in slot p against the stream's previous frame."""
     sp = st.spec
+    n = len(st.feats)
     m = orbfe.MatchPair()
-    m.prev_features = st.feats[1 - p].data_ptr()
+    m.prev_features = st.feats[(p - 1) % n].data_ptr()
     m.curr_features = st.feats[p].data_ptr()
-    m.prev_match = st.match.data_ptr()
+    m.prev_match = st.match[p].data_ptr()
     m.width, m.height, m.cap = sp.width, sp.height, sp.cap
     return m
 
 
 class Server:
+    """frames_per_stream: how many consecutive frames of one stream a batch
+    may hold (a stream with a backlog catches up in fewer, fuller batches).
+    Each stream keeps 2 * frames_per_stream + 2 slots (upload, features,
+    pyramid, match table) resident; frame k uses slot k mod slots."""
+
     def __init__(self, specs: Sequence[StreamSpec], device: int = 0,
                  hbm_budget: int = 1 << 30, max_batch: int = 64, inflight: int = 2,
                  fast_threshold: float = 0.15, lowe_ratio: float = 0.85,
                  keep_outputs: bool = False, warmup: bool = True, log_batches: bool = False,
-                 **launch_shape):
+                 frames_per_stream: int = 2, **launch_shape):
         self.dev = torch.device(f"cuda:{device}")
         self.specs = list(specs)
         self.budget = hbm_budget
         self.max_batch = max_batch
+        self.per_stream = max(1, frames_per_stream)
+        self.slots = 2 * self.per_stream + 2
         self.keep_outputs = keep_outputs
         # with log_batches: per launched batch, its (stream, seq, submission number)
         self.batch_log: Optional[List[List[Tuple[int, int, int]]]] = [] if log_batches else None
@@ -137,23 +145,26 @@ class Server:
         self.hip_streams = [torch.cuda.Stream(device=self.dev) for _ in range(inflight)]
         self.busy: List[Optional[Tuple[torch.cuda.Event, List[_Request], int]]] = [None] * inflight
         self.streams: List[_Stream] = []
+        n = self.slots
         for sp in self.specs:
             fb = orbfe.feature_array_bytes(LEVELS, sp.cap)
             npyr = sum(w * h for w, h in orbfe.level_dims(sp.width, sp.height)[1:])
             st = _Stream(
                 spec=sp,
-                feats=[torch.zeros(fb, dtype=torch.uint8, device=self.dev) for _ in range(2)],
-                pyramid=torch.zeros(npyr, dtype=torch.float32, device=self.dev),
-                match=torch.full((LEVELS * sp.cap,), -1, dtype=torch.int32, device=self.dev),
+                feats=[torch.zeros(fb, dtype=torch.uint8, device=self.dev) for _ in range(n)],
+                pyramid=[torch.zeros(npyr, dtype=torch.float32, device=self.dev)
+                         for _ in range(n)],
+                match=[torch.full((LEVELS * sp.cap,), -1, dtype=torch.int32, device=self.dev)
+                       for _ in range(n)],
                 upload=[torch.empty((sp.height, sp.width), dtype=torch.uint8, device=self.dev)
-                        for _ in range(2)],
+                        for _ in range(n)],
                 scratch=frame_scratch_bytes(sp.width, sp.height, sp.cap))
-            st.frame_c = [_frame_struct(st, p) for p in range(2)]
-            st.pair_c = [_pair_struct(st, p) for p in range(2)]
+            st.frame_c = [_frame_struct(st, p) for p in range(n)]
+            st.pair_c = [_pair_struct(st, p) for p in range(n)]
             self.streams.append(st)
         # streams with pending frames, keyed by their oldest frame's submission
-        # number: popping in key order is FIFO admission over all frames with
-        # at most one frame per stream per batch
+        # number: popping in key order is FIFO admission over all frames, a
+        # stream's frames in its own order
         self._ready: List[Tuple[int, int]] = []
         self._order = 0
         self.completed: List[Completed] = []
@@ -166,11 +177,14 @@ class Server:
     def warmup(self):
         """Creates every stream's plan in each in-flight context and grows
         the contexts' scratch arenas and pinned staging slots to the largest
-        batch admission can form (one frame of every stream), so serving
-        never allocates device or pinned memory.  Stream state is reset."""
-        n = len(self.streams)
-        frames = (orbfe.Frame * n)(*[st.frame_c[0] for st in self.streams])
-        pairs = (orbfe.MatchPair * n)(*[st.pair_c[0] for st in self.streams])
+        batch admission can form (frames_per_stream frames of every stream),
+        so serving never allocates device or pinned memory.  Stream state is
+        reset."""
+        k = self.per_stream
+        frames = [st.frame_c[p] for st in self.streams for p in range(1, k + 1)]
+        pairs = [st.pair_c[p] for st in self.streams for p in range(1, k + 1)]
+        frames = (orbfe.Frame * len(frames))(*frames)
+        pairs = (orbfe.MatchPair * len(pairs))(*pairs)
         for ctx, hs in zip(self.ctxs, self.hip_streams):
             with torch.cuda.stream(hs):
                 for _ in range(4):  # each slot of the staging ring
@@ -180,7 +194,8 @@ class Server:
         for st in self.streams:
             for f in st.feats:
                 f.zero_()
-            st.match.fill_(-1)
+            for m in st.match:
+                m.fill_(-1)
         torch.cuda.synchronize()
 
     # -- requests --------------------------------------------------------------
@@ -214,12 +229,15 @@ class Server:
                 self.busy[i] = None
 
     def _admit(self) -> Tuple[List[_Request], int]:
-        """FIFO admission: the oldest pending frames, one per stream (a
-        stream's frames stay in order), up to max_batch, within the HBM
-        budget (a frame is always admitted when nothing else is in flight)."""
+        """FIFO admission: the oldest pending frames, at most frames_per_stream
+        per stream (a stream's frames stay in order), up to max_batch, within
+        the HBM budget (a frame is always admitted when nothing else is in
+        flight)."""
         chosen: List[_Request] = []
         nbytes = 0
         ready = self._ready
+        taken: Dict[int, int] = {}
+        deferred: List[int] = []  # streams at their per-batch limit with frames left
         while ready and len(chosen) < self.max_batch:
             s = ready[0][1]
             st = self.streams[s]
@@ -229,21 +247,27 @@ class Server:
             heapq.heappop(ready)
             chosen.append(st.pending.popleft())
             nbytes += st.scratch
-        for r in chosen:  # back in line with the next frame, for later batches
-            st = self.streams[r.stream]
+            taken[s] = taken.get(s, 0) + 1
             if st.pending:
-                heapq.heappush(ready, (st.pending[0].order, r.stream))
+                if taken[s] < self.per_stream:
+                    heapq.heappush(ready, (st.pending[0].order, s))
+                else:
+                    deferred.append(s)
+        for s in deferred:  # back in line for later batches
+            heapq.heappush(ready, (self.streams[s].pending[0].order, s))
         return chosen, nbytes
 
     def launch(self) -> bool:
         """Admit pending frames into a free slot; returns True if launched.
 
-        Device order on the slot's HIP stream: each frame's upload (after
-        the batch that last read that upload buffer), then a wait for the
-        batches holding the admitted streams' previous frames (their
-        features, pyramid and match table are rewritten here), then one
-        batched detect and one batched match.  The uploads of a batch thus
-        overlap the kernels of the batch before it."""
+        Device order on the slot's HIP stream: each frame's upload (after the
+        batches that last used its stream slot: frames k - slots and
+        k - slots + 1), then a wait for the batches holding the admitted
+        frames' previous frames (their features are this batch's match
+        input), then one batched detect and one batched match.  The detect
+        stage of a batch completes before its match stage, so a stream's
+        consecutive frames can share a batch.  The uploads of a batch overlap
+        the kernels of the batch before it."""
         if not self._ready:
             return False
         try:
@@ -254,40 +278,44 @@ class Server:
         if not chosen:
             return False
         ctx, hs = self.ctxs[slot], self.hip_streams[slot]
+        n = self.slots
         frames, pairs = [], []
         waited = set()
+        in_batch = {(r.stream, r.seq) for r in chosen}
+
+        def wait(e):
+            if e is not None and id(e) not in waited:
+                hs.wait_event(e)
+                waited.add(id(e))
+
         with torch.cuda.stream(hs):
             for r in chosen:
                 st = self.streams[r.stream]
-                p = r.seq & 1
-                e = st.upload_event[p]
-                if e is not None and id(e) not in waited:
-                    hs.wait_event(e)
-                    waited.add(id(e))
+                p = r.seq % n
+                wait(st.events.get(r.seq - n))
+                wait(st.events.get(r.seq - n + 1))
                 st.upload[p].copy_(r.host, non_blocking=True)
                 frames.append(st.frame_c[p])
                 if r.seq > 0:
                     pairs.append(st.pair_c[p])
             for r in chosen:
-                e = self.streams[r.stream].last_event
-                if e is not None and id(e) not in waited:
-                    hs.wait_event(e)
-                    waited.add(id(e))
+                if (r.stream, r.seq - 1) not in in_batch:
+                    wait(self.streams[r.stream].events.get(r.seq - 1))
             ctx.detect_batch_async((orbfe.Frame * len(frames))(*frames), stream=hs)
             if pairs:
                 ctx.match_batch_async((orbfe.MatchPair * len(pairs))(*pairs), stream=hs)
             if self.keep_outputs:  # stream-ordered snapshots of the results
                 for r in chosen:
                     st = self.streams[r.stream]
+                    p = r.seq % n
                     self._snap[(r.stream, r.seq)] = (
-                        st.feats[r.seq & 1].clone(),
-                        st.match.clone() if r.seq > 0 else None)
+                        st.feats[p].clone(), st.match[p].clone() if r.seq > 0 else None)
             ev = torch.cuda.Event()
             ev.record(hs)
         for r in chosen:
             st = self.streams[r.stream]
-            st.last_event = ev
-            st.upload_event[r.seq & 1] = ev
+            st.events[r.seq] = ev
+            st.events.pop(r.seq - n, None)  # no later frame waits for it
         self.busy[slot] = (ev, chosen, nbytes)
         if self.batch_log is not None:
             self.batch_log.append([(r.stream, r.seq, r.order) for r in chosen])
