@@ -126,7 +126,7 @@ class Server:
                  hbm_budget: int = 1 << 30, max_batch: int = 64, inflight: int = 2,
                  fast_threshold: float = 0.15, lowe_ratio: float = 0.85,
                  keep_outputs: bool = False, warmup: bool = True, log_batches: bool = False,
-                 frames_per_stream: int = 2, **launch_shape):
+                 frames_per_stream: int = 3, **launch_shape):
         self.dev = torch.device(f"cuda:{device}")
         self.specs = list(specs)
         self.budget = hbm_budget
