@@ -105,7 +105,7 @@ def test_served_results_equal_plain(torch_gpu):
                 prev = feats
 
 
-@pytest.mark.parametrize("per_stream", [1, 2])
+@pytest.mark.parametrize("per_stream", [1, 2, 3])
 def test_admission_fifo_in_stream_order_within_budget(torch_gpu, per_stream):
     """Admission takes the oldest pending frames, at most per_stream per stream
     and in each stream's order, within the HBM budget; results stay the plain
