@@ -408,7 +408,8 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     const float wts[3] = {0.3125f, 0.375f, 0.3125f};
     for (uint32_t i = tid; i < 256; i += kDetectThreads) lut[i] = static_cast<float>(i) / 255.0f;
 
-    const DetectTile T = decode_tile<TH>(bv, job0, prm, blockIdx.x);
+    const DetectTile T =
+        decode_tile<TH>(bv, job0, prm, bv.n ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x);
     uint32_t pf[kPF];
     uint2 item;
     prefetch_tile<TH, kPF>(T, prm, tid, pf, item);
@@ -969,8 +970,9 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(BatchView bv, Job j
     const uint32_t lane = tid & 63;
     const uint32_t wave = tid >> 6;
     const uint32_t per_frame = levels * parts;
-    const uint32_t j = blockIdx.x / per_frame;
-    const uint32_t rem = blockIdx.x - j * per_frame;
+    const uint32_t blk = bv.n ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t j = blk / per_frame;
+    const uint32_t rem = blk - j * per_frame;
     const uint32_t l = rem / parts;
     const uint32_t part = rem - l * parts;
     const Job J = bv.n ? copy_const(const_ptr(bv.jobs + j)) : job0;
@@ -1384,9 +1386,10 @@ __global__ __launch_bounds__(kIndexThreads) void k_match_index(MatchBatchView bv
     const uint32_t tid = threadIdx.x;
     const MatchJob M = bv.n ? copy_const(const_ptr(bv.jobs)) : job0;  // layout constants only
     const uint32_t L = as_const(M.plan).L;
-    const uint32_t j = blockIdx.x / (2 * L);
-    const uint32_t side = (blockIdx.x / L) & 1;
-    const uint32_t l = blockIdx.x % L;
+    const uint32_t blk = bv.n ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t j = blk / (2 * L);
+    const uint32_t side = (blk / L) & 1;
+    const uint32_t l = blk % L;
     const MatchJob J = bv.n ? copy_const(const_ptr(bv.jobs + j)) : job0;
     ConstPlan P = as_const(J.plan);
     const FeatView A{side ? J.curr : J.prev, P.feat_stride};
