@@ -217,6 +217,7 @@ struct PlanEntry {
     uint64_t* mask = nullptr;
     orbfe::TapEntry* taps = nullptr;
     orbfe::TileDesc* tiles = nullptr;
+    orbfe::TileDesc* tiles_single = nullptr;  // dispatch order of single-frame launches
     // per-frame scratch sizes (bytes), 256-aligned
     size_t surv_bytes = 0, kp_bytes = 0, pyr_bytes = 0, feat_bytes = 0, match_bytes = 0;
     size_t sm_bytes = 0, index_bytes = 0;
@@ -429,6 +430,30 @@ int get_plan(orbfe_ctx* ctx, uint32_t W0, uint32_t H0, uint32_t cap, PlanEntry**
     if (rc) return rc;
     HIP_TRY(ctx, hipMemcpy(pe->tiles, td.data(), td.size() * sizeof(orbfe::TileDesc),
                            hipMemcpyHostToDevice));
+    // Single-frame launches: dispatch index i takes tile S[xcd_block(i, n)],
+    // S = the tiles of band 0 of every level, then band 1, ... (band x: tile
+    // rows [x * ty / 8, (x + 1) * ty / 8) of a level), so XCD x works on
+    // about band x of each level (balanced across levels, one L2 per
+    // region).
+    {
+        std::vector<uint32_t> order;
+        order.reserve(tiles);
+        for (uint32_t x = 0; x < 8; ++x)
+            for (uint32_t l = 0; l < L; ++l) {
+                const LevelGeom& G = P.lv[l];
+                for (uint32_t ty = x * G.tiles_y / 8; ty < (x + 1) * G.tiles_y / 8; ++ty)
+                    for (uint32_t tx = 0; tx < G.tiles_x; ++tx)
+                        order.push_back(G.tile_base + ty * G.tiles_x + tx);
+            }
+        if (order.size() != tiles) return fail(ctx, ORBFE_EHIP, "tile order covers %zu of %u tiles",
+                                               order.size(), tiles);
+        std::vector<orbfe::TileDesc> ts(td.size());
+        for (uint32_t i = 0; i < tiles; ++i) ts[i] = td[order[orbfe::xcd_block(i, tiles)]];
+        rc = dalloc(ctx, &pe->tiles_single, ts.size() * sizeof(orbfe::TileDesc));
+        if (rc) return rc;
+        HIP_TRY(ctx, hipMemcpy(pe->tiles_single, ts.data(), ts.size() * sizeof(orbfe::TileDesc),
+                               hipMemcpyHostToDevice));
+    }
     rc = dalloc(ctx, &pe->dev, sizeof(Plan));
     if (rc) return rc;
     HIP_TRY(ctx, hipMemcpy(pe->dev, &P, sizeof(Plan), hipMemcpyHostToDevice));
@@ -737,6 +762,7 @@ void orbfe_destroy(orbfe_ctx* ctx) {
         dfree(kv.second->mask);
         dfree(kv.second->taps);
         dfree(kv.second->tiles);
+        dfree(kv.second->tiles_single);
         dfree(kv.second->dev);
     }
     dfree(ctx->surv);
@@ -847,7 +873,7 @@ int orbfe_detect_async(orbfe_ctx* ctx, const uint8_t* image, size_t pitch, float
     if (pitch < P.W0) return fail(ctx, ORBFE_EINVAL, "pitch %zu < width %u", pitch, P.W0);
     Job job0{};
     job0.plan = ctx->main->dev;
-    job0.tiles = ctx->main->tiles;
+    job0.tiles = ctx->main->tiles_single;
     job0.desc_groups = P.desc_groups;
     job0.cap = P.cap;
     job0.feat_stride = P.feat_stride;

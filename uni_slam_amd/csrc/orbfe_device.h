@@ -128,6 +128,16 @@ struct Plan {
     LevelGeom lv[kMaxLevels];
 };
 
+// Logical block of dispatch index i in a grid of n: workgroups are dealt to
+// the 8 XCDs round-robin (i % 8), so XCD x is given the contiguous logical
+// range of its share; the blocks of one frame then share an L2.
+__host__ __device__ inline uint32_t xcd_block(uint32_t i, uint32_t n) {
+    constexpr uint32_t kXcds = 8;
+    const uint32_t q = n / kXcds, r = n % kXcds;
+    const uint32_t x = i % kXcds, k = i / kXcds;
+    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
 // One detect tile of a plan, filled on the host with the tile decode the
 // kernel would otherwise repeat: level | tile_x << 4 | tile_y << 16, then the
 // source rectangle of level 0 its staged texels sample (first dword column

@@ -144,16 +144,6 @@ __device__ inline T copy_const(const ORBFE_CONST T* p) {
 
 using ConstPlan = const ORBFE_CONST Plan&;
 
-// Logical block of dispatch index i in a grid of n: workgroups are dealt to
-// the 8 XCDs round-robin (i % 8), so XCD x is given the contiguous logical
-// range of its share; the blocks of one frame then share an L2.
-__device__ inline uint32_t xcd_block(uint32_t i, uint32_t n) {
-    constexpr uint32_t kXcds = 8;
-    const uint32_t q = n / kXcds, r = n % kXcds;
-    const uint32_t x = i % kXcds, k = i / kXcds;
-    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
-}
-
 // Job of block b from the batch's u16 block-job table (one scalar load).
 __device__ inline uint32_t job_of(const uint16_t* table, uint32_t b) {
     const uint32_t w = as_const(reinterpret_cast<const uint32_t*>(table) + (b >> 1));
