@@ -761,25 +761,32 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     uint16_t* wlist = reinterpret_cast<uint16_t*>(reinterpret_cast<uint8_t*>(stage) +
                                                   kDetectThreads * RPT) + wave * RPT * 64;
     const bool quick = prm.min_run >= 9;
+    // The texels and mask words of all RPT rows are read first (every index
+    // lies in the staged tile), then each row's test is formed without
+    // branches.
+    float cen[RPT], q7[RPT], q8[RPT];
+    uint64_t mrow[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const uint32_t r = wave * RPT + k;
+        cen[k] = tile[r + kHalo][lane + kHalo];
+        q7[k] = tile[r + kHalo + c_circle_y[7]][lane + kHalo + c_circle_x[7]];
+        q8[k] = tile[r + kHalo + c_circle_y[8]][lane + kHalo + c_circle_x[8]];
+        mrow[k] = mask_lds[r];
+    }
     uint32_t nlist = 0;
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
         const uint32_t r = wave * RPT + k;
         const uint32_t y = static_cast<uint32_t>(y0) + r;
         wscore[k * 64 + lane] = 0;
-        bool need = false;
-        if (x < w && y < h) {
-            const float center = tile[r + kHalo][lane + kHalo];
-            if (l > 0 && J.pyr) J.pyr[G.pyr_off + y * w + x] = center;
-            if (tile_x < G.wpr)
-                need = y == 0 ? (x % 9) < 3 : ((mask_lds[r] >> lane) & 1ull) != 0;
-            if (need && quick) {
-                const float p7 = tile[r + kHalo + c_circle_y[7]][lane + kHalo + c_circle_x[7]];
-                const float p8 = tile[r + kHalo + c_circle_y[8]][lane + kHalo + c_circle_x[8]];
-                need = __builtin_fabsf(center - p7) > prm.threshold &&
-                       __builtin_fabsf(center - p8) > prm.threshold;
-            }
-        }
+        const bool inb = x < w && y < h;
+        if (inb && l > 0 && J.pyr) J.pyr[G.pyr_off + y * w + x] = cen[k];
+        bool need = inb && tile_x < G.wpr &&
+                    (y == 0 ? (x % 9) < 3 : ((mrow[k] >> lane) & 1ull) != 0);
+        if (quick)
+            need = need & (__builtin_fabsf(cen[k] - q7[k]) > prm.threshold) &
+                   (__builtin_fabsf(cen[k] - q8[k]) > prm.threshold);
         const uint64_t bits = __ballot(need);
         if (need) {
             const uint32_t pos = nlist + __popcll(bits & ((1ull << lane) - 1ull));
