@@ -168,15 +168,35 @@ def test_match_bf_against_oracle(torch_gpu, nq, nt):
     assert np.array_equal(idx, ri) and np.array_equal(d1, r1) and np.array_equal(d2, r2)
 
 
-def test_match_bf_ties_pick_lowest_index(torch_gpu):
+@pytest.mark.parametrize("nt,dups", [(600, [17, 333, 590]),
+                                     (20000, [31, 32, 8191, 8192, 19999])])
+def test_match_bf_ties_pick_lowest_index(torch_gpu, nt, dups):
+    # duplicates of train 5 in other 32-row tiles and (nt = 20000) in other
+    # train splits of at most 8192 rows
     rng = np.random.default_rng(4)
-    t = rng.integers(0, 2**32, (600, 8), dtype=np.uint64).astype(np.uint32)
-    t[[17, 333, 590]] = t[5]  # duplicates of train 5
-    q = t[[5, 17, 333]].copy()
+    t = rng.integers(0, 2**32, (nt, 8), dtype=np.uint64).astype(np.uint32)
+    t[dups] = t[5]
+    q = t[[5] + dups].copy()
     with orbfe.Context(64, 48) as ctx:
-        _say("match_bf ties")
+        _say(f"match_bf ties {nt}")
         idx, d1, d2 = _bf_gpu(torch_gpu, ctx, q, t)
-    assert list(idx) == [5, 5, 5] and list(d1) == [0, 0, 0] and list(d2) == [0, 0, 0]
+    k = len(dups) + 1
+    assert list(idx) == [5] * k and list(d1) == [0] * k and list(d2) == [0] * k
+
+
+def test_match_bf_extreme_distances(torch_gpu):
+    # distances 0 and 256 (all bits differ); one train row only for a query
+    # of all ones besides its complement
+    ones = np.full(8, 0xFFFFFFFF, np.uint32)
+    zeros = np.zeros(8, np.uint32)
+    t = np.stack([ones, ones, zeros, ones])
+    q = np.stack([zeros, ones])
+    with orbfe.Context(64, 48) as ctx:
+        _say("match_bf extremes")
+        idx, d1, d2 = _bf_gpu(torch_gpu, ctx, q, t)
+        i1, e1, e2 = _bf_gpu(torch_gpu, ctx, q[:1], t[:1])
+    assert list(idx) == [2, 0] and list(d1) == [0, 0] and list(d2) == [256, 0]
+    assert list(i1) == [0] and list(e1) == [256] and list(e2) == [0xFFFFFFFF]
 
 
 @pytest.mark.parametrize("n", [16384, 65536])

@@ -29,12 +29,14 @@ hipError_t launch_match_index(uint32_t blocks, MatchBatchView bv, const MatchJob
                               hipStream_t s);
 hipError_t launch_match_dir(uint32_t lanes, bool mutual, uint32_t blocks, MatchBatchView bv,
                             const MatchJob& job0, float ratio, hipStream_t s);
-uint32_t bf_queries_per_block();
+uint32_t bf_mfma_queries_per_block();
+uint32_t bf_mfma_window();
+hipError_t launch_bf_mfma(const uint32_t* q, uint32_t nq, const uint32_t* t, uint32_t nt,
+                          uint32_t splits, uint32_t split_len, uint2* part, hipStream_t s);
+hipError_t launch_bf_merge(const uint2* part, uint32_t nq, uint32_t splits, uint32_t* idx,
+                           uint32_t* d1, uint32_t* d2, hipStream_t s);
 hipError_t launch_valu_probe(uint32_t* sink, uint32_t blocks, hipStream_t s,
                              double* instr_per_launch);
-hipError_t launch_bf(const uint32_t* q, uint32_t nq, const uint32_t* t, uint32_t nt,
-                     uint32_t splits, uint32_t split_len, uint2* part, uint32_t* idx,
-                     uint32_t* d1, uint32_t* d2, hipStream_t s, int phase);
 }  // namespace orbfe
 
 using orbfe::BatchView;
@@ -1137,16 +1139,19 @@ int orbfe_match_bf_async(orbfe_ctx* ctx, const uint32_t* query, uint32_t nq,
         return fail(ctx, ORBFE_EINVAL, "null pointer");
     if (nt >= (1u << orbfe::kBfKeyShift))
         return fail(ctx, ORBFE_ERANGE, "train count %u >= 2^%u", nt, orbfe::kBfKeyShift);
-    // Enough workgroups to cover the 256 CUs several times over: split the
-    // train set when the query blocks alone are too few.
-    const uint32_t qblocks = ceil_div(nq, orbfe::bf_queries_per_block());
+    // The train set is cut into splits of at most 8192 rows (the index
+    // field of the matrix-core keys), and into more when the query blocks
+    // alone are too few to cover the 256 CUs several times over; no split
+    // is empty.
+    const uint32_t qblocks = ceil_div(nq, orbfe::bf_mfma_queries_per_block());
     uint32_t splits = 1;
     if (nt > 0) {
         const uint32_t want = std::max<uint32_t>(1, 2048 / qblocks);
         const uint32_t max_splits = std::max<uint32_t>(1, nt / 256);
-        splits = std::min(want, max_splits);
+        splits = std::max(ceil_div(nt, orbfe::bf_mfma_window()), std::min(want, max_splits));
     }
     const uint32_t split_len = nt > 0 ? ceil_div(nt, splits) : 0;
+    if (nt > 0) splits = ceil_div(nt, split_len);
     const size_t need = static_cast<size_t>(splits) * nq * sizeof(uint2);
     if (need > ctx->bf_part_bytes) {
         HIP_TRY(ctx, hipStreamSynchronize(stream));
@@ -1158,13 +1163,12 @@ int orbfe_match_bf_async(orbfe_ctx* ctx, const uint32_t* query, uint32_t nq,
         }
         ctx->bf_part_bytes = need;
     }
-    HIP_TRY(ctx, timed(ctx, "k_bf_partial", stream, [&] {
-                return orbfe::launch_bf(query, nq, train, nt, splits, split_len, ctx->bf_part,
-                                        idx, d1, d2, stream, 0);
+    HIP_TRY(ctx, timed(ctx, "k_bf_mfma", stream, [&] {
+                return orbfe::launch_bf_mfma(query, nq, train, nt, splits, split_len,
+                                             ctx->bf_part, stream);
             }));
     HIP_TRY(ctx, timed(ctx, "k_bf_merge", stream, [&] {
-                return orbfe::launch_bf(query, nq, train, nt, splits, split_len, ctx->bf_part,
-                                        idx, d1, d2, stream, 1);
+                return orbfe::launch_bf_merge(ctx->bf_part, nq, splits, idx, d1, d2, stream);
             }));
     return ORBFE_OK;
 }

@@ -14,8 +14,10 @@
 //   k_match_dir pipeline matcher, one wave per query: LOD+-1 pool, candidates
 //               from the cells under the search window, XOR/popcount, best/
 //               second-best keys reduced across the wave.
-//   k_bf_partial / k_bf_merge  brute-force best/second-best Hamming search,
-//               queries in registers, train tiles in LDS.
+//   k_bf_mfma / k_bf_merge  brute-force best/second-best Hamming search:
+//               distances as int8 sign-vector dot products on the matrix
+//               cores (v_mfma_i32_32x32x32_i8), best two keys per train
+//               split, merged across splits.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1729,59 +1731,8 @@ template __global__ void k_match_dir<64, false>(MatchBatchView, MatchJob, float)
 template __global__ void k_match_dir<64, true>(MatchBatchView, MatchJob, float);
 
 // ---------------------------------------------------------------------------
-// Brute-force matcher.
+// Brute-force matcher: merge of the per-split best/second-best keys.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kBfThreads = 256;
-constexpr uint32_t kBfQPT = 4;          // queries per thread
-constexpr uint32_t kBfTile = 256;       // train descriptors per LDS tile
-
-__global__ __launch_bounds__(kBfThreads) void k_bf_partial(const uint32_t* __restrict__ query,
-                                                            uint32_t nq,
-                                                            const uint32_t* __restrict__ train,
-                                                            uint32_t nt, uint32_t split_len,
-                                                            uint2* __restrict__ part) {
-    __shared__ uint4 tr[kBfTile][2];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t split = blockIdx.y;
-    const uint32_t t_begin = split * split_len;
-    const uint32_t t_end = umin(t_begin + split_len, nt);
-    uint32_t q[kBfQPT][8];
-    uint32_t min1[kBfQPT], min2[kBfQPT];
-#pragma unroll
-    for (uint32_t k = 0; k < kBfQPT; ++k) {
-        const uint32_t qi = (blockIdx.x * kBfQPT + k) * kBfThreads + tid;
-        const uint4* src = reinterpret_cast<const uint4*>(query + 8ull * (qi < nq ? qi : 0));
-        const uint4 a = src[0], bq = src[1];
-        q[k][0] = a.x; q[k][1] = a.y; q[k][2] = a.z; q[k][3] = a.w;
-        q[k][4] = bq.x; q[k][5] = bq.y; q[k][6] = bq.z; q[k][7] = bq.w;
-        min1[k] = 0xFFFFFFFFu;
-        min2[k] = 0xFFFFFFFFu;
-    }
-    for (uint32_t base = t_begin; base < t_end; base += kBfTile) {
-        const uint32_t n = umin(kBfTile, t_end - base);
-        __syncthreads();
-        for (uint32_t e = tid; e < 2 * n; e += kBfThreads) {
-            const uint4* src = reinterpret_cast<const uint4*>(train + 8ull * (base + (e >> 1)));
-            tr[e >> 1][e & 1] = src[e & 1];
-        }
-        __syncthreads();
-        for (uint32_t t = 0; t < n; ++t) {
-            const uint4 a = tr[t][0], c = tr[t][1];
-            const uint32_t jt = base + t;
-#pragma unroll
-            for (uint32_t k = 0; k < kBfQPT; ++k) {
-                const uint32_t key = (hamming256(q[k], a, c) << kBfKeyShift) | jt;
-                keep_two_smallest(min1[k], min2[k], key);
-            }
-        }
-    }
-#pragma unroll
-    for (uint32_t k = 0; k < kBfQPT; ++k) {
-        const uint32_t qi = (blockIdx.x * kBfQPT + k) * kBfThreads + tid;
-        if (qi < nq) part[static_cast<size_t>(split) * nq + qi] = make_uint2(min1[k], min2[k]);
-    }
-}
-
 __global__ __launch_bounds__(256) void k_bf_merge(const uint2* __restrict__ part, uint32_t nq,
                                                    uint32_t splits, uint32_t* __restrict__ idx,
                                                    uint32_t* __restrict__ d1,
@@ -1801,6 +1752,184 @@ __global__ __launch_bounds__(256) void k_bf_merge(const uint2* __restrict__ part
     d1[qi] = m1 == 0xFFFFFFFFu ? 0xFFFFFFFFu : (m1 >> kBfKeyShift);
     d2[qi] = m2 == 0xFFFFFFFFu ? 0xFFFFFFFFu : (m2 >> kBfKeyShift);
 }
+
+// ---------------------------------------------------------------------------
+// Brute-force matcher on the matrix cores.  A 256-bit descriptor is read as
+// 256 signs, one int8 per bit: train descriptors as +64 for a 0 bit and -64
+// for a 1 bit, queries the other way round.  The dot product of the two
+// vectors is then 4096 * (2d - 256) = 8192 d - 2^20 for Hamming distance d,
+// exact in int32.  v_mfma_i32_32x32x32_i8 forms it for 32 train rows x 32
+// queries, 32 bits per step (8 steps per descriptor); its accumulator input
+// carries each train row's index t in the split (t < 8192), so every output
+// is the key 8192 d + t - 2^20, which orders as (d, t): per query, the two
+// smallest keys are the best and second-best train descriptors, ties to the
+// lower index.  Rows past the split end
+// get a bias of 2^30 and never rank below a real row.
+//
+// Accumulator layout of the 32x32 products (lane l, register r): query
+// column l % 32, train row 8 (r / 4) + 4 (l / 32) + r % 4.  Operand lane l
+// holds train row (A) or query (B) l % 32 and 16 bits of the step's 32-bit
+// word (bm_signs16); both operands use the same bit order, so which bit
+// lands in which byte does not change the dot product.
+// ---------------------------------------------------------------------------
+typedef int32_t bm_i32x4 __attribute__((ext_vector_type(4)));
+typedef int32_t bm_i32x16 __attribute__((ext_vector_type(16)));
+
+constexpr uint32_t kBmThreads = 256;  // 4 waves, each its own queries
+constexpr uint32_t kBmWindow = 8192;  // train rows per split: 13-bit index in the key
+constexpr int32_t kBmBias = 1 << 20;
+constexpr int32_t kBmPastEnd = 1 << 30;
+
+// 16 bits of a 32-bit word to 16 int8 signs, +64 (0x40) for a 0 bit and
+// -64 (0xC0) for a 1 bit.  `v` is the word shifted left by 4 (l / 32): lane
+// half 0 takes bits 4..7 of every byte, half 1 bits 0..3; int32 j of the
+// result holds bit 7 - j of each byte of `v`, moved to bit 7 of the byte.
+__device__ inline bm_i32x4 bm_signs16(uint32_t v) {
+    bm_i32x4 r;
+    r.x = static_cast<int32_t>((v & 0x80808080u) | 0x40404040u);
+    r.y = static_cast<int32_t>(((v << 1) & 0x80808080u) | 0x40404040u);
+    r.z = static_cast<int32_t>(((v << 2) & 0x80808080u) | 0x40404040u);
+    r.w = static_cast<int32_t>(((v << 3) & 0x80808080u) | 0x40404040u);
+    return r;
+}
+
+// Middle value of three (v_med3_i32).  `after` is not read: it orders the
+// instruction after the code that produces it (below, a compiler-generated
+// read of the same matrix-core result, which carries the wait states the
+// result needs before VALU may read it).
+__device__ inline int32_t med3_i32_after(int32_t a, int32_t b, int32_t c, int32_t after) {
+    int32_t r;
+    asm("v_med3_i32 %0, %1, %2, %3 ; after %4" : "=v"(r) : "v"(a), "v"(b), "v"(c), "v"(after));
+    return r;
+}
+
+// Key relative to the tile at `tb` (8192 d + t - tb - 2^20) to the key the
+// merge compares across splits (d << kBfKeyShift | t); kept keys of rows
+// past the end (or none) become 0xFFFFFFFF.
+__device__ inline uint32_t bm_global_key(int32_t k, uint32_t tb, uint32_t t0) {
+    if (k >= kBmPastEnd - kBmBias - static_cast<int32_t>(kBmWindow)) return 0xFFFFFFFFu;
+    const uint32_t u = static_cast<uint32_t>(k + kBmBias) + (tb - t0);  // 8192 d + t - t0
+    return ((u >> 13) << kBfKeyShift) | (t0 + (u & (kBmWindow - 1)));
+}
+
+template <uint32_t Q>
+struct BmQueries {
+    bm_i32x4 b[Q][8];   // sign operands of the wave's 32 Q queries
+    int32_t m1[Q], m2[Q];  // the two smallest keys per query (this lane's rows)
+};
+
+// One tile of 32 train rows (this lane's row words lo/hi) against the
+// wave's queries; `cin` is the accumulator input (row offsets in the tile,
+// plus kBmPastEnd for rows past the split end).
+template <uint32_t Q>
+__device__ __forceinline__ void bm_tile(BmQueries<Q>& st, const uint4& lo, const uint4& hi,
+                                        uint32_t half, const bm_i32x16& cin) {
+    const uint32_t w[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+    bm_i32x16 acc[Q];
+#pragma unroll
+    for (uint32_t s = 0; s < 8; ++s) {
+        const bm_i32x4 a = bm_signs16(w[s] << (4 * half));
+#pragma unroll
+        for (uint32_t g = 0; g < Q; ++g)
+            acc[g] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, st.b[g][s], s ? acc[g] : cin, 0, 0,
+                                                           0);
+    }
+#pragma unroll
+    for (uint32_t g = 0; g < Q; ++g) {
+#pragma unroll
+        for (uint32_t r = 0; r < 16; ++r) {
+            const int32_t k = acc[g][r];
+            const int32_t n1 = st.m1[g] < k ? st.m1[g] : k;  // first read of k
+            st.m2[g] = med3_i32_after(st.m1[g], st.m2[g], k, n1);
+            st.m1[g] = n1;
+        }
+    }
+}
+
+// Workgroup b: split b % splits (consecutive workgroups go to different
+// XCDs, so each XCD's L2 holds the rows of few splits), query block
+// b / splits of 4 waves x 32 Q queries.
+template <uint32_t Q>
+__global__ __launch_bounds__(kBmThreads) void k_bf_mfma(const uint32_t* __restrict__ query,
+                                                         uint32_t nq,
+                                                         const uint32_t* __restrict__ train,
+                                                         uint32_t nt, uint32_t split_len,
+                                                         uint32_t splits,
+                                                         uint2* __restrict__ part) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t col = lane & 31;
+    const uint32_t half = lane >> 5;
+    const uint32_t split = blockIdx.x % splits;
+    const uint32_t qb = blockIdx.x / splits;
+    const uint32_t t0 = split * split_len;
+    const uint32_t t1 = umin(t0 + split_len, nt);
+    const uint32_t q0 = (qb * (kBmThreads / 64) + wave) * 32 * Q;
+    if (q0 >= nq) return;  // wave-uniform; no barriers below
+
+    BmQueries<Q> st;
+#pragma unroll
+    for (uint32_t g = 0; g < Q; ++g) {
+        const uint32_t qi = umin(q0 + 32 * g + col, nq - 1);
+        const uint4* src = reinterpret_cast<const uint4*>(query + 8ull * qi);
+        const uint4 lo = src[0], hi = src[1];
+        const uint32_t w[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+        for (uint32_t s = 0; s < 8; ++s) st.b[g][s] = bm_signs16(~w[s] << (4 * half));
+        st.m1[g] = st.m2[g] = 0x7FFFFFFF;
+    }
+
+    // The running keys are kept relative to the current tile: the
+    // accumulator input is the row's offset in the tile, and moving to the
+    // next tile lowers the kept keys by 32 (a uniform shift keeps their
+    // order; at most 256 tiles, so the initial maximum cannot wrap).  Rows
+    // past the split end (last tile only) carry kBmPastEnd.
+    uint32_t tb = t0;
+    if (t0 < t1) {
+        bm_i32x16 crow, ctail;
+        const uint32_t rem = (t1 - t0) % 32;
+#pragma unroll
+        for (uint32_t r = 0; r < 16; ++r) {
+            const uint32_t o = 8 * (r >> 2) + 4 * half + (r & 3);
+            crow[r] = static_cast<int32_t>(o);  // + (8192 d - 2^20) from the products
+            ctail[r] = crow[r] + (rem != 0 && o >= rem ? kBmPastEnd : 0);
+        }
+        const uint4* tr = reinterpret_cast<const uint4*>(train);
+        uint32_t ti = umin(t0 + col, t1 - 1);
+        uint4 lo = tr[2ull * ti], hi = tr[2ull * ti + 1];
+        for (; tb + 32 < t1; tb += 32) {
+            const uint4 clo = lo, chi = hi;
+            ti = umin(tb + 32 + col, t1 - 1);  // next tile (clamped: the last row again)
+            lo = tr[2ull * ti];
+            hi = tr[2ull * ti + 1];
+            bm_tile<Q>(st, clo, chi, half, crow);
+#pragma unroll
+            for (uint32_t g = 0; g < Q; ++g) {
+                st.m1[g] -= 32;
+                st.m2[g] -= 32;
+            }
+        }
+        bm_tile<Q>(st, lo, hi, half, ctail);
+    }
+#pragma unroll
+    for (uint32_t g = 0; g < Q; ++g) {
+        const int32_t o1 = __shfl_xor(st.m1[g], 32);
+        const int32_t o2 = __shfl_xor(st.m2[g], 32);
+        const int32_t n1 = st.m1[g] < o1 ? st.m1[g] : o1;
+        const int32_t hi1 = st.m1[g] < o1 ? o1 : st.m1[g];
+        const int32_t lo2 = st.m2[g] < o2 ? st.m2[g] : o2;
+        const int32_t n2 = hi1 < lo2 ? hi1 : lo2;
+        const uint32_t qi = q0 + 32 * g + col;
+        if (half == 0 && qi < nq)
+            part[static_cast<size_t>(split) * nq + qi] =
+                make_uint2(bm_global_key(n1, tb, t0), bm_global_key(n2, tb, t0));
+    }
+}
+
+template __global__ void k_bf_mfma<2>(const uint32_t*, uint32_t, const uint32_t*, uint32_t,
+                                      uint32_t, uint32_t, uint2*);
+template __global__ void k_bf_mfma<4>(const uint32_t*, uint32_t, const uint32_t*, uint32_t,
+                                      uint32_t, uint32_t, uint2*);
 
 // ---------------------------------------------------------------------------
 // Host-side launchers (called from orbfe_api.cpp).
@@ -1934,19 +2063,22 @@ hipError_t launch_valu_probe(uint32_t* sink, uint32_t blocks, hipStream_t s,
     return hipGetLastError();
 }
 
-uint32_t bf_queries_per_block() { return kBfQPT * kBfThreads; }
+constexpr uint32_t kBmGroups = 4;  // 32-query groups per wave
+uint32_t bf_mfma_queries_per_block() { return kBmThreads / 64 * 32 * kBmGroups; }
+uint32_t bf_mfma_window() { return kBmWindow; }
 
-hipError_t launch_bf(const uint32_t* q, uint32_t nq, const uint32_t* t, uint32_t nt,
-                     uint32_t splits, uint32_t split_len, uint2* part, uint32_t* idx,
-                     uint32_t* d1, uint32_t* d2, hipStream_t s, int phase) {
-    if (phase == 0) {
-        const uint32_t qb = (nq + kBfQPT * kBfThreads - 1) / (kBfQPT * kBfThreads);
-        hipLaunchKernelGGL(k_bf_partial, dim3(qb, splits), dim3(kBfThreads), 0, s, q, nq, t, nt,
-                           split_len, part);
-    } else {
-        hipLaunchKernelGGL(k_bf_merge, dim3((nq + 255) / 256), dim3(256), 0, s, part, nq, splits,
-                           idx, d1, d2);
-    }
+hipError_t launch_bf_mfma(const uint32_t* q, uint32_t nq, const uint32_t* t, uint32_t nt,
+                          uint32_t splits, uint32_t split_len, uint2* part, hipStream_t s) {
+    const uint32_t qb = (nq + bf_mfma_queries_per_block() - 1) / bf_mfma_queries_per_block();
+    hipLaunchKernelGGL(k_bf_mfma<kBmGroups>, dim3(qb * splits), dim3(kBmThreads), 0, s, q, nq, t,
+                       nt, split_len, splits, part);
+    return hipGetLastError();
+}
+
+hipError_t launch_bf_merge(const uint2* part, uint32_t nq, uint32_t splits, uint32_t* idx,
+                           uint32_t* d1, uint32_t* d2, hipStream_t s) {
+    hipLaunchKernelGGL(k_bf_merge, dim3((nq + 255) / 256), dim3(256), 0, s, part, nq, splits, idx,
+                       d1, d2);
     return hipGetLastError();
 }
 
