@@ -242,6 +242,7 @@ def phase_matcher(torch, orbfe, synth, n, reps, int_peak_tops):
     d2 = torch.empty_like(idx)
     ctx = orbfe.Context(64, 48)
     stream = torch.cuda.current_stream()
+    mfma_probe = ctx.probe_mfma_rate(stream=stream) / 1e15
     dt = timed(torch, lambda: ctx.match_bf_async(dq, n, dt_, n, idx, d1, d2, stream=stream), reps)
     ctx.timing(True)
     ctx.timing_reset()
@@ -261,6 +262,10 @@ def phase_matcher(torch, orbfe, synth, n, reps, int_peak_tops):
             "mfma_int8_Pops": round(mfma_pops, 3),
             "mfma_int8_peak_Pops": PEAK_INT8_MFMA_POPS,
             "mfma_frac": round(mfma_pops / PEAK_INT8_MFMA_POPS, 4),
+            "mfma_int8_probe_Pops": round(mfma_probe, 3),
+            "mfma_frac_of_probe": round(mfma_pops / mfma_probe, 4),
+            "mfma_probe_source": "orbfe_probe_mfma_rate (2 independent v_mfma_i32_32x32x32_i8 "
+                                 "chains per wave, 8 waves per SIMD, every CU)",
             "xor_popcount_equiv_Tops": round(16 * pairs / kern / 1e12, 2),
             "int_peak_Tops": round(int_peak_tops, 2),
             "k_bf_mfma_us": round(kern * 1e6, 2)}

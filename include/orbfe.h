@@ -232,6 +232,14 @@ int orbfe_timing_reset(struct orbfe_ctx* ctx);
 int orbfe_probe_valu_rate(struct orbfe_ctx* ctx, hipStream_t stream,
                           double* wave_instructions_per_s);
 
+/* Sustained int8 matrix-core rate of the context's device, in int8
+ * operations (2 per multiply-add) per second: a probe kernel of independent
+ * v_mfma_i32_32x32x32_i8 chains (the instruction of the brute-force
+ * matcher) on every CU, timed with HIP events on `stream` (synchronizes).
+ * No counterpart in the reference: the measured ceiling beside the published
+ * 5.0 POP/s that the brute-force matcher is quoted against. */
+int orbfe_probe_mfma_rate(struct orbfe_ctx* ctx, hipStream_t stream, double* int8_ops_per_s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,8 @@ hipError_t launch_bf_merge(const uint2* part, uint32_t nq, uint32_t splits, uint
                            uint32_t* d1, uint32_t* d2, hipStream_t s);
 hipError_t launch_valu_probe(uint32_t* sink, uint32_t blocks, hipStream_t s,
                              double* instr_per_launch);
+hipError_t launch_mfma_probe(uint32_t* sink, uint32_t blocks, hipStream_t s,
+                             double* ops_per_launch);
 }  // namespace orbfe
 
 using orbfe::BatchView;
@@ -1173,12 +1175,17 @@ int orbfe_match_bf_async(orbfe_ctx* ctx, const uint32_t* query, uint32_t nq,
     return ORBFE_OK;
 }
 
-int orbfe_probe_valu_rate(orbfe_ctx* ctx, hipStream_t stream, double* rate) {
+// Times 5 launches of a probe kernel (after one warm-up) with HIP events on
+// `stream`; *rate = work per launch / seconds per launch.
+static int probe_rate(orbfe_ctx* ctx, hipStream_t stream, uint32_t blocks_per_cu,
+                      const char* what,
+                      hipError_t (*launch)(uint32_t*, uint32_t, hipStream_t, double*),
+                      double* rate) {
     if (!ctx || !rate) return ORBFE_EINVAL;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     int cus = 0;
     HIP_TRY(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-    const uint32_t blocks = static_cast<uint32_t>(cus) * 32;  // 8 workgroups per CU, 4 rounds
+    const uint32_t blocks = static_cast<uint32_t>(cus) * blocks_per_cu;
     uint32_t* sink = nullptr;
     int rc = dalloc(ctx, &sink, 4ull * blocks);
     if (rc) return rc;
@@ -1186,9 +1193,9 @@ int orbfe_probe_valu_rate(orbfe_ctx* ctx, hipStream_t stream, double* rate) {
     double per = 0.0;
     hipError_t e = hipEventCreate(&ev[0]);
     if (e == hipSuccess) e = hipEventCreate(&ev[1]);
-    if (e == hipSuccess) e = orbfe::launch_valu_probe(sink, blocks, stream, &per);  // warm-up
+    if (e == hipSuccess) e = launch(sink, blocks, stream, &per);  // warm-up
     if (e == hipSuccess) e = hipEventRecord(ev[0], stream);
-    for (int r = 0; r < 5 && e == hipSuccess; ++r) e = orbfe::launch_valu_probe(sink, blocks, stream, &per);
+    for (int r = 0; r < 5 && e == hipSuccess; ++r) e = launch(sink, blocks, stream, &per);
     if (e == hipSuccess) e = hipEventRecord(ev[1], stream);
     if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
     float ms = 0.0f;
@@ -1196,9 +1203,19 @@ int orbfe_probe_valu_rate(orbfe_ctx* ctx, hipStream_t stream, double* rate) {
     for (auto x : ev)
         if (x) (void)hipEventDestroy(x);
     dfree(sink);
-    if (e != hipSuccess) return fail(ctx, ORBFE_EHIP, "valu probe: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return fail(ctx, ORBFE_EHIP, "%s probe: %s", what, hipGetErrorString(e));
     *rate = 5.0 * per / (static_cast<double>(ms) * 1e-3);
     return ORBFE_OK;
+}
+
+int orbfe_probe_valu_rate(orbfe_ctx* ctx, hipStream_t stream, double* rate) {
+    // 8 workgroups per CU, 4 rounds
+    return probe_rate(ctx, stream, 32, "valu", orbfe::launch_valu_probe, rate);
+}
+
+int orbfe_probe_mfma_rate(orbfe_ctx* ctx, hipStream_t stream, double* rate) {
+    // 8 workgroups of 4 waves per CU: 8 waves per SIMD, 2 chains each
+    return probe_rate(ctx, stream, 8, "mfma", orbfe::launch_mfma_probe, rate);
 }
 
 int orbfe_timing_enable(orbfe_ctx* ctx, int enable) {

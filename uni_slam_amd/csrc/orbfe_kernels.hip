@@ -2063,6 +2063,32 @@ hipError_t launch_valu_probe(uint32_t* sink, uint32_t blocks, hipStream_t s,
     return hipGetLastError();
 }
 
+// Matrix-core rate probe: two independent v_mfma_i32_32x32x32_i8 chains per
+// wave (65536 int8 ops each), the form the brute-force matcher issues.
+constexpr uint32_t kProbeMfmaIters = 4096;
+
+__global__ __launch_bounds__(256) void k_mfma_probe(uint32_t* sink, uint32_t seed) {
+    const int32_t v = static_cast<int32_t>(seed + threadIdx.x);
+    const bm_i32x4 a = {v, v ^ 0x5a5a5a5a, 3, 4};
+    const bm_i32x4 b = {5, v, 7, 8};
+    bm_i32x16 c0 = {}, c1 = {};
+    for (uint32_t i = 0; i < kProbeMfmaIters; ++i) {
+        c0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c0, 0, 0, 0);
+        c1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(b, a, c1, 0, 0, 0);
+    }
+    int32_t x = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < 16; ++r) x ^= c0[r] + c1[r];
+    if (x == 0x12345678) sink[blockIdx.x] = static_cast<uint32_t>(x);
+}
+
+hipError_t launch_mfma_probe(uint32_t* sink, uint32_t blocks, hipStream_t s,
+                             double* ops_per_launch) {
+    hipLaunchKernelGGL(k_mfma_probe, dim3(blocks), dim3(256), 0, s, sink, 1u);
+    *ops_per_launch = static_cast<double>(blocks) * 4.0 * 2.0 * kProbeMfmaIters * 65536.0;
+    return hipGetLastError();
+}
+
 constexpr uint32_t kBmGroups = 4;  // 32-query groups per wave
 uint32_t bf_mfma_queries_per_block() { return kBmThreads / 64 * 32 * kBmGroups; }
 uint32_t bf_mfma_window() { return kBmWindow; }

@@ -90,6 +90,7 @@ _SIGNATURES = [
                                     C.POINTER(C.c_double)]),
     ("orbfe_timing_reset", C.c_int, [C.c_void_p]),
     ("orbfe_probe_valu_rate", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]),
+    ("orbfe_probe_mfma_rate", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]),
 ]
 
 ENTRY_POINTS = [s[0] for s in _SIGNATURES]
@@ -280,6 +281,13 @@ class Context:
         """Sustained VALU issue rate of the device (wave64 instructions/s)."""
         r = C.c_double()
         self._check(lib().orbfe_probe_valu_rate(self._h, _stream(stream), C.byref(r)))
+        return float(r.value)
+
+    def probe_mfma_rate(self, stream=None) -> float:
+        """Sustained int8 matrix-core rate of the device (int8 ops/s,
+        v_mfma_i32_32x32x32_i8 chains)."""
+        r = C.c_double()
+        self._check(lib().orbfe_probe_mfma_rate(self._h, _stream(stream), C.byref(r)))
         return float(r.value)
 
 
