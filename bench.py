@@ -418,8 +418,8 @@ def main():
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
                 "traffic": measured_traffic(dom),
                 "valu": valu_roofline(dom, per_kernel[dom]["avg_us"], peak["wave_instr_per_s"]),
-                "limiter": "per-tile latency of barrier-separated phases; VALU issue about 0.55 "
-                           "of the probed peak in 32-frame batches, 0.34 on one frame (neither "
+                "limiter": "per-tile latency of barrier-separated phases; VALU issue about 0.58 "
+                           "of the probed peak in 32-frame batches, 0.36 on one frame (neither "
                            "HBM- nor VALU-bound; profiles/valu_1280x720.md)",
                 "avg_us": per_kernel[dom]["avg_us"], "bytes_per_launch": kb[dom]}
 
