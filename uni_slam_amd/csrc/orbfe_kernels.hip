@@ -1818,21 +1818,40 @@ struct BmQueries {
     int32_t m1[Q], m2[Q];  // the two smallest keys per query (this lane's rows)
 };
 
-// One tile of 32 train rows (this lane's row words lo/hi) against the
-// wave's queries; `cin` is the accumulator input (row offsets in the tile,
-// plus kBmPastEnd for rows past the split end).
+// Moves the kept keys from one tile's base to the next (32 rows on).
 template <uint32_t Q>
-__device__ __forceinline__ void bm_tile(BmQueries<Q>& st, const uint4& lo, const uint4& hi,
-                                        uint32_t half, const bm_i32x16& cin) {
-    const uint32_t w[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+__device__ __forceinline__ void bm_shift(BmQueries<Q>& st) {
+#pragma unroll
+    for (uint32_t g = 0; g < Q; ++g) {
+        st.m1[g] -= 32;
+        st.m2[g] -= 32;
+    }
+}
+
+// One tile of 32 train rows against the wave's queries: the train operands
+// of the tile's 8 steps from LDS (`sa`), the products into the two smallest
+// keys per query.  `crow` is the accumulator input (each row's offset in the
+// tile); rows at offsets >= `valid` (a split's ragged last tile) get
+// kBmPastEnd added before the fold.
+template <uint32_t Q>
+__device__ __forceinline__ void bm_tile_lds(BmQueries<Q>& st, const bm_i32x4 (*sa)[64],
+                                            uint32_t lane, const bm_i32x16& crow,
+                                            uint32_t valid) {
     bm_i32x16 acc[Q];
 #pragma unroll
     for (uint32_t s = 0; s < 8; ++s) {
-        const bm_i32x4 a = bm_signs16(w[s] << (4 * half));
+        const bm_i32x4 a = sa[s][lane];
 #pragma unroll
         for (uint32_t g = 0; g < Q; ++g)
-            acc[g] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, st.b[g][s], s ? acc[g] : cin, 0, 0,
-                                                           0);
+            acc[g] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, st.b[g][s], s ? acc[g] : crow, 0,
+                                                           0, 0);
+    }
+    if (valid < 32) {  // workgroup-uniform
+#pragma unroll
+        for (uint32_t g = 0; g < Q; ++g)
+#pragma unroll
+            for (uint32_t r = 0; r < 16; ++r)
+                acc[g][r] += static_cast<uint32_t>(crow[r]) >= valid ? kBmPastEnd : 0;
     }
 #pragma unroll
     for (uint32_t g = 0; g < Q; ++g) {
@@ -1865,7 +1884,11 @@ __global__ __launch_bounds__(kBmThreads) void k_bf_mfma(const uint32_t* __restri
     const uint32_t t0 = split * split_len;
     const uint32_t t1 = umin(t0 + split_len, nt);
     const uint32_t q0 = (qb * (kBmThreads / 64) + wave) * 32 * Q;
-    if (q0 >= nq) return;  // wave-uniform; no barriers below
+    // Waves past the last query still stage their share of every tile (the
+    // barriers below are workgroup-wide); their results are not written.
+    // Train operands of a tile, double-buffered: wave w expands steps 2w and
+    // 2w + 1 of the tile's 32 rows, every wave reads all 8.
+    __shared__ bm_i32x4 sa[2][8][64];
 
     BmQueries<Q> st;
 #pragma unroll
@@ -1886,30 +1909,30 @@ __global__ __launch_bounds__(kBmThreads) void k_bf_mfma(const uint32_t* __restri
     // past the split end (last tile only) carry kBmPastEnd.
     uint32_t tb = t0;
     if (t0 < t1) {
-        bm_i32x16 crow, ctail;
-        const uint32_t rem = (t1 - t0) % 32;
+        bm_i32x16 crow;
 #pragma unroll
-        for (uint32_t r = 0; r < 16; ++r) {
-            const uint32_t o = 8 * (r >> 2) + 4 * half + (r & 3);
-            crow[r] = static_cast<int32_t>(o);  // + (8192 d - 2^20) from the products
-            ctail[r] = crow[r] + (rem != 0 && o >= rem ? kBmPastEnd : 0);
-        }
-        const uint4* tr = reinterpret_cast<const uint4*>(train);
+        for (uint32_t r = 0; r < 16; ++r)  // + (8192 d - 2^20) from the products
+            crow[r] = static_cast<int32_t>(8 * (r >> 2) + 4 * half + (r & 3));
+        const uint2* tr = reinterpret_cast<const uint2*>(train);  // 4 word pairs per row
         uint32_t ti = umin(t0 + col, t1 - 1);
-        uint4 lo = tr[2ull * ti], hi = tr[2ull * ti + 1];
-        for (; tb + 32 < t1; tb += 32) {
-            const uint4 clo = lo, chi = hi;
-            ti = umin(tb + 32 + col, t1 - 1);  // next tile (clamped: the last row again)
-            lo = tr[2ull * ti];
-            hi = tr[2ull * ti + 1];
-            bm_tile<Q>(st, clo, chi, half, crow);
-#pragma unroll
-            for (uint32_t g = 0; g < Q; ++g) {
-                st.m1[g] -= 32;
-                st.m2[g] -= 32;
+        uint2 wp = tr[4ull * ti + wave];  // words 2 wave, 2 wave + 1 of this lane's row
+        uint32_t buf = 0;
+        for (;; tb += 32) {
+            sa[buf][2 * wave][lane] = bm_signs16(wp.x << (4 * half));
+            sa[buf][2 * wave + 1][lane] = bm_signs16(wp.y << (4 * half));
+            const bool more = tb + 32 < t1;  // workgroup-uniform
+            if (more) {
+                ti = umin(tb + 32 + col, t1 - 1);  // next tile (clamped: the last row again)
+                wp = tr[4ull * ti + wave];
             }
+            // The buffer written here was last read two tiles ago, before
+            // every wave passed the previous tile's barrier.
+            __syncthreads();
+            bm_tile_lds<Q>(st, sa[buf], lane, crow, more ? 32u : t1 - tb);
+            if (!more) break;
+            bm_shift<Q>(st);
+            buf ^= 1u;
         }
-        bm_tile<Q>(st, lo, hi, half, ctail);
     }
 #pragma unroll
     for (uint32_t g = 0; g < Q; ++g) {
@@ -1925,11 +1948,6 @@ __global__ __launch_bounds__(kBmThreads) void k_bf_mfma(const uint32_t* __restri
                 make_uint2(bm_global_key(n1, tb, t0), bm_global_key(n2, tb, t0));
     }
 }
-
-template __global__ void k_bf_mfma<2>(const uint32_t*, uint32_t, const uint32_t*, uint32_t,
-                                      uint32_t, uint32_t, uint2*);
-template __global__ void k_bf_mfma<4>(const uint32_t*, uint32_t, const uint32_t*, uint32_t,
-                                      uint32_t, uint32_t, uint2*);
 
 // ---------------------------------------------------------------------------
 // Host-side launchers (called from orbfe_api.cpp).
