@@ -1763,8 +1763,8 @@ __global__ __launch_bounds__(256) void k_bf_merge(const uint2* __restrict__ part
 // carries each train row's index t in the split (t < 8192), so every output
 // is the key 8192 d + t - 2^20, which orders as (d, t): per query, the two
 // smallest keys are the best and second-best train descriptors, ties to the
-// lower index.  Rows past the split end
-// get a bias of 2^30 and never rank below a real row.
+// lower index.  Rows past the split end get a bias of 2^30 and never rank
+// below a real row.
 //
 // Accumulator layout of the 32x32 products (lane l, register r): query
 // column l % 32, train row 8 (r / 4) + 4 (l / 32) + r % 4.  Operand lane l
@@ -1858,7 +1858,9 @@ __device__ __forceinline__ void bm_tile_lds(BmQueries<Q>& st, const bm_i32x4 (*s
 #pragma unroll
         for (uint32_t r = 0; r < 16; ++r) {
             const int32_t k = acc[g][r];
-            const int32_t n1 = st.m1[g] < k ? st.m1[g] : k;  // first read of k
+            // compiler-generated reads of k (the bias above, this min) come
+            // before the asm med3
+            const int32_t n1 = st.m1[g] < k ? st.m1[g] : k;
             st.m2[g] = med3_i32_after(st.m1[g], st.m2[g], k, n1);
             st.m1[g] = n1;
         }
