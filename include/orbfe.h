@@ -63,7 +63,8 @@ typedef struct orbfe_config {
     /* Launch shapes (no counterpart in the reference; results do not depend
        on them).  0 = default.  detect_tile_rows: rows of a detect tile, 8, 16
        (default) or 32.  describe_lanes: lanes per keypoint in the describe
-       kernel, 8, 16 (default) or 32.  match_lanes: lanes per query in the
+       kernel, 8, 16 or 32 (0: 16 for batches, 32 for a single frame).
+       match_lanes: lanes per query in the
        pipeline matcher, 16, 32 or 64 (0: 16 for batches, 64 for a single
        frame pair).  uni_slam_amd/autotune.py picks them by timing the
        variants on a workload. */

@@ -249,6 +249,8 @@ struct orbfe_ctx {
     Params prm{};
     uint32_t match_lanes_batch = orbfe::kMatchLanesBatch;  // matcher lanes per query
     uint32_t match_lanes_pair = orbfe::kMatchLanesPair;
+    uint32_t desc_lanes_batch = orbfe::kDescLanesDefault;  // describe lanes per keypoint
+    uint32_t desc_lanes_frame = orbfe::kDescLanesFrame;
     std::string err;
     std::map<std::tuple<uint32_t, uint32_t, uint32_t>, std::unique_ptr<PlanEntry>> plans;
     PlanEntry* main = nullptr;
@@ -497,7 +499,8 @@ hipError_t timed(orbfe_ctx* ctx, const char* name, hipStream_t s, F&& launch) {
 }
 
 int launch_detect_stage(orbfe_ctx* ctx, const BatchView& bv, const Job& job0, uint32_t tiles,
-                        uint32_t frames, uint32_t desc_blocks, hipStream_t s) {
+                        uint32_t frames, uint32_t desc_lanes, uint32_t desc_blocks,
+                        hipStream_t s) {
     const Params prm = ctx->prm;
     HIP_TRY(ctx, timed(ctx, "k_detect", s, [&] {
                 return orbfe::launch_detect(ctx->cfg.detect_tile_rows, tiles, bv, job0, prm, s);
@@ -506,8 +509,7 @@ int launch_detect_stage(orbfe_ctx* ctx, const BatchView& bv, const Job& job0, ui
                 return orbfe::launch_compact(frames, bv, job0, ctx->cfg.levels, s);
             }));
     HIP_TRY(ctx, timed(ctx, "k_describe", s, [&] {
-                return orbfe::launch_describe(ctx->cfg.describe_lanes, desc_blocks, bv, job0, prm,
-                                              s);
+                return orbfe::launch_describe(desc_lanes, desc_blocks, bv, job0, prm, s);
             }));
     return ORBFE_OK;
 }
@@ -805,16 +807,18 @@ int orbfe_create(const orbfe_config* cfg, int device, orbfe_ctx** out) {
     if (cfg->orientation_radius < 0 || cfg->orientation_radius > 15) return ORBFE_ERANGE;
     if (cfg->fast_min_run > 16) return ORBFE_ERANGE;
     const uint32_t th = cfg->detect_tile_rows ? cfg->detect_tile_rows : orbfe::kTileHDefault;
-    const uint32_t dl = cfg->describe_lanes ? cfg->describe_lanes : orbfe::kDescLanesDefault;
+    const uint32_t dl = cfg->describe_lanes;
     const uint32_t ml = cfg->match_lanes;
-    if ((th != 8 && th != 16 && th != 32) || (dl != 8 && dl != 16 && dl != 32) ||
+    if ((th != 8 && th != 16 && th != 32) || (dl != 0 && dl != 8 && dl != 16 && dl != 32) ||
         (ml != 0 && ml != 16 && ml != 32 && ml != 64))
         return ORBFE_ERANGE;
     auto ctx = std::make_unique<orbfe_ctx>();
     ctx->device = device;
     ctx->cfg = *cfg;
     ctx->cfg.detect_tile_rows = th;  // every plan of a context shares the launch shapes
-    ctx->cfg.describe_lanes = dl;
+    ctx->desc_lanes_batch = dl ? dl : orbfe::kDescLanesDefault;
+    ctx->desc_lanes_frame = dl ? dl : orbfe::kDescLanesFrame;
+    ctx->cfg.describe_lanes = ctx->desc_lanes_batch;  // the plans' (batch) describe shape
     ctx->match_lanes_batch = ml ? ml : orbfe::kMatchLanesBatch;
     ctx->match_lanes_pair = ml ? ml : orbfe::kMatchLanesPair;
     ctx->prm.threshold = cfg->fast_threshold;
@@ -878,7 +882,8 @@ int orbfe_detect_async(orbfe_ctx* ctx, const uint8_t* image, size_t pitch, float
     Job job0{};
     job0.plan = ctx->main->dev;
     job0.tiles = ctx->main->tiles_single;
-    job0.desc_groups = P.desc_groups;
+    // single-frame launches: their own describe shape (desc_lanes_frame)
+    job0.desc_groups = ceil_div(P.cap, 256 / ctx->desc_lanes_frame);
     job0.cap = P.cap;
     job0.feat_stride = P.feat_stride;
     job0.img = image;
@@ -889,7 +894,8 @@ int orbfe_detect_async(orbfe_ctx* ctx, const uint8_t* image, size_t pitch, float
     job0.kp = ctx->kp;
     job0.smooth = ctx->smooth;
     const BatchView bv{nullptr, nullptr, nullptr, nullptr, nullptr, 0};
-    return launch_detect_stage(ctx, bv, job0, P.tiles, 1, P.L * P.desc_groups, stream);
+    return launch_detect_stage(ctx, bv, job0, P.tiles, 1, ctx->desc_lanes_frame,
+                               P.L * job0.desc_groups, stream);
 }
 
 int orbfe_detect(orbfe_ctx* ctx, const uint8_t* image, size_t pitch, float* pyramid,
@@ -934,7 +940,7 @@ int orbfe_detect_batch_async(orbfe_ctx* ctx, const orbfe_frame* frames, uint32_t
     rc = commit_table(ctx, slot, lay.table_bytes, stream);
     if (rc) return rc;
     const BatchView bv = detect_view(lay, static_cast<uint8_t*>(slot->dev));
-    return launch_detect_stage(ctx, bv, Job{}, lay.tile_prefix[n], n,
+    return launch_detect_stage(ctx, bv, Job{}, lay.tile_prefix[n], n, ctx->desc_lanes_batch,
                                lay.desc_prefix[n], stream);
 }
 
@@ -975,7 +981,7 @@ int orbfe_batch_create(orbfe_ctx* ctx, const orbfe_frame* frames, uint32_t n,
 int orbfe_batch_detect_async(orbfe_ctx* ctx, orbfe_batch* batch, hipStream_t stream) {
     if (!ctx || !batch || batch->ctx != ctx) return fail(ctx, ORBFE_EINVAL, "bad batch");
     return launch_detect_stage(ctx, batch->view, Job{}, batch->tiles, batch->n,
-                               batch->desc_blocks, stream);
+                               ctx->desc_lanes_batch, batch->desc_blocks, stream);
 }
 
 void orbfe_batch_destroy(orbfe_batch* batch) {

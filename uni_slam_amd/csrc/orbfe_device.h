@@ -32,7 +32,8 @@ constexpr int kDetectThreads = 256;
 
 // k_describe: desc_lanes lanes per keypoint (8, 16 or 32), so 256 /
 // desc_lanes keypoints per 256-thread workgroup.
-constexpr uint32_t kDescLanesDefault = 16;
+constexpr uint32_t kDescLanesDefault = 16;  // batches
+constexpr uint32_t kDescLanesFrame = 32;    // single-frame launches (latency-bound)
 // k_match_dir: match_lanes lanes per query (16, 32 or 64), so 256 /
 // match_lanes queries per 256-thread workgroup.  By default a batch uses 16
 // (many queries keep the device full) and a single frame pair 64 (its

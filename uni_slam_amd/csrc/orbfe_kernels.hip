@@ -1095,10 +1095,11 @@ struct LevelSrc {
 // LANES lanes per keypoint (8, 16 or 32; 256 / LANES keypoints per
 // workgroup): lane s of a group scores ring pixels s, s + LANES, ... and
 // evaluates tests TPL*s .. TPL*s + TPL - 1, TPL = 256 / LANES.  Every
-// variant computes each record with the same operations in the same order.
+// variant computes each record with the same operations in the same order;
+// kAllLoads only changes how many gather loads are in flight at once.
 constexpr uint32_t kDescribeThreads = 256;
 
-template <uint32_t LANES>
+template <uint32_t LANES, bool kAllLoads>
 __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job job0,
                                                                 Params prm) {
     constexpr uint32_t KPB = kDescribeThreads / LANES;  // keypoints per block
@@ -1264,32 +1265,17 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
         desc_order<LANES>() + (static_cast<size_t>(bin) * 2 * LANES + sub) * PPL);
     float* ks = samp[tid / LANES];
     uint32_t bits = 0;
-#pragma unroll
-    for (uint32_t half = 0; half < 2; ++half) {
-        uint32_t ids[PPL / 4];
-#pragma unroll
-        for (uint32_t k = 0; k < PPL / 4; ++k) ids[k] = order[half * LANES * PPL / 4 + k];
-        constexpr uint32_t kB = PPL < 8 ? PPL : 8;  // loads in flight per batch
-#pragma unroll
-        for (uint32_t k0 = 0; k0 < PPL; k0 += kB) {
-            float v[kB];
-            uint32_t qs[kB];
-#pragma unroll
-            for (uint32_t u = 0; u < kB; ++u) {
-                const uint32_t k = k0 + u;
-                const uint32_t q = (ids[k / 4] >> (8 * (k % 4))) & 0xFFu;
-                qs[u] = q;
-                const float2 pp = ppt[half][q];
-                const uint32_t ax4 =
-                    umin(4 * __float_as_uint((pp.x * cs - pp.y * sn) + kRound) + xb4, xmax4);
-                const uint32_t ay =
-                    umin(__float_as_uint((pp.x * sn + pp.y * cs) + kRound) + yb, sh - 1);
-                // byte offsets < 4 * sw * sh (24-bit factors)
-                v[u] = load_f32(sm, __umul24(ay, sw4) + ax4);
-            }
-#pragma unroll
-            for (uint32_t u = 0; u < kB; ++u) ks[qs[u] + (qs[u] >> 4)] = v[u];
-        }
+    // Byte offset in the smoothed image of BRIEF point q of a half, rotated
+    // by the keypoint's angle (offsets < 4 * sw * sh: 24-bit factors).
+    const auto point_offset = [&](uint32_t half, uint32_t q) -> uint32_t {
+        const float2 pp = ppt[half][q];
+        const uint32_t ax4 =
+            umin(4 * __float_as_uint((pp.x * cs - pp.y * sn) + kRound) + xb4, xmax4);
+        const uint32_t ay = umin(__float_as_uint((pp.x * sn + pp.y * cs) + kRound) + yb, sh - 1);
+        return __umul24(ay, sw4) + ax4;
+    };
+    // The tests of a half, once its samples are in ks.
+    const auto half_tests = [&](uint32_t half) {
         wave_sync();
 #pragma unroll
         for (uint32_t k = half; k < TPL; k += 2) {
@@ -1298,6 +1284,51 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
             bits |= (sa > sb ? 1u : 0u) << k;
         }
         wave_sync();
+    };
+    if constexpr (kAllLoads) {
+        // Single-frame launches (latency-bound): both halves' loads in flight
+        // at once, then each half through LDS.
+        float v[2][PPL];
+        uint32_t qs[2][PPL];
+#pragma unroll
+        for (uint32_t half = 0; half < 2; ++half) {
+            uint32_t ids[PPL / 4];
+#pragma unroll
+            for (uint32_t k = 0; k < PPL / 4; ++k) ids[k] = order[half * LANES * PPL / 4 + k];
+#pragma unroll
+            for (uint32_t k = 0; k < PPL; ++k) {
+                qs[half][k] = (ids[k / 4] >> (8 * (k % 4))) & 0xFFu;
+                v[half][k] = load_f32(sm, point_offset(half, qs[half][k]));
+            }
+        }
+#pragma unroll
+        for (uint32_t half = 0; half < 2; ++half) {
+#pragma unroll
+            for (uint32_t k = 0; k < PPL; ++k) ks[qs[half][k] + (qs[half][k] >> 4)] = v[half][k];
+            half_tests(half);
+        }
+    } else {
+#pragma unroll
+        for (uint32_t half = 0; half < 2; ++half) {
+            uint32_t ids[PPL / 4];
+#pragma unroll
+            for (uint32_t k = 0; k < PPL / 4; ++k) ids[k] = order[half * LANES * PPL / 4 + k];
+            constexpr uint32_t kB = PPL < 8 ? PPL : 8;  // loads in flight per batch
+#pragma unroll
+            for (uint32_t k0 = 0; k0 < PPL; k0 += kB) {
+                float v[kB];
+                uint32_t qs[kB];
+#pragma unroll
+                for (uint32_t u = 0; u < kB; ++u) {
+                    const uint32_t k = k0 + u;
+                    qs[u] = (ids[k / 4] >> (8 * (k % 4))) & 0xFFu;
+                    v[u] = load_f32(sm, point_offset(half, qs[u]));
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < kB; ++u) ks[qs[u] + (qs[u] >> 4)] = v[u];
+            }
+            half_tests(half);
+        }
     }
     // Record word e: x, y, strength, angle, then descriptor word e - 4 =
     // bits of lanes LPW*(e-4) .. LPW*(e-4) + LPW - 1, low lane first.
@@ -2017,12 +2048,26 @@ hipError_t launch_compact(uint32_t frames, BatchView bv, const Job& job0, uint32
     return hipGetLastError();
 }
 
+// Single-frame launches (no batch table) use the variant with both gather
+// halves in flight (latency-bound: a few waves per SIMD); batches keep one
+// half at a time (fewer VGPRs, more resident waves).
+template <uint32_t LANES>
+void launch_describe_t(uint32_t blocks, BatchView bv, const Job& job0, const Params& prm,
+                       hipStream_t s) {
+    if (bv.n == 0)
+        hipLaunchKernelGGL((k_describe<LANES, true>), dim3(blocks), dim3(kDescribeThreads), 0, s,
+                           bv, job0, prm);
+    else
+        hipLaunchKernelGGL((k_describe<LANES, false>), dim3(blocks), dim3(kDescribeThreads), 0, s,
+                           bv, job0, prm);
+}
+
 hipError_t launch_describe(uint32_t lanes, uint32_t blocks, BatchView bv, const Job& job0,
                            const Params& prm, hipStream_t s) {
     switch (lanes) {
-        case 8: hipLaunchKernelGGL(k_describe<8>, dim3(blocks), dim3(kDescribeThreads), 0, s, bv, job0, prm); break;
-        case 16: hipLaunchKernelGGL(k_describe<16>, dim3(blocks), dim3(kDescribeThreads), 0, s, bv, job0, prm); break;
-        case 32: hipLaunchKernelGGL(k_describe<32>, dim3(blocks), dim3(kDescribeThreads), 0, s, bv, job0, prm); break;
+        case 8: launch_describe_t<8>(blocks, bv, job0, prm, s); break;
+        case 16: launch_describe_t<16>(blocks, bv, job0, prm, s); break;
+        case 32: launch_describe_t<32>(blocks, bv, job0, prm, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
