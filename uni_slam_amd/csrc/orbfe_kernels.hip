@@ -1436,9 +1436,9 @@ __global__ __launch_bounds__(kIndexThreads) void k_match_index(MatchBatchView bv
         float x, y;
     };
     static_assert(kIndexHeld == 4, "four held records");
-    const auto load = [&](uint32_t i) -> Held {
+    const auto load = [&](uint32_t i, uint32_t lim) -> Held {
         Held h{};
-        if (i < n) {
+        if (i < lim) {
             const uint32_t* r = A.rec(l, i);
             h.head = *reinterpret_cast<const uint4*>(r);
             h.lo = *reinterpret_cast<const uint4*>(r + 4);
@@ -1446,10 +1446,21 @@ __global__ __launch_bounds__(kIndexThreads) void k_match_index(MatchBatchView bv
         }
         return h;
     };
-    Held h0 = load(tid);
-    Held h1 = load(tid + kIndexThreads);
-    Held h2 = load(tid + 2 * kIndexThreads);
-    Held h3 = load(tid + 3 * kIndexThreads);
+    // A single pair (latency-bound) loads the level's slots up to the cap
+    // without waiting for its count (slots past the count are read, never
+    // used); batches load only the counted records.
+    Held h0, h1, h2, h3;
+    if (bv.n == 0) {
+        h0 = load(tid, P.cap);
+        h1 = load(tid + kIndexThreads, P.cap);
+        h2 = load(tid + 2 * kIndexThreads, P.cap);
+        h3 = load(tid + 3 * kIndexThreads, P.cap);
+    } else {
+        h0 = load(tid, n);
+        h1 = load(tid + kIndexThreads, n);
+        h2 = load(tid + 2 * kIndexThreads, n);
+        h3 = load(tid + 3 * kIndexThreads, n);
+    }
     // The mutual pass writes only the matched entries of prev_match; the
     // direction pass appends to the query list.
     if (side == 0)
