@@ -1595,10 +1595,6 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
         l = b / groups;
         slot = (b - l * groups) * QPB + grp;
         if (slot >= M.cap) return;
-        if (slot >= FeatView{M.curr, M.feat_stride}.count(l, M.cap)) {
-            if (sub == 0) M.b_to_a[l * M.cap + slot] = -1;
-            return;
-        }
     } else {
         const uint32_t seg = b % kListSegs;
         const uint32_t t = (b / kListSegs) * QPB + grp;
@@ -1621,19 +1617,33 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
     const uint32_t p0 = ql;
     const uint32_t p1 = ql > 0 ? ql - 1 : ql + 1;
     const uint32_t p2 = ql + 1;
-    uint32_t npool = 1, total_pool = C.count(ql, M.cap);
-    if (ql > 0) { total_pool += C.count(ql - 1, M.cap); ++npool; }
-    if (ql + 1 < M.levels) { total_pool += C.count(ql + 1, M.cap); ++npool; }
+    // The query's record (qs < cap: inside its level block), the pool
+    // counts and, in the direction pass, the query level's own count are
+    // loaded together before any of them is used.
+    const uint32_t* qr = Q.rec(ql, qs);
+    const uint4 qw0 = *reinterpret_cast<const uint4*>(qr);
+    const uint4 qw1 = *reinterpret_cast<const uint4*>(qr + 4);
+    const uint4 qw2 = *reinterpret_cast<const uint4*>(qr + 8);
+    const uint32_t c0 = C.count(ql, M.cap);
+    const uint32_t c1 = C.count(ql > 0 ? ql - 1 : ql, M.cap);
+    const uint32_t c2 = C.count(ql + 1 < M.levels ? ql + 1 : ql, M.cap);
+    if (!kMutual && slot >= FeatView{M.curr, M.feat_stride}.count(l, M.cap)) {
+        if (sub == 0) M.b_to_a[l * M.cap + slot] = -1;
+        return;
+    }
+    uint32_t npool = 1, total_pool = c0;
+    if (ql > 0) { total_pool += c1; ++npool; }
+    if (ql + 1 < M.levels) { total_pool += c2; ++npool; }
     const auto pool_level = [&](uint32_t s) { return s == 0 ? p0 : (s == 1 ? p1 : p2); };
 
     int32_t result = -1;
     if (total_pool >= 2) {
-        const uint32_t* qr = Q.rec(ql, qs);
         float qx, qy;
-        rec_lod0(qr, P.lv[ql].lod0_scale, &qx, &qy);
-        uint32_t qd[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) qd[k] = qr[4 + k];
+        {
+            const uint32_t head[2] = {qw0.x, qw0.y};
+            rec_lod0(head, P.lv[ql].lod0_scale, &qx, &qy);
+        }
+        const uint32_t qd[8] = {qw1.x, qw1.y, qw1.z, qw1.w, qw2.x, qw2.y, qw2.z, qw2.w};
         const float lo_x = qx - P.max_dx;
         const float hi_x = qx + P.max_dx;
         const float dy = P.max_dy;
