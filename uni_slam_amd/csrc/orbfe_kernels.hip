@@ -1598,10 +1598,13 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
     } else {
         const uint32_t seg = b % kListSegs;
         const uint32_t t = (b / kListSegs) * QPB + grp;
-        if (t >= M.q_list[seg * kListStride]) return;  // written by the direction pass
         const uint32_t blocks = M.levels * ((M.cap + QPB - 1) / QPB);
+        // Entry t of the segment (t < the segment's capacity for every block
+        // of the grid) is loaded together with the segment's count, which
+        // the direction pass wrote.
         const uint32_t e = M.q_list[kListSegs * kListStride +
                                     seg * query_segment_capacity(blocks, QPB) + t];
+        if (t >= M.q_list[seg * kListStride]) return;
         l = e >> 24;
         slot = e & 0xFFFFFFu;
     }
