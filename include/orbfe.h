@@ -59,7 +59,11 @@ typedef struct orbfe_config {
     uint32_t features_per_level; /* FEATURE_BLOCK_SZ; 0 = W0*H0/250, common.hpp:47 */
     uint32_t border_zero;   /* textureLoad out of range: 0 = Tint clamp
                                (robustness.cc:344-355), 1 = read zero */
-    float lowe_ratio;       /* kLoweTestThreshold, match_features_cpu.cpp:75 (0.85) */
+    float lowe_ratio;       /* kLoweTestThreshold, match_features_cpu.cpp:75 (0.85);
+                               must lie in (0, 1] (ORBFE_ERANGE otherwise): above
+                               1 a tie at the best distance could pass the
+                               ratio test, and which tied candidate the
+                               reference keeps depends on its unstable sort */
     /* Launch shapes (no counterpart in the reference; results do not depend
        on them).  0 = default.  detect_tile_rows: rows of a detect tile, 8, 16
        (default) or 32.  describe_lanes: lanes per keypoint in the describe

@@ -48,6 +48,18 @@ def test_create_rejects_unsupported_launch_shapes():
         assert not h.value
 
 
+def test_create_rejects_ratio_outside_unit_interval():
+    """A Lowe ratio above 1 would let tied best distances pass, where the
+    reference's choice depends on its unstable sort; 0, negatives and NaN
+    are refused as well (checked before any device call)."""
+    import ctypes as C
+    for bad in (1.0001, 1.5, 0.0, -0.8, float("nan")):
+        cfg = orbfe.default_config(320, 240, lowe_ratio=bad)
+        h = C.c_void_p()
+        assert orbfe.lib().orbfe_create(C.byref(cfg), 0, C.byref(h)) == orbfe.ERANGE, bad
+        assert not h.value
+
+
 def test_load_features_host_entry_point():
     cap = 5
     feats = np.zeros(6 * (8 + 48 * cap), np.uint8)

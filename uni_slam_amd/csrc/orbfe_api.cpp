@@ -226,6 +226,18 @@ struct PlanEntry {
     size_t surv_bytes = 0, kp_bytes = 0, pyr_bytes = 0, feat_bytes = 0, match_bytes = 0;
     size_t sm_bytes = 0, index_bytes = 0;
     size_t pair_bytes = 0;  // per pair matcher scratch: b_to_a, then the query list
+
+    PlanEntry() = default;
+    PlanEntry(const PlanEntry&) = delete;
+    PlanEntry& operator=(const PlanEntry&) = delete;
+    // Frees the plan's device buffers (on the current device: the context
+    // that owns the plan selects its device first).
+    ~PlanEntry() {
+        for (void* p : {static_cast<void*>(dev), static_cast<void*>(mask),
+                        static_cast<void*>(taps), static_cast<void*>(tiles),
+                        static_cast<void*>(tiles_single)})
+            if (p) (void)hipFree(p);
+    }
 };
 
 struct Event2 {
@@ -764,13 +776,7 @@ void orbfe_destroy(orbfe_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     (void)hipDeviceSynchronize();
-    for (auto& kv : ctx->plans) {
-        dfree(kv.second->mask);
-        dfree(kv.second->taps);
-        dfree(kv.second->tiles);
-        dfree(kv.second->tiles_single);
-        dfree(kv.second->dev);
-    }
+    ctx->plans.clear();  // PlanEntry frees its device buffers
     dfree(ctx->surv);
     dfree(ctx->kp);
     dfree(ctx->pyr_scratch);
@@ -804,6 +810,11 @@ int orbfe_create(const orbfe_config* cfg, int device, orbfe_ctx** out) {
     *out = nullptr;
     if (cfg->levels == 0 || cfg->levels > ORBFE_MAX_LEVELS) return ORBFE_ERANGE;
     if (!(cfg->scale_factor > 1.0)) return ORBFE_ERANGE;
+    // A ratio above 1 lets a tie at the best distance pass the ratio test,
+    // and the reference then keeps whichever tied candidate its unstable
+    // sort by x puts first (match_features_cpu.cpp:101-146): refused.  NaN
+    // fails the comparison as well.
+    if (!(cfg->lowe_ratio > 0.0f && cfg->lowe_ratio <= 1.0f)) return ORBFE_ERANGE;
     if (cfg->orientation_radius < 0 || cfg->orientation_radius > 15) return ORBFE_ERANGE;
     if (cfg->fast_min_run > 16) return ORBFE_ERANGE;
     const uint32_t th = cfg->detect_tile_rows ? cfg->detect_tile_rows : orbfe::kTileHDefault;
@@ -836,9 +847,14 @@ int orbfe_create(const orbfe_config* cfg, int device, orbfe_ctx** out) {
     if (hipSetDevice(device) != hipSuccess) return ORBFE_EHIP;
     if (orbfe::upload_pattern() != hipSuccess) return ORBFE_EHIP;
     if (hipStreamCreate(&c->stream) != hipSuccess) return ORBFE_EHIP;
+    // From here on every failure goes through orbfe_destroy, which releases
+    // the stream, the plans and whatever scratch was allocated.
     PlanEntry* pe = nullptr;
-    if (get_plan(c, cfg->frame_width, cfg->frame_height, cfg->features_per_level, &pe))
-        return ORBFE_EINVAL;
+    if (const int rc = get_plan(c, cfg->frame_width, cfg->frame_height,
+                                cfg->features_per_level, &pe)) {
+        orbfe_destroy(ctx.release());
+        return rc;
+    }
     c->main = pe;
     if (dalloc(c, &c->surv, pe->surv_bytes) || dalloc(c, &c->kp, pe->kp_bytes) ||
         dalloc(c, &c->pyr_scratch, pe->pyr_bytes) || dalloc(c, &c->smooth, pe->sm_bytes) ||
