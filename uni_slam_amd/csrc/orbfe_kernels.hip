@@ -1848,14 +1848,15 @@ __device__ inline bm_i32x4 bm_signs16(uint32_t v) {
     return r;
 }
 
-// Middle value of three (v_med3_i32).  `after` is not read: it orders the
-// instruction after the code that produces it (below, a compiler-generated
-// read of the same matrix-core result, which carries the wait states the
-// result needs before VALU may read it).
-__device__ inline int32_t med3_i32_after(int32_t a, int32_t b, int32_t c, int32_t after) {
-    int32_t r;
-    asm("v_med3_i32 %0, %1, %2, %3 ; after %4" : "=v"(r) : "v"(a), "v"(b), "v"(c), "v"(after));
-    return r;
+// Middle value of three, written as max(min(a, b), min(max(a, b), c)),
+// the form the backend folds into one v_med3_i32.  Plain code rather than
+// inline asm: the reads of a matrix-core result stay visible to the
+// compiler, whose hazard recognizer inserts the wait states such a read
+// needs.
+__device__ inline int32_t med3_i32(int32_t a, int32_t b, int32_t c) {
+    const int32_t lo = a < b ? a : b, hi = a < b ? b : a;
+    const int32_t m = hi < c ? hi : c;
+    return lo < m ? m : lo;
 }
 
 // Key relative to the tile at `tb` (8192 d + t - tb - 2^20) to the key the
@@ -1913,10 +1914,8 @@ __device__ __forceinline__ void bm_tile_lds(BmQueries<Q>& st, const bm_i32x4 (*s
 #pragma unroll
         for (uint32_t r = 0; r < 16; ++r) {
             const int32_t k = acc[g][r];
-            // compiler-generated reads of k (the bias above, this min) come
-            // before the asm med3
             const int32_t n1 = st.m1[g] < k ? st.m1[g] : k;
-            st.m2[g] = med3_i32_after(st.m1[g], st.m2[g], k, n1);
+            st.m2[g] = med3_i32(st.m1[g], st.m2[g], k);
             st.m1[g] = n1;
         }
     }
