@@ -221,6 +221,23 @@ int orbfe_match_bf_async(struct orbfe_ctx* ctx, const uint32_t* query, uint32_t 
                          const uint32_t* train, uint32_t nt, uint32_t* idx,
                          uint32_t* d1, uint32_t* d2, hipStream_t stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the search of orbfe_match_bf_async, with identical
+ * results, whose per-split partial results live in a static device array
+ * of ORBFE_BF_NOALLOC_SCRATCH_BYTES that the library holds on every device
+ * it runs on until the process ends, instead of context memory grown with
+ * hipMalloc.  The call allocates nothing.  Limit: nq * ceil(nt / 8192) *
+ * 8 bytes must fit the array (nq * ceil(nt / 8192) <= 2^21, e.g. 65536
+ * queries against up to 262144 train descriptors); larger searches fail
+ * with ORBFE_ERANGE.  Every call on one device shares that device's array:
+ * calls that overlap on one GPU (on different streams, or from different
+ * contexts) race; calls ordered on one stream do not. */
+#define ORBFE_BF_NOALLOC_SCRATCH_BYTES (16u << 20)
+int orbfe_match_bf_noalloc_async(struct orbfe_ctx* ctx, const uint32_t* query, uint32_t nq,
+                                 const uint32_t* train, uint32_t nt, uint32_t* idx,
+                                 uint32_t* d1, uint32_t* d2, hipStream_t stream);
+
 /* Kernel timing: when enabled, every kernel launch of the context is
  * bracketed by HIP events on its stream; orbfe_timing_read synchronizes and
  * returns, for kernel `name`, the launch count and summed duration (ms). */

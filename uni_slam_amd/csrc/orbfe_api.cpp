@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <memory>
 #include <string>
 #include <tuple>
@@ -35,6 +36,7 @@ hipError_t launch_bf_mfma(const uint32_t* q, uint32_t nq, const uint32_t* t, uin
                           uint32_t splits, uint32_t split_len, uint2* part, hipStream_t s);
 hipError_t launch_bf_merge(const uint2* part, uint32_t nq, uint32_t splits, uint32_t* idx,
                            uint32_t* d1, uint32_t* d2, hipStream_t s);
+hipError_t bf_scratch_address(void** out);
 hipError_t launch_valu_probe(uint32_t* sink, uint32_t blocks, hipStream_t s,
                              double* instr_per_launch);
 hipError_t launch_mfma_probe(uint32_t* sink, uint32_t blocks, hipStream_t s,
@@ -282,6 +284,7 @@ struct orbfe_ctx {
     // brute-force partials
     uint2* bf_part = nullptr;
     size_t bf_part_bytes = 0;
+    uint2* bf_static = nullptr;  // the device's static scratch (bf_scratch_address)
     // dynamic batches
     StageSlot ring[4];
     uint32_t ring_next = 0;
@@ -348,6 +351,42 @@ template <typename T>
 void dfree(T*& p) {
     if (p) (void)hipFree(p);
     p = nullptr;
+}
+
+// Device address of the static brute-force scratch of each device, looked
+// up once per device (with that device current) on the first orbfe_create
+// there.
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_addr;
+
+int scratch_address(int device, void** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_addr.find(device);
+    if (it == g_scratch_addr.end()) {
+        void* p = nullptr;
+        if (orbfe::bf_scratch_address(&p) != hipSuccess || !p) return ORBFE_EHIP;
+        it = g_scratch_addr.emplace(device, p).first;
+    }
+    *out = it->second;
+    return ORBFE_OK;
+}
+
+// Train splits of a brute-force search: at most 8192 rows each (the index
+// field of the matrix-core keys), and more when the query blocks alone are
+// too few to cover the 256 CUs several times over; no split is empty.
+// Partial results take splits * nq * 8 bytes.
+void bf_split_plan(uint32_t nq, uint32_t nt, uint32_t* splits_out, uint32_t* len_out) {
+    const uint32_t qblocks = ceil_div(nq, orbfe::bf_mfma_queries_per_block());
+    uint32_t splits = 1;
+    if (nt > 0) {
+        const uint32_t want = std::max<uint32_t>(1, 2048 / qblocks);
+        const uint32_t max_splits = std::max<uint32_t>(1, nt / 256);
+        splits = std::max(ceil_div(nt, orbfe::bf_mfma_window()), std::min(want, max_splits));
+    }
+    const uint32_t split_len = nt > 0 ? ceil_div(nt, splits) : 0;
+    if (nt > 0) splits = ceil_div(nt, split_len);
+    *splits_out = splits;
+    *len_out = split_len;
 }
 
 uint32_t default_cap(uint32_t W0, uint32_t H0) {
@@ -847,6 +886,10 @@ int orbfe_create(const orbfe_config* cfg, int device, orbfe_ctx** out) {
     if (hipSetDevice(device) != hipSuccess) return ORBFE_EHIP;
     if (orbfe::upload_pattern() != hipSuccess) return ORBFE_EHIP;
     if (hipStreamCreate(&c->stream) != hipSuccess) return ORBFE_EHIP;
+    if (scratch_address(device, reinterpret_cast<void**>(&c->bf_static)) != ORBFE_OK) {
+        orbfe_destroy(ctx.release());
+        return ORBFE_EHIP;
+    }
     // From here on every failure goes through orbfe_destroy, which releases
     // the stream, the plans and whatever scratch was allocated.
     PlanEntry* pe = nullptr;
@@ -1163,19 +1206,8 @@ int orbfe_match_bf_async(orbfe_ctx* ctx, const uint32_t* query, uint32_t nq,
         return fail(ctx, ORBFE_EINVAL, "null pointer");
     if (nt >= (1u << orbfe::kBfKeyShift))
         return fail(ctx, ORBFE_ERANGE, "train count %u >= 2^%u", nt, orbfe::kBfKeyShift);
-    // The train set is cut into splits of at most 8192 rows (the index
-    // field of the matrix-core keys), and into more when the query blocks
-    // alone are too few to cover the 256 CUs several times over; no split
-    // is empty.
-    const uint32_t qblocks = ceil_div(nq, orbfe::bf_mfma_queries_per_block());
-    uint32_t splits = 1;
-    if (nt > 0) {
-        const uint32_t want = std::max<uint32_t>(1, 2048 / qblocks);
-        const uint32_t max_splits = std::max<uint32_t>(1, nt / 256);
-        splits = std::max(ceil_div(nt, orbfe::bf_mfma_window()), std::min(want, max_splits));
-    }
-    const uint32_t split_len = nt > 0 ? ceil_div(nt, splits) : 0;
-    if (nt > 0) splits = ceil_div(nt, split_len);
+    uint32_t splits = 0, split_len = 0;
+    bf_split_plan(nq, nt, &splits, &split_len);
     const size_t need = static_cast<size_t>(splits) * nq * sizeof(uint2);
     if (need > ctx->bf_part_bytes) {
         HIP_TRY(ctx, hipStreamSynchronize(stream));
@@ -1193,6 +1225,40 @@ int orbfe_match_bf_async(orbfe_ctx* ctx, const uint32_t* query, uint32_t nq,
             }));
     HIP_TRY(ctx, timed(ctx, "k_bf_merge", stream, [&] {
                 return orbfe::launch_bf_merge(ctx->bf_part, nq, splits, idx, d1, d2, stream);
+            }));
+    return ORBFE_OK;
+}
+
+int orbfe_match_bf_noalloc_async(orbfe_ctx* ctx, const uint32_t* query, uint32_t nq,
+                                 const uint32_t* train, uint32_t nt, uint32_t* idx, uint32_t* d1,
+                                 uint32_t* d2, hipStream_t stream) {
+    if (!ctx) return ORBFE_EINVAL;
+    if (nq == 0) return ORBFE_OK;
+    if (!query || !idx || !d1 || !d2 || (nt > 0 && !train))
+        return fail(ctx, ORBFE_EINVAL, "null pointer");
+    if (nt >= (1u << orbfe::kBfKeyShift))
+        return fail(ctx, ORBFE_ERANGE, "train count %u >= 2^%u", nt, orbfe::kBfKeyShift);
+    if (!ctx->bf_static) return fail(ctx, ORBFE_EHIP, "static scratch address unknown");
+    // The split plan of orbfe_match_bf_async.  Its occupancy term keeps
+    // splits * nq <= 2^20 (at most 2048 / ceil(nq / 512) splits), so only
+    // the ceil(nt / 8192) splits the key's index field needs can exceed the
+    // scratch.
+    static_assert(orbfe::kBfScratchBytes == ORBFE_BF_NOALLOC_SCRATCH_BYTES, "orbfe.h limit");
+    constexpr size_t kEntries = orbfe::kBfScratchBytes / sizeof(uint2);
+    uint32_t splits = 0, split_len = 0;
+    bf_split_plan(nq, nt, &splits, &split_len);
+    if (static_cast<size_t>(splits) * nq > kEntries)
+        return fail(ctx, ORBFE_ERANGE,
+                    "%u queries x %u train splits need %zu bytes of partial results; the static "
+                    "scratch holds %zu (nq * ceil(nt / 8192) <= %zu)",
+                    nq, splits, static_cast<size_t>(splits) * nq * sizeof(uint2),
+                    orbfe::kBfScratchBytes, kEntries);
+    HIP_TRY(ctx, timed(ctx, "k_bf_mfma", stream, [&] {
+                return orbfe::launch_bf_mfma(query, nq, train, nt, splits, split_len,
+                                             ctx->bf_static, stream);
+            }));
+    HIP_TRY(ctx, timed(ctx, "k_bf_merge", stream, [&] {
+                return orbfe::launch_bf_merge(ctx->bf_static, nq, splits, idx, d1, d2, stream);
             }));
     return ORBFE_OK;
 }

@@ -69,6 +69,9 @@ constexpr uint32_t kMatchKeyShift = 22;
 constexpr uint32_t kMatchIdxBits = 20;
 // Brute-force matcher keys: distance above a 23-bit train index.
 constexpr uint32_t kBfKeyShift = 23;
+// Static partial-result scratch of the allocation-free brute-force matcher
+// (ORBFE_BF_NOALLOC_SCRATCH_BYTES of orbfe.h): 8 bytes per query and split.
+constexpr size_t kBfScratchBytes = size_t(16) << 20;
 
 // The smoothed image of a level holds G(P), the 9-tap Gaussian of
 // generate_features.wgsl:149-172, for every integer point P of an extended

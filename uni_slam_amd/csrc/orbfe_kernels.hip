@@ -2196,4 +2196,16 @@ hipError_t launch_bf_merge(const uint2* part, uint32_t nq, uint32_t splits, uint
     return hipGetLastError();
 }
 
+// Partial results of orbfe_match_bf_noalloc_async: a static device array
+// (one copy per GPU, held until the process ends) instead of context memory
+// grown with hipMalloc.  Host code never passes `scratch` itself to a
+// kernel: in host code the name is only a stand-in, and the device address
+// comes from bf_scratch_address.
+static __device__ __align__(256) uint2 scratch[kBfScratchBytes / sizeof(uint2)];
+
+// Device address of `scratch` on the current device.
+hipError_t bf_scratch_address(void** out) {
+    return hipGetSymbolAddress(out, HIP_SYMBOL(scratch));
+}
+
 }  // namespace orbfe

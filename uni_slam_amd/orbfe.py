@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(_HERE, "liborbfe.so")
 OK = 0
 EINVAL, EHIP, ENOMEM, ERANGE = -1, -2, -3, -4  # ORBFE_* error codes of orbfe.h
 MAX_LEVELS = 8
+NOALLOC_SCRATCH_BYTES = 16 << 20  # ORBFE_BF_NOALLOC_SCRATCH_BYTES
 
 GPU_FEATURE_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("strength", "<u4"),
                               ("orientation", "<f4"), ("descriptor", "<u4", (8,))])
@@ -85,6 +86,9 @@ _SIGNATURES = [
     ("orbfe_match_bf_async", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
                                        C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_void_p]),
+    ("orbfe_match_bf_noalloc_async", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
+                                               C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                               C.c_void_p]),
     ("orbfe_timing_enable", C.c_int, [C.c_void_p, C.c_int]),
     ("orbfe_timing_read", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_uint32),
                                     C.POINTER(C.c_double)]),
@@ -241,6 +245,13 @@ class Context:
     def match_bf_async(self, query, nq: int, train, nt: int, idx, d1, d2, stream=None):
         self._check(lib().orbfe_match_bf_async(self._h, _ptr(query), nq, _ptr(train), nt,
                                                _ptr(idx), _ptr(d1), _ptr(d2), _stream(stream)))
+
+    def match_bf_noalloc_async(self, query, nq: int, train, nt: int, idx, d1, d2,
+                               stream=None):
+        """match_bf_async without any allocation (static device scratch)."""
+        self._check(lib().orbfe_match_bf_noalloc_async(self._h, _ptr(query), nq, _ptr(train),
+                                                       nt, _ptr(idx), _ptr(d1), _ptr(d2),
+                                                       _stream(stream)))
 
     # -- host-memory forms ---------------------------------------------------
     def detect(self, image: np.ndarray, with_pyramid: bool = False):
