@@ -181,6 +181,20 @@ void orbfe_batch_destroy(struct orbfe_batch* batch);
 int orbfe_load_features(const void* features, uint32_t levels, uint32_t cap,
                         orbfe_feature* out, uint32_t* counts);
 
+/* Zero-copy readback: replaces GPU::readBuffer of the feature slab
+ * (compute/src/gpu.cpp:892-929) and the FromGpuModel loop of
+ * LoadDataCPUPass::loadFeatures (load_features_cpu.cpp:164-188).  A kernel
+ * on `stream` writes the FeatureSet of a device feature array of this
+ * context's frame size straight into pinned host memory: out gets the
+ * orbfe_feature (wslam::Feature) records of every level back to back,
+ * min(count, cap) per level, and counts[l] = min(count, cap).  out and
+ * counts must come from hipHostMalloc (room for levels*cap records and
+ * levels counts; ORBFE_EINVAL otherwise); they hold the result once the
+ * stream has passed this call's work.  Only the kept records cross the bus,
+ * not the cap-sized slab. */
+int orbfe_export_features_async(struct orbfe_ctx* ctx, const void* features,
+                                orbfe_feature* out, uint32_t* counts, hipStream_t stream);
+
 /* FindMatches(prev, curr) of match_features_cpu.cpp:152-207 on two device
  * feature arrays of this context's frame size: LOD+-1 candidate pools, the
  * 10% spatial window, the Lowe ratio, the mutual check.  prev_match gets

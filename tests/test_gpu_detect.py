@@ -184,3 +184,37 @@ def test_orientation_radius_and_min_run_match_oracle(torch_gpu, radius, min_run)
                      min_run=min_run)
         rf, rp = O.detect_frame(a, p, with_pyramid=True)
         _assert_same(feats, pyr, rf, rp, 6, cap)
+
+
+@pytest.mark.parametrize("W,H,cap,flat", [(752, 480, 1000, False), (1280, 720, 300, False),
+                                          (640, 480, 4000, False), (200, 150, 120, True)])
+def test_export_features_zero_copy_equals_load_features(torch_gpu, W, H, cap, flat):
+    """orbfe_export_features_async writes the FeatureSet (min(count, cap)
+    wslam::Feature records per level, counts) straight into pinned host
+    memory: equal to the oracle's loadFeatures of the oracle's feature array,
+    with levels over their cap (cap 300) and with empty levels (flat frame)."""
+    torch = torch_gpu
+    a, _, _ = synth.frame_pair(7, W, H)
+    if flat:
+        a[:] = 128
+    with orbfe.Context(W, H, features_per_level=cap, fast_threshold=thr20()) as ctx:
+        img = torch.from_numpy(a).cuda()
+        feats = torch.zeros(ctx.feature_bytes, dtype=torch.uint8, device="cuda")
+        out = torch.full((6 * cap * 13,), 0xAB, dtype=torch.int32).pin_memory()
+        counts = torch.full((6,), -1, dtype=torch.int32).pin_memory()
+        _say(f"export {W}x{H} cap {cap} flat={flat}")
+        ctx.detect_async(img, W, feats)
+        ctx.export_features_async(feats, out, counts)
+        torch.cuda.synchronize()
+        got_counts = [int(c) for c in counts.numpy()]
+        got = out.numpy().view(np.uint8).view(orbfe.FEATURE_DTYPE)[:sum(got_counts)]
+        host = feats.cpu().numpy()
+    rf, _ = O.detect_frame(a, O.params(W, H, threshold=thr20(), cap=cap))
+    assert np.array_equal(host, rf)
+    want, want_counts = O.load_features(rf, 6, cap)
+    assert got_counts == want_counts
+    assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
+    if flat:
+        assert sum(got_counts) == 0
+    else:
+        assert min(want_counts) > 0

@@ -27,30 +27,24 @@ def _feature_key(f):
             float(f["orientation"]), tuple(int(d) for d in f["descriptor"]))
 
 
-@pytest.mark.gpu
-def test_pipeline_driver_matches_oracle(tmp_path, torch_gpu):
-    W, H, N, cap, ratio = 752, 480, 4, 1000, 0.8
-    frames, _ = synth.sequence(3, W, H, n_frames=N)
-    raw = tmp_path / "frames.raw"
-    frames.tofile(raw)
-    out = tmp_path / "out.txt"
-    print(f"wslam_pipeline {W}x{H} x{N}", flush=True)
-    r = subprocess.run([os.path.join(PKG, "wslam_pipeline"), str(raw), str(W), str(H), str(N),
-                        str(cap), repr(thr20()), repr(ratio), str(out)],
-                       capture_output=True, text=True, timeout=120)
-    assert r.returncode == 0, r.stderr
-    lines = out.read_text().splitlines()
+def _expected_lines(frames, W, H, cap, ratio, advance):
+    """The driver's output as the oracle gives it, with the loader's reference
+    shift (load_features_cpu.cpp:111-142): the reference set moves to the
+    previous frame's set when the frame's advance marker is set or no
+    reference exists yet, and is held otherwise."""
     p = O.params(W, H, threshold=thr20(), cap=cap)
-    sets = []
-    for t in range(N):
-        fa, _ = O.detect_frame(frames[t], p)
-        sets.append(O.load_features(fa, 6, cap))
     want = []
-    for t in range(N):
-        want.append(f"frame {t} " + " ".join(str(c) for c in sets[t][1]))
-        if t == 0:
-            continue
-        (A, ca), (B, cb) = sets[t - 1], sets[t]
+    ref = cur = None
+    for t in range(len(frames)):
+        fa, _ = O.detect_frame(frames[t], p)
+        new = O.load_features(fa, 6, cap)
+        if not (ref is not None and advance[t] == "0") and cur is not None:
+            ref = cur
+        cur = new
+        want.append(f"frame {t} " + " ".join(str(c) for c in new[1]))
+        if ref is None:
+            continue  # no reference set: the match stage stops
+        (A, ca), (B, cb) = ref, cur
         pm, _ = O.find_matches(A, ca, B, cb, W, H, ratio)
         offs_b = np.concatenate([[0], np.cumsum(cb)])
         offs_a = np.concatenate([[0], np.cumsum(ca)])
@@ -66,4 +60,38 @@ def test_pipeline_driver_matches_oracle(tmp_path, torch_gpu):
             pairs.sort(key=lambda z: z[0])
             for _, b, a in pairs:
                 want.append(f"m {t} {l} {b['x']} {b['y']} {b['lod']} {a['x']} {a['y']}")
-    assert lines == want
+    return want
+
+
+def _run_driver(tmp_path, frames, W, H, cap, ratio, *opts):
+    raw = tmp_path / "frames.raw"
+    frames.tofile(raw)
+    out = tmp_path / "out.txt"
+    print(f"wslam_pipeline {W}x{H} x{len(frames)} {' '.join(opts)}", flush=True)
+    r = subprocess.run([os.path.join(PKG, "wslam_pipeline"), str(raw), str(W), str(H),
+                        str(len(frames)), str(cap), repr(thr20()), repr(ratio), str(out), *opts],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return out.read_text().splitlines()
+
+
+@pytest.mark.gpu
+def test_pipeline_driver_matches_oracle(tmp_path, torch_gpu):
+    W, H, N, cap, ratio = 752, 480, 4, 1000, 0.8
+    frames, _ = synth.sequence(3, W, H, n_frames=N)
+    lines = _run_driver(tmp_path, frames, W, H, cap, ratio)
+    assert lines == _expected_lines(frames, W, H, cap, ratio, "1" * N)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("opts", [("--f32",), ("--graph",), ("--f32", "--graph")])
+def test_pipeline_driver_held_keyframe_f32_and_graphs(tmp_path, torch_gpu, opts):
+    """The reference's f32 FrameBW input, the loader holding the reference
+    keyframe on frames without the advance marker, and both stages replayed
+    from captured HIP graphs (one per buffer slot combination): the driver's
+    features and MatchResult equal the oracle's on every frame."""
+    W, H, N, cap, ratio = 752, 480, 8, 1000, 0.8
+    frames, _ = synth.sequence(5, W, H, n_frames=N)
+    advance = "10100111"  # frames 1, 3, 4 keep the reference of an earlier frame
+    lines = _run_driver(tmp_path, frames, W, H, cap, ratio, *opts, "--advance", advance)
+    assert lines == _expected_lines(frames, W, H, cap, ratio, advance)

@@ -37,6 +37,8 @@ hipError_t launch_bf_mfma(const uint32_t* q, uint32_t nq, const uint32_t* t, uin
 hipError_t launch_bf_merge(const uint2* part, uint32_t nq, uint32_t splits, uint32_t* idx,
                            uint32_t* d1, uint32_t* d2, hipStream_t s);
 hipError_t bf_scratch_address(void** out);
+hipError_t launch_export(const uint8_t* feat, uint32_t levels, uint32_t cap, uint32_t stride,
+                         uint32_t* out, uint32_t* counts, hipStream_t s);
 hipError_t launch_valu_probe(uint32_t* sink, uint32_t blocks, hipStream_t s,
                              double* instr_per_launch);
 hipError_t launch_mfma_probe(uint32_t* sink, uint32_t blocks, hipStream_t s,
@@ -1076,6 +1078,27 @@ int orbfe_load_features(const void* features, uint32_t levels, uint32_t cap, orb
         }
         counts[l] = take;
     }
+    return ORBFE_OK;
+}
+
+int orbfe_export_features_async(orbfe_ctx* ctx, const void* features, orbfe_feature* out,
+                                uint32_t* counts, hipStream_t stream) {
+    if (!ctx) return ORBFE_EINVAL;
+    if (!features || !out || !counts) return fail(ctx, ORBFE_EINVAL, "null pointer");
+    // The kernel writes through the device addresses of the pinned buffers.
+    void* d_out = nullptr;
+    void* d_counts = nullptr;
+    if (hipHostGetDevicePointer(&d_out, out, 0) != hipSuccess || !d_out ||
+        hipHostGetDevicePointer(&d_counts, counts, 0) != hipSuccess || !d_counts) {
+        (void)hipGetLastError();
+        return fail(ctx, ORBFE_EINVAL, "out and counts must be pinned host memory (hipHostMalloc)");
+    }
+    const Plan& P = ctx->main->host;
+    HIP_TRY(ctx, timed(ctx, "k_export", stream, [&] {
+                return orbfe::launch_export(static_cast<const uint8_t*>(features), P.L, P.cap,
+                                            P.feat_stride, static_cast<uint32_t*>(d_out),
+                                            static_cast<uint32_t*>(d_counts), stream);
+            }));
     return ORBFE_OK;
 }
 

@@ -2006,6 +2006,56 @@ __global__ __launch_bounds__(kBmThreads) void k_bf_mfma(const uint32_t* __restri
 }
 
 // ---------------------------------------------------------------------------
+// k_export: the FeatureSet of a device feature array (the FromGpuModel loop
+// of LoadDataCPUPass::loadFeatures, load_features_cpu.cpp:164-188) written
+// by the GPU straight into pinned host memory: per level min(count, cap)
+// wslam::Feature records of 13 dwords {x, y, lod, strength, orientation,
+// descriptor[8]}, levels back to back, and the per-level counts.  Block
+// (level, chunk) writes the records [256 chunk, 256 chunk + 256) of its
+// level; its threads store consecutive dwords of that output span, so
+// every store instruction covers 1 KB of contiguous host memory.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kExportThreads = 256;
+constexpr uint32_t kFeatureDwords = 13;  // sizeof(wslam::Feature) / 4
+
+__global__ __launch_bounds__(kExportThreads) void k_export(const uint8_t* __restrict__ feat,
+                                                            uint32_t levels, uint32_t cap,
+                                                            uint32_t stride, uint32_t chunks,
+                                                            uint32_t* __restrict__ out,
+                                                            uint32_t* __restrict__ counts) {
+    const uint32_t l = blockIdx.x / chunks;
+    const uint32_t chunk = blockIdx.x - l * chunks;
+    // Records of the levels before l: the output offset of this level.
+    uint32_t base = 0, kept = 0;
+    for (uint32_t k = 0; k <= l; ++k) {
+        const uint32_t c = umin(load_u32(reinterpret_cast<const uint32_t*>(feat + k * stride), 0),
+                                cap);
+        if (k < l) base += c;
+        else kept = c;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < levels)
+        counts[threadIdx.x] =
+            umin(load_u32(reinterpret_cast<const uint32_t*>(feat + threadIdx.x * stride), 0), cap);
+    const uint32_t r0 = chunk * kExportThreads;
+    if (r0 >= kept) return;
+    const uint32_t n = umin(kept - r0, kExportThreads);
+    const uint8_t* src = feat + l * stride + 8 + 48u * r0;
+    uint32_t* dst = out + static_cast<size_t>(base + r0) * kFeatureDwords;
+    for (uint32_t e = threadIdx.x; e < n * kFeatureDwords; e += kExportThreads) {
+        const uint32_t rec = e / kFeatureDwords, field = e - rec * kFeatureDwords;
+        // source dword of the 48-byte record: x, y, strength, orientation,
+        // descriptor[8]; field 2 (lod) has none
+        const uint32_t sw = field < 2 ? field : (field == 2 ? 0 : field - 1);
+        const uint32_t v = load_u32(reinterpret_cast<const uint32_t*>(src + 48u * rec), sw);
+        uint32_t o;
+        if (field < 2) o = static_cast<uint32_t>(__uint_as_float(v));  // u32(f32), models.hpp:113-120
+        else if (field == 2) o = l;
+        else o = v;
+        dst[e] = o;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Host-side launchers (called from orbfe_api.cpp).
 // ---------------------------------------------------------------------------
 // Gather order of the describe kernel (g_desc_order): for each angle bin,
@@ -2193,6 +2243,14 @@ hipError_t launch_bf_merge(const uint2* part, uint32_t nq, uint32_t splits, uint
                            uint32_t* d1, uint32_t* d2, hipStream_t s) {
     hipLaunchKernelGGL(k_bf_merge, dim3((nq + 255) / 256), dim3(256), 0, s, part, nq, splits, idx,
                        d1, d2);
+    return hipGetLastError();
+}
+
+hipError_t launch_export(const uint8_t* feat, uint32_t levels, uint32_t cap, uint32_t stride,
+                         uint32_t* out, uint32_t* counts, hipStream_t s) {
+    const uint32_t chunks = (cap + kExportThreads - 1) / kExportThreads;
+    hipLaunchKernelGGL(k_export, dim3(levels * chunks), dim3(kExportThreads), 0, s, feat, levels,
+                       cap, stride, chunks, out, counts);
     return hipGetLastError();
 }
 

@@ -6,14 +6,16 @@
 // synchronisation; CPU passes between them flush the batch first, in
 // insertion order.
 //
-// With graph capture enabled, each HIP batch is captured into a HIP graph on
-// its first execution and replayed afterwards.  That is only valid for
-// passes that enqueue identical work every time (fixed buffers, fixed
-// sizes); the graph is recaptured whenever a pass of the batch reports a
-// change through HIPPass-independent means (resetGraphs()).
+// With graph capture enabled, each HIP batch is captured into a HIP graph
+// the first time it runs under a given combination of its passes'
+// graphKey()s, and that graph is replayed whenever the combination comes
+// back.  The passes' prepareHost() runs before every execution, captured or
+// replayed, outside the captured region; prepareExecute runs only while
+// capturing.  resetGraphs() drops every graph.
 
 #include <hip/hip_runtime_api.h>
 
+#include <map>
 #include <memory>
 #include <optional>
 #include <string>
@@ -50,7 +52,9 @@ class HipStage : public Stage {
     std::vector<PassPtr> passes_;
     std::shared_ptr<Device> device_;
     bool capture_ = false;
-    std::vector<hipGraphExec_t> graphs_;  // one per HIP batch position
+    // per HIP batch position: graph of each key combination (nullptr when
+    // the batch enqueued nothing under it)
+    std::vector<std::map<uint64_t, hipGraphExec_t>> graphs_;
 };
 
 }  // namespace wslam::compute

@@ -11,6 +11,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <cstdint>
 #include <functional>
 #include <memory>
 #include <optional>
@@ -38,6 +39,16 @@ class HIPPass {
     [[nodiscard]] virtual std::string getId() const = 0;
     [[nodiscard]] virtual std::optional<std::string> initialize() = 0;
     [[nodiscard]] virtual std::optional<std::string> prepareExecute(hipStream_t stream) = 0;
+
+    // Host work of one execution that must happen every time and outside
+    // any captured region (e.g. staging the frame into pinned memory).  The
+    // stage calls it before prepareExecute, and before each replay of a
+    // captured graph in place of prepareExecute.
+    [[nodiscard]] virtual std::optional<std::string> prepareHost() { return std::nullopt; }
+    // Names the device work prepareExecute would enqueue now (which buffers,
+    // which sizes): equal keys mean identical work, so a HipStage with graph
+    // capture replays the graph it captured under the same key.
+    [[nodiscard]] virtual uint64_t graphKey() const { return 0; }
 
    protected:
     std::shared_ptr<Device> device_;

@@ -31,4 +31,17 @@ struct FrameU8 {
     std::vector<uint8_t> pixels;  // row-major, pitch = width
 };
 
+namespace data {
+// The reference's grayscale frame (data/include/data/provider_base.hpp:57-62),
+// published by its sensor loader under res:frame:<i>:lod:0
+// (wslam/src/feature_detect/sensor_loader.cpp:105): f32 pixels
+// float(u8) / 255 (data/src/provider_base.cpp:121-124).
+struct FrameBW {
+    uint64_t timestamp;
+    std::vector<float> pixels;
+    uint16_t width;
+    uint16_t height;
+};
+}  // namespace data
+
 }  // namespace wslam

@@ -16,7 +16,11 @@
 //
 // The passes share a FeatureBuffers object: the device feature arrays of the
 // current set (gen:feat:-0), the reference set (gen:feat:-1) and the frame
-// being detected, plus pinned staging.  Keys and layouts are the reference's.
+// being detected, plus pinned staging (the frame, the exported FeatureSet,
+// the match table).  Keys and layouts are the reference's.  Every pass
+// enqueues the same work for the same buffer slots, so the stages may run
+// with HIP graph capture (HipStage::enableGraphCapture): one graph per slot
+// combination, the frame staged on the host before each replay.
 
 #include <hip/hip_runtime_api.h>
 
@@ -69,9 +73,17 @@ class FeatureBuffers {
     // detected array becomes current.
     void rotate(bool advance_reference);
 
+    // Slot of each device array (0..2), for graph keys.
+    int nextSlot() const { return next_; }
+    int currentSlot() const { return cur_; }
+    int referenceSlot() const { return ref_; }
+
     uint8_t* hostFrame() const { return host_frame_; }
     uint8_t* deviceFrame() const { return dev_frame_; }
-    uint8_t* hostFeatures() const { return host_features_; }
+    // FeatureSet of the last detected frame, written by the GPU into pinned
+    // memory (orbfe_export_features_async): records level by level, counts.
+    const Feature* hostSet() const { return reinterpret_cast<const Feature*>(host_set_); }
+    const uint32_t* hostCounts() const { return host_counts_; }
     int32_t* deviceMatch() const { return dev_match_; }
     int32_t* hostMatch() const { return host_match_; }
 
@@ -86,7 +98,8 @@ class FeatureBuffers {
     bool has_ref_ = false, has_cur_ = false;
     uint8_t* host_frame_ = nullptr;
     uint8_t* dev_frame_ = nullptr;
-    uint8_t* host_features_ = nullptr;
+    orbfe_feature* host_set_ = nullptr;
+    uint32_t* host_counts_ = nullptr;
     int32_t* dev_match_ = nullptr;
     int32_t* host_match_ = nullptr;
 };
@@ -97,12 +110,20 @@ class FeatureDetectHipPass : public compute::HIPPass {
                          std::shared_ptr<FeatureBuffers> buffers);
     [[nodiscard]] std::string getId() const override;
     [[nodiscard]] std::optional<std::string> initialize() override;
+    [[nodiscard]] std::optional<std::string> prepareHost() override;
     [[nodiscard]] std::optional<std::string> prepareExecute(hipStream_t stream) override;
+    [[nodiscard]] uint64_t graphKey() const override;
 
    private:
     compute::Store& storage_;
     std::shared_ptr<FeatureBuffers> buffers_;
 };
+
+// Frame of the store to u8 pixels: FrameU8 as is, or the reference's f32
+// FrameBW, whose pixels are exactly float(k) / 255 (k = round(v * 255),
+// checked per pixel; other values are refused).  Returns the error text.
+std::optional<std::string> StageFrameU8(const compute::Store& storage, const std::string& key,
+                                        uint32_t width, uint32_t height, uint8_t* out);
 
 class LoadFeaturesHipPass : public compute::Pass {
    public:
@@ -124,6 +145,7 @@ class MatchFeaturesHipPass : public compute::HIPPass {
     [[nodiscard]] std::string getId() const override;
     [[nodiscard]] std::optional<std::string> initialize() override;
     [[nodiscard]] std::optional<std::string> prepareExecute(hipStream_t stream) override;
+    [[nodiscard]] uint64_t graphKey() const override;
 
    private:
     std::shared_ptr<FeatureBuffers> buffers_;

@@ -86,8 +86,9 @@ void HipStage::add_pass(std::unique_ptr<Pass> pass) { passes_.emplace_back(std::
 void HipStage::add_pass(std::unique_ptr<HIPPass> pass) { passes_.emplace_back(std::move(pass)); }
 
 void HipStage::resetGraphs() {
-    for (auto g : graphs_)
-        if (g) (void)hipGraphExecDestroy(g);
+    for (auto& by_key : graphs_)
+        for (auto& kv : by_key)
+            if (kv.second) (void)hipGraphExecDestroy(kv.second);
     graphs_.clear();
 }
 
@@ -102,15 +103,29 @@ std::optional<std::string> HipStage::initialize() {
 std::optional<std::string> HipStage::executeHIPBatch(std::vector<HIPPass*>& batch,
                                                      size_t batch_index) {
     if (batch.empty()) return std::nullopt;
+    for (auto* pass : batch) {
+        if (auto err = pass->prepareHost()) return "preparing pass " + pass->getId() + ": " + *err;
+    }
     const hipStream_t s = device_->stream();
     const bool capture = capture_ && !device_->hostOnly();
-    if (capture && batch_index < graphs_.size() && graphs_[batch_index]) {
-        const hipError_t e = hipGraphLaunch(graphs_[batch_index], s);
-        if (e != hipSuccess) return hip_message("hipGraphLaunch", e);
-        return device_->synchronize();
-    }
+    uint64_t key = 0xcbf29ce484222325ull;  // FNV-1a over the passes' keys
     if (capture) {
-        const hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        for (auto* pass : batch) {
+            const uint64_t k = pass->graphKey();
+            for (int b = 0; b < 8; ++b) key = (key ^ ((k >> (8 * b)) & 0xFFu)) * 0x100000001b3ull;
+        }
+        if (graphs_.size() <= batch_index) graphs_.resize(batch_index + 1);
+        const auto it = graphs_[batch_index].find(key);
+        if (it != graphs_[batch_index].end()) {
+            if (it->second) {
+                const hipError_t e = hipGraphLaunch(it->second, s);
+                if (e != hipSuccess) return hip_message("hipGraphLaunch", e);
+            }
+            return device_->synchronize();
+        }
+        // Relaxed: the passes' library calls may make non-stream HIP calls
+        // (pointer queries) while the stream records.
+        const hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed);
         if (e != hipSuccess) return hip_message("hipStreamBeginCapture", e);
     }
     std::optional<std::string> failure;
@@ -128,14 +143,26 @@ std::optional<std::string> HipStage::executeHIPBatch(std::vector<HIPPass*>& batc
             return failure;
         }
         if (e != hipSuccess) return hip_message("hipStreamEndCapture", e);
+        size_t nodes = 0;
+        hipError_t en = hipGraphGetNodes(graph, nullptr, &nodes);
+        if (en != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            return hip_message("hipGraphGetNodes", en);
+        }
         hipGraphExec_t exec = nullptr;
-        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (ei != hipSuccess) return hip_message("hipGraphInstantiate", ei);
-        if (graphs_.size() <= batch_index) graphs_.resize(batch_index + 1, nullptr);
-        graphs_[batch_index] = exec;
-        const hipError_t el = hipGraphLaunch(exec, s);
-        if (el != hipSuccess) return hip_message("hipGraphLaunch", el);
+        if (nodes > 0) {
+            const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ei != hipSuccess) return hip_message("hipGraphInstantiate", ei);
+            const hipError_t el = hipGraphLaunch(exec, s);
+            if (el != hipSuccess) {
+                (void)hipGraphExecDestroy(exec);
+                return hip_message("hipGraphLaunch", el);
+            }
+        } else {
+            (void)hipGraphDestroy(graph);
+        }
+        graphs_[batch_index].emplace(key, exec);
     } else if (failure) {
         return failure;
     }

@@ -3,9 +3,18 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <cmath>
+#include <cstddef>
 #include <cstring>
 
 namespace wslam {
+
+// orbfe_feature is the C form of wslam::Feature: the exported records are
+// the FeatureSet's elements byte for byte.
+static_assert(sizeof(orbfe_feature) == sizeof(Feature));
+static_assert(offsetof(orbfe_feature, lod) == offsetof(Feature, lod));
+static_assert(offsetof(orbfe_feature, orientation) == offsetof(Feature, orientation));
+static_assert(offsetof(orbfe_feature, descriptor) == offsetof(Feature, descriptor));
 
 namespace {
 std::string hip_msg(const char* what, hipError_t e) {
@@ -28,7 +37,8 @@ FeatureBuffers::~FeatureBuffers() {
     if (dev_frame_) (void)hipFree(dev_frame_);
     if (dev_match_) (void)hipFree(dev_match_);
     if (host_frame_) (void)hipHostFree(host_frame_);
-    if (host_features_) (void)hipHostFree(host_features_);
+    if (host_set_) (void)hipHostFree(host_set_);
+    if (host_counts_) (void)hipHostFree(host_counts_);
     if (host_match_) (void)hipHostFree(host_match_);
     if (ctx_) orbfe_destroy(ctx_);
 }
@@ -62,7 +72,11 @@ std::optional<std::string> FeatureBuffers::initialize() {
     if ((e = hipHostMalloc(reinterpret_cast<void**>(&host_frame_), frame_bytes,
                            hipHostMallocDefault)) != hipSuccess)
         return hip_msg("hipHostMalloc", e);
-    if ((e = hipHostMalloc(reinterpret_cast<void**>(&host_features_), feature_bytes_,
+    if ((e = hipHostMalloc(reinterpret_cast<void**>(&host_set_),
+                           sizeof(orbfe_feature) * kLevelsOfDetail * cap_,
+                           hipHostMallocDefault)) != hipSuccess)
+        return hip_msg("hipHostMalloc", e);
+    if ((e = hipHostMalloc(reinterpret_cast<void**>(&host_counts_), 4 * kLevelsOfDetail,
                            hipHostMallocDefault)) != hipSuccess)
         return hip_msg("hipHostMalloc", e);
     if ((e = hipHostMalloc(reinterpret_cast<void**>(&host_match_), match_bytes,
@@ -96,28 +110,69 @@ std::string FeatureDetectHipPass::getId() const { return "[Feature detect HIP pa
 
 std::optional<std::string> FeatureDetectHipPass::initialize() { return buffers_->initialize(); }
 
-std::optional<std::string> FeatureDetectHipPass::prepareExecute(hipStream_t stream) {
-    const auto name = ResourceIdentifier::GetFrameName(0);
-    auto frame = storage_.getPtr<FrameU8>(name);
-    if (!frame) return "end of data (no frame under " + name + ")";
-    const FrameU8& f = **frame;
+std::optional<std::string> StageFrameU8(const compute::Store& storage, const std::string& key,
+                                        uint32_t width, uint32_t height, uint8_t* out) {
+    const auto size_msg = [&](uint32_t w, uint32_t h) {
+        return "frame is " + std::to_string(w) + "x" + std::to_string(h) +
+               ", the stage was built for " + std::to_string(width) + "x" +
+               std::to_string(height);
+    };
+    const size_t n = static_cast<size_t>(width) * height;
+    if (auto u8 = storage.getPtr<FrameU8>(key)) {
+        const FrameU8& f = **u8;
+        if (f.width != width || f.height != height) return size_msg(f.width, f.height);
+        if (f.pixels.size() != n) return "data is empty";
+        std::memcpy(out, f.pixels.data(), n);
+        return std::nullopt;
+    }
+    if (auto bw = storage.getPtr<data::FrameBW>(key)) {
+        const data::FrameBW& f = **bw;
+        if (f.width != width || f.height != height) return size_msg(f.width, f.height);
+        if (f.pixels.size() != n) return "data is empty";
+        // The loader's pixels are fl(k / 255) for a u8 k (provider_base.cpp:121-124):
+        // k = round(v * 255) recovers k, and fl(k / 255) == v confirms it.
+        float unit[256];
+        for (int k = 0; k < 256; ++k) unit[k] = static_cast<float>(k) / 255.0f;
+        bool ok = true;
+        for (size_t i = 0; i < n; ++i) {
+            const float v = f.pixels[i];
+            const float r = std::nearbyint(v * 255.0f);
+            const uint32_t k = r >= 0.0f && r <= 255.0f ? static_cast<uint32_t>(r) : 0u;
+            ok &= unit[k] == v;
+            out[i] = static_cast<uint8_t>(k);
+        }
+        if (!ok) return "frame pixels are not float(u8) / 255 values";
+        return std::nullopt;
+    }
+    return "end of data (no frame under " + key + ")";
+}
+
+std::optional<std::string> FeatureDetectHipPass::prepareHost() {
     const auto& o = buffers_->opts();
-    if (f.width != o.frame_width || f.height != o.frame_height)
-        return "frame is " + std::to_string(f.width) + "x" + std::to_string(f.height) +
-               ", the stage was built for " + std::to_string(o.frame_width) + "x" +
-               std::to_string(o.frame_height);
-    if (f.pixels.size() != static_cast<size_t>(f.width) * f.height) return "data is empty";
-    std::memcpy(buffers_->hostFrame(), f.pixels.data(), f.pixels.size());
-    hipError_t e = hipMemcpyAsync(buffers_->deviceFrame(), buffers_->hostFrame(),
-                                  f.pixels.size(), hipMemcpyHostToDevice, stream);
+    return StageFrameU8(storage_, ResourceIdentifier::GetFrameName(0), o.frame_width,
+                        o.frame_height, buffers_->hostFrame());
+}
+
+std::optional<std::string> FeatureDetectHipPass::prepareExecute(hipStream_t stream) {
+    const auto& o = buffers_->opts();
+    const size_t bytes = static_cast<size_t>(o.frame_width) * o.frame_height;
+    hipError_t e = hipMemcpyAsync(buffers_->deviceFrame(), buffers_->hostFrame(), bytes,
+                                  hipMemcpyHostToDevice, stream);
     if (e != hipSuccess) return hip_msg("upload frame", e);
-    const int rc = orbfe_detect_async(buffers_->ctx(), buffers_->deviceFrame(), f.width, nullptr,
-                                      buffers_->next(), stream);
+    int rc = orbfe_detect_async(buffers_->ctx(), buffers_->deviceFrame(), o.frame_width, nullptr,
+                                buffers_->next(), stream);
     if (rc != ORBFE_OK) return orbfe_msg("orbfe_detect_async", buffers_->ctx(), rc);
-    e = hipMemcpyAsync(buffers_->hostFeatures(), buffers_->next(), buffers_->featureBytes(),
-                       hipMemcpyDeviceToHost, stream);
-    if (e != hipSuccess) return hip_msg("read back features", e);
+    // The FeatureSet straight into pinned memory (no slab copy, no CPU loop).
+    rc = orbfe_export_features_async(
+        buffers_->ctx(), buffers_->next(),
+        reinterpret_cast<orbfe_feature*>(const_cast<Feature*>(buffers_->hostSet())),
+        const_cast<uint32_t*>(buffers_->hostCounts()), stream);
+    if (rc != ORBFE_OK) return orbfe_msg("orbfe_export_features_async", buffers_->ctx(), rc);
     return std::nullopt;
+}
+
+uint64_t FeatureDetectHipPass::graphKey() const {
+    return static_cast<uint64_t>(buffers_->nextSlot());
 }
 
 // --- LoadFeaturesHipPass -------------------------------------------------------------
@@ -130,22 +185,15 @@ std::string LoadFeaturesHipPass::getId() const { return "[Load features HIP pass
 std::optional<std::string> LoadFeaturesHipPass::initialize() { return std::nullopt; }
 
 std::optional<std::string> LoadFeaturesHipPass::execute() {
-    // loadFeatures (load_features_cpu.cpp:164-188): min(count, cap) per level.
+    // loadFeatures (load_features_cpu.cpp:164-188): the GPU already wrote
+    // min(count, cap) wslam::Feature records per level into pinned memory.
     FeatureSet result;
-    const uint32_t cap = buffers_->cap();
-    const uint8_t* base = buffers_->hostFeatures();
-    const size_t stride = gpumodels::FeatureBlockBytes(cap);
+    const Feature* rec = buffers_->hostSet();
+    const uint32_t* counts = buffers_->hostCounts();
     for (uint32_t lod = 0; lod < kLevelsOfDetail; ++lod) {
-        const uint8_t* block = base + stride * lod;
-        uint32_t count;
-        std::memcpy(&count, block, 4);
-        const uint32_t take = count < cap ? count : cap;
-        result[lod].reserve(take);
-        for (uint32_t i = 0; i < take; ++i) {
-            gpumodels::Feature g;
-            std::memcpy(&g, block + 8 + sizeof(gpumodels::Feature) * i, sizeof(g));
-            result[lod].push_back(FromGpuModel(g, lod));
-        }
+        if (counts[lod] > buffers_->cap()) return "exported level count exceeds the cap";
+        result[lod].assign(rec, rec + counts[lod]);
+        rec += counts[lod];
     }
     // shiftFeatureSets (load_features_cpu.cpp:111-142)
     const bool advance_requested =
@@ -187,6 +235,12 @@ std::optional<std::string> MatchFeaturesHipPass::prepareExecute(hipStream_t stre
                        hipMemcpyDeviceToHost, stream);
     if (e != hipSuccess) return hip_msg("read back matches", e);
     return std::nullopt;
+}
+
+uint64_t MatchFeaturesHipPass::graphKey() const {
+    if (!buffers_->hasReference()) return 0;  // enqueues nothing
+    return 1u | static_cast<uint64_t>(buffers_->referenceSlot()) << 4 |
+           static_cast<uint64_t>(buffers_->currentSlot()) << 8;
 }
 
 // --- PublishMatchesPass -----------------------------------------------------------------

@@ -3,10 +3,18 @@
 // frame, the feature counts and the MatchResult in its map order.
 //
 //   wslam_pipeline FRAMES.raw W H N CAP THRESHOLD RATIO OUT.txt
+//                  [--f32] [--graph] [--advance BITS]
 //
-// FRAMES.raw holds N frames of W*H u8 pixels back to back.  Every frame asks
-// the loader to advance the reference set (frame-to-frame matching, as the
-// reference does before its map origin is set).
+// FRAMES.raw holds N frames of W*H u8 pixels back to back.
+//   --f32      publish each frame as the reference's FrameBW (f32 pixels
+//              u8 / 255, data/src/provider_base.cpp:121-124) instead of FrameU8;
+//   --graph    run both stages with HIP graph capture (one graph per buffer
+//              slot combination, replayed);
+//   --advance  BITS has one character per frame: '1' sets the loader's
+//              advance marker for that frame (the reference set moves to the
+//              previous frame), '0' leaves it unset (the reference keyframe is
+//              held, load_features_cpu.cpp:111-142).  Default: all '1'
+//              (frame-to-frame matching, as before the map origin is set).
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -19,12 +27,32 @@
 using namespace wslam;
 
 int main(int argc, char** argv) {
-    if (argc != 9) {
-        std::cerr << "usage: wslam_pipeline FRAMES.raw W H N CAP THRESHOLD RATIO OUT.txt\n";
+    if (argc < 9) {
+        std::cerr << "usage: wslam_pipeline FRAMES.raw W H N CAP THRESHOLD RATIO OUT.txt "
+                     "[--f32] [--graph] [--advance BITS]\n";
         return 2;
     }
     const std::string in = argv[1];
     const uint32_t W = std::stoul(argv[2]), H = std::stoul(argv[3]), N = std::stoul(argv[4]);
+    bool f32 = false, graph = false;
+    std::string advance(N, '1');
+    for (int i = 9; i < argc; ++i) {
+        const std::string a = argv[i];
+        if (a == "--f32") {
+            f32 = true;
+        } else if (a == "--graph") {
+            graph = true;
+        } else if (a == "--advance" && i + 1 < argc) {
+            advance = argv[++i];
+            if (advance.size() != N) {
+                std::cerr << "--advance needs one character per frame\n";
+                return 2;
+            }
+        } else {
+            std::cerr << "unknown option " << a << "\n";
+            return 2;
+        }
+    }
     FeatureDetectOpts opts;
     opts.frame_width = W;
     opts.frame_height = H;
@@ -44,8 +72,12 @@ int main(int argc, char** argv) {
     }
     compute::Compute comp;
     auto buffers = std::make_shared<FeatureBuffers>(device, opts);
-    comp.addStage(CreateFeatureDetectStage(device, comp.getStorage(), buffers));
-    comp.addStage(CreateMatchStage(device, comp.getStorage(), buffers));
+    auto detect = CreateFeatureDetectStage(device, comp.getStorage(), buffers);
+    auto match = CreateMatchStage(device, comp.getStorage(), buffers);
+    detect->enableGraphCapture(graph);
+    match->enableGraphCapture(graph);
+    comp.addStage(std::move(detect));
+    comp.addStage(std::move(match));
     if (auto e = comp.initialize()) {
         std::cerr << "initialize: " << *e << "\n";
         return 1;
@@ -63,8 +95,20 @@ int main(int argc, char** argv) {
             std::cerr << "short read at frame " << t << "\n";
             return 1;
         }
-        store.set(ResourceIdentifier::GetFrameName(0), std::move(frame));
-        store.set(ResourceIdentifier::FeatureReferenceAdvanceName, true);
+        store.erase(ResourceIdentifier::GetFrameName(0));
+        if (f32) {
+            data::FrameBW bw;
+            bw.timestamp = t;
+            bw.width = static_cast<uint16_t>(W);
+            bw.height = static_cast<uint16_t>(H);
+            bw.pixels.resize(frame.pixels.size());
+            for (size_t i = 0; i < frame.pixels.size(); ++i)
+                bw.pixels[i] = static_cast<float>(frame.pixels[i]) / 255.0f;
+            store.set(ResourceIdentifier::GetFrameName(0), std::move(bw));
+        } else {
+            store.set(ResourceIdentifier::GetFrameName(0), std::move(frame));
+        }
+        if (advance[t] == '1') store.set(ResourceIdentifier::FeatureReferenceAdvanceName, true);
         if (auto e = comp.execute()) {
             std::cerr << "frame " << t << ": " << *e << "\n";
             return 1;

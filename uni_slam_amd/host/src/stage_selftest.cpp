@@ -8,6 +8,7 @@
 
 #include "compute/compute.hpp"
 #include "compute/hip_stage.hpp"
+#include "wslam/feature_detect_hip.hpp"
 
 using namespace wslam::compute;
 
@@ -100,6 +101,30 @@ int main() {
         expect(s.get<int>("k").value_or(0) == 5, "store get");
         expect(!s.get<float>("k"), "store type check");
         expect(s.take<int>("k").value_or(0) == 5 && !s.has("k"), "store take erases");
+    }
+    {
+        // The reference's f32 frame (every u8 value as float(k) / 255)
+        // stages to the same bytes as the u8 frame; a value that is not of
+        // that form, a wrong size and a missing frame are refused.
+        using namespace wslam;
+        Store s;
+        data::FrameBW bw{7, {}, 16, 16};
+        for (int k = 0; k < 256; ++k) bw.pixels.push_back(static_cast<float>(k) / 255.0f);
+        s.set("f", bw);
+        uint8_t out[256];
+        expect(!StageFrameU8(s, "f", 16, 16, out), "f32 frame stages");
+        bool same = true;
+        for (int k = 0; k < 256; ++k) same &= out[k] == k;
+        expect(same, "f32 frame -> u8 recovers every value");
+        bw.pixels[77] = 0.5f;  // 127.5 / 255: not a u8 value
+        s.set("g", bw);
+        expect(StageFrameU8(s, "g", 16, 16, out).has_value(), "non-u8 pixel refused");
+        expect(StageFrameU8(s, "f", 32, 8, out).has_value(), "wrong size refused");
+        expect(StageFrameU8(s, "missing", 16, 16, out).value_or("").find("end of data") == 0,
+               "missing frame is end of data");
+        FrameU8 u{16, 16, 0, std::vector<uint8_t>(256, 9)};
+        s.set("u", u);
+        expect(!StageFrameU8(s, "u", 16, 16, out) && out[200] == 9, "u8 frame stages");
     }
     if (failures == 0) std::cout << "stage selftest ok\n";
     return failures == 0 ? 0 : 1;

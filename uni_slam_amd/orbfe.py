@@ -73,6 +73,8 @@ _SIGNATURES = [
     ("orbfe_batch_destroy", None, [C.c_void_p]),
     ("orbfe_load_features", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
                                       C.POINTER(C.c_uint32)]),
+    ("orbfe_export_features_async", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_void_p]),
     ("orbfe_match_pipeline_async", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                              C.c_void_p]),
     ("orbfe_match_pipeline_batch_async", C.c_int, [C.c_void_p, C.POINTER(MatchPair),
@@ -225,6 +227,12 @@ class Context:
         h = C.c_void_p()
         self._check(lib().orbfe_batch_create(self._h, arr, len(frames), C.byref(h)))
         return Batch(self, h)
+
+    def export_features_async(self, features, out, counts, stream=None):
+        """FeatureSet of a device feature array into pinned host buffers
+        (out: levels*cap FEATURE_DTYPE records, counts: levels u32)."""
+        self._check(lib().orbfe_export_features_async(self._h, _ptr(features), _ptr(out),
+                                                      _ptr(counts), _stream(stream)))
 
     def match_pipeline_async(self, prev_features, curr_features, prev_match, stream=None):
         self._check(lib().orbfe_match_pipeline_async(self._h, _ptr(prev_features),
