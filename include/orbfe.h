@@ -252,6 +252,108 @@ int orbfe_match_bf_noalloc_async(struct orbfe_ctx* ctx, const uint32_t* query, u
                                  const uint32_t* train, uint32_t nt, uint32_t* idx,
                                  uint32_t* d1, uint32_t* d2, hipStream_t stream);
 
+/* Device scratch (bytes) that one frame of width x height with `cap`
+ * features per level takes in a dynamic batch of this context (detect and
+ * match stages, pyramid not exported): what a serving loop admits against
+ * its HBM budget.  Creates the frame size's plan if needed. */
+int orbfe_frame_scratch_bytes(struct orbfe_ctx* ctx, uint32_t width, uint32_t height,
+                              uint32_t cap, size_t* bytes);
+
+/* ---------------------------------------------------------------------------
+ * Served form: continuous batching of frames from many camera streams on one
+ * GPU.  The reference runs one frame per stage submission from one host loop
+ * (wslam/src/main.cpp:64-135, feature_detect.cpp:43-86); this is that loop
+ * when many streams share the device.  Per stream a ring of
+ * 2 * frames_per_stream + 2 slots stays resident in HBM (upload buffer,
+ * feature array, match table; frame k of a stream uses slot k mod slots), so
+ * each stream's previous-frame descriptors stay on the device.  Each launch
+ * admits the oldest pending frames (FIFO over all streams, at most
+ * frames_per_stream consecutive frames of one stream, each stream in order)
+ * within max_batch, the HBM budget (orbfe_frame_scratch_bytes per frame) and
+ * the free in-flight slots, uploads them from pinned host memory and runs
+ * one batched detect and one batched match (each frame against its stream's
+ * previous frame) on that slot's HIP stream; a batch's uploads overlap the
+ * kernels of the batch before it.  Every frame's features and match table
+ * equal the plain per-frame path's (orbfe_detect_async +
+ * orbfe_match_pipeline_async) bit for bit.  Create allocates every buffer and
+ * warms every plan, arena and staging slot; serving allocates no device
+ * memory.  One host thread drives a server.
+ * ------------------------------------------------------------------------- */
+typedef struct orbfe_stream_spec {
+    uint32_t width;
+    uint32_t height;
+    uint32_t cap;       /* features per level; 0 = width*height/250 */
+    uint32_t reserved;
+} orbfe_stream_spec;
+
+typedef struct orbfe_server_config {
+    orbfe_config detect;        /* tunables and launch shapes of every stream
+                                   (frame size and cap fields are per stream) */
+    size_t hbm_budget;          /* device scratch of the frames in flight; 0 = 1 GiB */
+    uint32_t max_batch;         /* frames per batch; 0 = 64 */
+    uint32_t inflight;          /* batches on the device at once (one HIP
+                                   stream each), 1..8; 0 = 2 */
+    uint32_t frames_per_stream; /* consecutive frames of one stream per batch; 0 = 3 */
+    uint32_t log_batches;       /* nonzero: record every batch's frames
+                                   (orbfe_server_batch_log) */
+} orbfe_server_config;
+
+typedef struct orbfe_arrival {
+    double t;          /* arrival time, seconds from the start of the schedule */
+    uint32_t stream;
+    uint32_t seq;      /* frame number within the stream (picks the pool frame) */
+} orbfe_arrival;
+
+typedef struct orbfe_completion {
+    uint32_t stream;
+    uint32_t seq;      /* frame number within the stream, in submission order */
+    double t_arrival;  /* seconds on the server clock */
+    double t_done;     /* when the server saw the frame's batch complete */
+} orbfe_completion;
+
+int orbfe_server_config_default(orbfe_server_config* cfg);
+int orbfe_server_create(const orbfe_server_config* cfg, const orbfe_stream_spec* streams,
+                        uint32_t n_streams, int device, struct orbfe_server** out);
+void orbfe_server_destroy(struct orbfe_server* srv);
+const char* orbfe_server_last_error(const struct orbfe_server* srv);
+/* Seconds since the server was created (steady clock): the time base of
+ * t_arrival and t_done. */
+double orbfe_server_clock(const struct orbfe_server* srv);
+/* Queues one frame of `stream` (width*height u8 in pinned host memory, valid
+ * until the frame completes).  features_out / match_out (nullable, pinned
+ * host memory) receive the frame's feature array and its match table against
+ * the stream's previous frame (seq > 0) by the time it completes.  *seq (nullable)
+ * gets the frame's number within the stream. */
+int orbfe_server_submit(struct orbfe_server* srv, uint32_t stream, const uint8_t* frame,
+                        double t_arrival, void* features_out, int32_t* match_out,
+                        uint32_t* seq);
+/* Retires finished batches and launches as many new ones as the in-flight
+ * slots allow; hands out up to max_done completions (n_done of them) and the
+ * number of batches launched.  Never blocks. */
+int orbfe_server_pump(struct orbfe_server* srv, orbfe_completion* done, uint32_t max_done,
+                      uint32_t* n_done, uint32_t* launched);
+/* 1 when nothing is pending, in flight or waiting to be handed out. */
+int orbfe_server_idle(const struct orbfe_server* srv);
+int orbfe_server_stats(const struct orbfe_server* srv, uint64_t* batches, uint32_t* max_batch);
+/* With log_batches: the frames of every launched batch, in launch order
+ * (batch number, stream, seq, submission number); *n gets the total. */
+typedef struct orbfe_batch_entry {
+    uint32_t batch;
+    uint32_t stream;
+    uint32_t seq;
+    uint32_t order;
+} orbfe_batch_entry;
+int orbfe_server_batch_log(const struct orbfe_server* srv, orbfe_batch_entry* out, size_t max,
+                           size_t* n);
+/* Replays a schedule: arrival i (sorted by t, each stream's in seq order) is
+ * submitted at t * time_scale seconds after the call starts (time_scale 0:
+ * every frame waiting at once), with frame pool[stream * pool_per_stream +
+ * seq % pool_per_stream]; pumps until every frame completed.  done gets n
+ * completions in completion order; *wall_s the elapsed seconds. */
+int orbfe_server_run(struct orbfe_server* srv, const orbfe_arrival* arrivals, uint32_t n,
+                     const uint8_t* const* pool, uint32_t pool_per_stream, double time_scale,
+                     orbfe_completion* done, double* wall_s);
+
 /* Kernel timing: when enabled, every kernel launch of the context is
  * bracketed by HIP events on its stream; orbfe_timing_read synchronizes and
  * returns, for kernel `name`, the launch count and summed duration (ms). */

@@ -24,7 +24,9 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_bindings_cover_the_header():
-    assert sorted(orbfe.ENTRY_POINTS) == _declared()
+    from uni_slam_amd import serve
+    assert sorted(orbfe.ENTRY_POINTS + serve.ENTRY_POINTS) == _declared()
+    serve._lib()  # every server entry point binds
 
 
 def test_config_default_matches_reference_constants():

@@ -95,3 +95,24 @@ def test_pipeline_driver_held_keyframe_f32_and_graphs(tmp_path, torch_gpu, opts)
     advance = "10100111"  # frames 1, 3, 4 keep the reference of an earlier frame
     lines = _run_driver(tmp_path, frames, W, H, cap, ratio, *opts, "--advance", advance)
     assert lines == _expected_lines(frames, W, H, cap, ratio, advance)
+
+
+@pytest.mark.gpu
+def test_native_server_driver_equals_plain(tmp_path, torch_gpu):
+    """The served form driven from C++ (wslam_serve): 5 streams at all four
+    resolutions, 4 frames each, submitted at once and batched across
+    streams; every frame's features and match table equal the plain
+    per-frame path's."""
+    streams = [(640, 480, 500, 31), (752, 480, 800, 32), (1280, 720, 1500, 33),
+               (1920, 1080, 3000, 34), (752, 480, 1200, 35)]
+    args = []
+    for w, h, cap, seed in streams:
+        fr, _ = synth.sequence(seed, w, h, n_frames=4)
+        path = tmp_path / f"s{seed}.raw"
+        fr.tofile(path)
+        args += [str(w), str(h), str(cap), str(path)]
+    print("wslam_serve 5 streams x 4 frames", flush=True)
+    r = subprocess.run([os.path.join(PKG, "wslam_serve"), "4", repr(thr20()), "0.8", *args],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "every frame equals the plain path" in r.stdout

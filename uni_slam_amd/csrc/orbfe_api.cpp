@@ -1081,6 +1081,18 @@ int orbfe_load_features(const void* features, uint32_t levels, uint32_t cap, orb
     return ORBFE_OK;
 }
 
+int orbfe_frame_scratch_bytes(orbfe_ctx* ctx, uint32_t width, uint32_t height, uint32_t cap,
+                              size_t* bytes) {
+    if (!ctx || !bytes) return ORBFE_EINVAL;
+    PlanEntry* pe = nullptr;
+    const int rc = get_plan(ctx, width, height, cap, &pe);
+    if (rc) return rc;
+    // the arrays layout_detect and layout_match place per frame / pair
+    *bytes = pe->surv_bytes + pe->kp_bytes + pe->sm_bytes + pe->pyr_bytes + pe->pair_bytes +
+             2 * pe->index_bytes;
+    return ORBFE_OK;
+}
+
 int orbfe_export_features_async(orbfe_ctx* ctx, const void* features, orbfe_feature* out,
                                 uint32_t* counts, hipStream_t stream) {
     if (!ctx) return ORBFE_EINVAL;

@@ -1,28 +1,21 @@
-"""Continuous-batching front-end server for many camera streams on one GPU.
+"""Served form of the front-end: Python binding of liborbfe's continuous-
+batching server (orbfe_server_* of include/orbfe.h).
 
-Frames arrive from many streams at their own times.  The server keeps, per
-stream, a ring of slots resident in HBM (upload buffer, feature array,
-pyramid and match table; frame k uses slot k mod slots), so a stream's
-previous-frame descriptors stay on the device.  Each scheduling tick takes
-the frames that have arrived, admits the oldest ones (a few consecutive
-frames per stream at most, each stream in its own order) as far as the HBM
-budget and the in-flight slots allow, uploads them and launches one batched
-detect (all admitted frames, any mix of sizes) and one batched match (each
-frame against its stream's previous frame) on one of two HIP streams.  A
-batch's uploads overlap the kernels of the batch before it; its kernels wait
-(on the device) for the batches holding its streams' previous frames.
-Latency = completion - arrival.
+The server itself is native: per-stream slot rings resident in HBM, FIFO
+admission over all streams within an HBM budget, batched detect + match per
+in-flight slot, device-side waits between a stream's consecutive frames, and
+the real-time schedule replay loop (orbfe_server_run) all live in
+uni_slam_amd/csrc/orbfe_server.cpp.  This module turns Python schedules and
+torch pinned frames into its C structures, and computes latency percentiles.
 
 Results are bit-identical to the plain per-frame path: the batched kernels
 compute each frame exactly as the per-frame launch does.
 """
 from __future__ import annotations
 
-import collections
-import gc
-import heapq
+import ctypes as C
 import time
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
@@ -33,56 +26,86 @@ from . import orbfe
 LEVELS = 6
 
 
-def frame_scratch_bytes(width: int, height: int, cap: int) -> int:
-    """Device scratch one in-flight frame uses in a batch, as laid out by
-    liborbfe: survivor bitmap, keypoint list and smoothed image of the
-    detect stage; match table, query list and the two sides' cell indices
-    (cell starts and the records sorted by cell) of the match stage."""
-    dims = orbfe.level_dims(width, height)
-    words = sum(h * ((w + 63) // 64) for w, h in dims)
-    smooth = sum((w + 4) * (h + 4) for w, h in dims)
+class StreamSpec(C.Structure):
+    """orbfe_stream_spec: width, height, cap (0 = width*height/250)."""
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("cap", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
-    def al(v):
-        return (v + 255) // 256 * 256
-
-    def al4(v):
-        return (v + 3) // 4 * 4
-    idx_words = al4(48 * 24 + 1) + 12 * cap
-    detect = al(8 * words) + al(4 * LEVELS * cap) + al(4 * smooth)
-    qlist_words = 64 * 32 + LEVELS * cap + 64 * LEVELS + 64 * 64
-    match = al(4 * LEVELS * cap) + al(4 * qlist_words) + 2 * al(4 * LEVELS * idx_words)
-    return detect + match
+    def __init__(self, width: int, height: int, cap: int = 0):
+        super().__init__(width, height, cap, 0)
 
 
-@dataclass
-class StreamSpec:
-    width: int
-    height: int
-    cap: int
+class ServerConfig(C.Structure):
+    _fields_ = [("detect", orbfe.Config), ("hbm_budget", C.c_size_t),
+                ("max_batch", C.c_uint32), ("inflight", C.c_uint32),
+                ("frames_per_stream", C.c_uint32), ("log_batches", C.c_uint32)]
 
 
-@dataclass
-class _Request:
-    stream: int
-    seq: int            # frame number within its stream
-    t_arrival: float
-    host: torch.Tensor  # pinned u8 (H, W)
-    order: int = 0      # submission number (FIFO admission key)
+class Arrival(C.Structure):
+    _fields_ = [("t", C.c_double), ("stream", C.c_uint32), ("seq", C.c_uint32)]
 
 
-@dataclass
-class _Stream:
-    spec: StreamSpec
-    feats: List[torch.Tensor]     # per slot
-    pyramid: List[torch.Tensor]   # per slot
-    match: List[torch.Tensor]     # per slot: the slot's frame against the frame before
-    upload: List[torch.Tensor]    # per slot
-    scratch: int
-    # prebuilt C descriptors of the frame / pair of each slot
-    frame_c: List[orbfe.Frame] = field(default_factory=list)
-    pair_c: List[orbfe.MatchPair] = field(default_factory=list)
-    pending: collections.deque = field(default_factory=collections.deque)
-    events: Dict[int, torch.cuda.Event] = field(default_factory=dict)  # seq -> its batch
+class Completion(C.Structure):
+    _fields_ = [("stream", C.c_uint32), ("seq", C.c_uint32), ("t_arrival", C.c_double),
+                ("t_done", C.c_double)]
+
+
+class BatchEntry(C.Structure):
+    _fields_ = [("batch", C.c_uint32), ("stream", C.c_uint32), ("seq", C.c_uint32),
+                ("order", C.c_uint32)]
+
+
+_SIGS = [
+    ("orbfe_frame_scratch_bytes", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                            C.POINTER(C.c_size_t)]),
+    ("orbfe_server_config_default", C.c_int, [C.POINTER(ServerConfig)]),
+    ("orbfe_server_create", C.c_int, [C.POINTER(ServerConfig), C.POINTER(StreamSpec), C.c_uint32,
+                                      C.c_int, C.POINTER(C.c_void_p)]),
+    ("orbfe_server_destroy", None, [C.c_void_p]),
+    ("orbfe_server_last_error", C.c_char_p, [C.c_void_p]),
+    ("orbfe_server_clock", C.c_double, [C.c_void_p]),
+    ("orbfe_server_submit", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_double, C.c_void_p,
+                                      C.c_void_p, C.POINTER(C.c_uint32)]),
+    ("orbfe_server_pump", C.c_int, [C.c_void_p, C.POINTER(Completion), C.c_uint32,
+                                    C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    ("orbfe_server_idle", C.c_int, [C.c_void_p]),
+    ("orbfe_server_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
+    ("orbfe_server_batch_log", C.c_int, [C.c_void_p, C.POINTER(BatchEntry), C.c_size_t,
+                                         C.POINTER(C.c_size_t)]),
+    ("orbfe_server_run", C.c_int, [C.c_void_p, C.POINTER(Arrival), C.c_uint32,
+                                   C.POINTER(C.c_void_p), C.c_uint32, C.c_double,
+                                   C.POINTER(Completion), C.POINTER(C.c_double)]),
+]
+ENTRY_POINTS = [s[0] for s in _SIGS]
+_bound = False
+
+
+def _lib():
+    global _bound
+    L = orbfe.lib()
+    if not _bound:
+        for name, res, args in _SIGS:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _bound = True
+    return L
+
+
+def frame_scratch_bytes(width: int, height: int, cap: int, ctx: Optional[orbfe.Context] = None
+                        ) -> int:
+    """Device scratch one in-flight frame takes in a batch (detect and match
+    stages), as liborbfe lays it out (orbfe_frame_scratch_bytes)."""
+    own = ctx is None
+    if own:
+        ctx = orbfe.Context(width, height, features_per_level=cap)
+    try:
+        n = C.c_size_t()
+        ctx._check(_lib().orbfe_frame_scratch_bytes(ctx._h, width, height, cap, C.byref(n)))
+        return int(n.value)
+    finally:
+        if own:
+            ctx.close()
 
 
 @dataclass
@@ -93,236 +116,128 @@ class Completed:
     t_done: float
 
 
-def _frame_struct(st: _Stream, p: int) -> orbfe.Frame:
-    sp = st.spec
-    f = orbfe.Frame()
-    f.image = st.upload[p].data_ptr()
-    f.pitch = sp.width
-    f.width, f.height, f.cap = sp.width, sp.height, sp.cap
-    f.pyramid = st.pyramid[p].data_ptr()
-    f.features = st.feats[p].data_ptr()
-    return f
-
-
-def _pair_struct(st: _Stream, p: int) -> orbfe.MatchPair:
-    """Pair of the frame in slot p against the stream's previous frame."""
-    sp = st.spec
-    n = len(st.feats)
-    m = orbfe.MatchPair()
-    m.prev_features = st.feats[(p - 1) % n].data_ptr()
-    m.curr_features = st.feats[p].data_ptr()
-    m.prev_match = st.match[p].data_ptr()
-    m.width, m.height, m.cap = sp.width, sp.height, sp.cap
-    return m
-
-
 class Server:
-    """frames_per_stream: how many consecutive frames of one stream a batch
-    may hold (a stream with a backlog catches up in fewer, fuller batches).
-    Each stream keeps 2 * frames_per_stream + 2 slots (upload, features,
-    pyramid, match table) resident; frame k uses slot k mod slots."""
+    """A liborbfe server for the given streams.  frames_per_stream: how many
+    consecutive frames of one stream a batch may hold (each stream keeps
+    2 * frames_per_stream + 2 slots resident).  With keep_outputs, every
+    frame's feature array and match table are copied to pinned host memory
+    by the time it completes (outputs[(stream, seq)])."""
 
     def __init__(self, specs: Sequence[StreamSpec], device: int = 0,
                  hbm_budget: int = 1 << 30, max_batch: int = 64, inflight: int = 2,
                  fast_threshold: float = 0.15, lowe_ratio: float = 0.85,
-                 keep_outputs: bool = False, warmup: bool = True, log_batches: bool = False,
+                 keep_outputs: bool = False, log_batches: bool = False,
                  frames_per_stream: int = 3, **launch_shape):
-        self.dev = torch.device(f"cuda:{device}")
-        self.specs = list(specs)
-        self.budget = hbm_budget
-        self.max_batch = max_batch
-        self.per_stream = max(1, frames_per_stream)
-        self.slots = 2 * self.per_stream + 2
+        L = _lib()
+        self.specs = [StreamSpec(s.width, s.height, s.cap) for s in specs]
+        cfg = ServerConfig()
+        L.orbfe_server_config_default(C.byref(cfg))
+        cfg.detect.fast_threshold = fast_threshold
+        cfg.detect.lowe_ratio = lowe_ratio
+        for k, v in launch_shape.items():
+            setattr(cfg.detect, k, v)
+        cfg.hbm_budget = hbm_budget
+        cfg.max_batch = max_batch
+        cfg.inflight = inflight
+        cfg.frames_per_stream = frames_per_stream
+        cfg.log_batches = 1 if log_batches else 0
+        arr = (StreamSpec * len(self.specs))(*self.specs)
+        h = C.c_void_p()
+        rc = L.orbfe_server_create(C.byref(cfg), arr, len(self.specs), device, C.byref(h))
+        if rc != orbfe.OK:
+            raise orbfe.OrbfeError(f"orbfe_server_create failed ({rc})")
+        self._h = h
         self.keep_outputs = keep_outputs
-        # with log_batches: per launched batch, its (stream, seq, submission number)
-        self.batch_log: Optional[List[List[Tuple[int, int, int]]]] = [] if log_batches else None
         self.outputs: Dict[Tuple[int, int], Tuple[np.ndarray, Optional[np.ndarray]]] = {}
-        self._snap: Dict[Tuple[int, int], Tuple[torch.Tensor, Optional[torch.Tensor]]] = {}
-        w0, h0 = self.specs[0].width, self.specs[0].height
-        self.ctxs = [orbfe.Context(w0, h0, device=device, fast_threshold=fast_threshold,
-                                   lowe_ratio=lowe_ratio, **launch_shape)
-                     for _ in range(inflight)]
-        self.hip_streams = [torch.cuda.Stream(device=self.dev) for _ in range(inflight)]
-        self.busy: List[Optional[Tuple[torch.cuda.Event, List[_Request], int]]] = [None] * inflight
-        self.streams: List[_Stream] = []
-        n = self.slots
-        for sp in self.specs:
-            fb = orbfe.feature_array_bytes(LEVELS, sp.cap)
-            npyr = sum(w * h for w, h in orbfe.level_dims(sp.width, sp.height)[1:])
-            st = _Stream(
-                spec=sp,
-                feats=[torch.zeros(fb, dtype=torch.uint8, device=self.dev) for _ in range(n)],
-                pyramid=[torch.zeros(npyr, dtype=torch.float32, device=self.dev)
-                         for _ in range(n)],
-                match=[torch.full((LEVELS * sp.cap,), -1, dtype=torch.int32, device=self.dev)
-                       for _ in range(n)],
-                upload=[torch.empty((sp.height, sp.width), dtype=torch.uint8, device=self.dev)
-                        for _ in range(n)],
-                scratch=frame_scratch_bytes(sp.width, sp.height, sp.cap))
-            st.frame_c = [_frame_struct(st, p) for p in range(n)]
-            st.pair_c = [_pair_struct(st, p) for p in range(n)]
-            self.streams.append(st)
-        # streams with pending frames, keyed by their oldest frame's submission
-        # number: popping in key order is FIFO admission over all frames, a
-        # stream's frames in its own order
-        self._ready: List[Tuple[int, int]] = []
-        self._order = 0
+        self._pending_out: Dict[Tuple[int, int], Tuple[torch.Tensor, torch.Tensor]] = {}
+        self._hold: Dict[Tuple[int, int], torch.Tensor] = {}  # frames until they complete
         self.completed: List[Completed] = []
-        self.in_flight_bytes = 0
-        self.max_batch_seen = 0
-        self.batches = 0
-        if warmup:
-            self.warmup()
+        self._done_buf = (Completion * 4096)()
 
-    def warmup(self):
-        """Creates every stream's plan in each in-flight context and grows
-        the contexts' scratch arenas and pinned staging slots to the largest
-        batch admission can form (frames_per_stream frames of every stream),
-        so serving never allocates device or pinned memory.  Stream state is
-        reset."""
-        k = self.per_stream
-        frames = [st.frame_c[p] for st in self.streams for p in range(1, k + 1)]
-        pairs = [st.pair_c[p] for st in self.streams for p in range(1, k + 1)]
-        frames = (orbfe.Frame * len(frames))(*frames)
-        pairs = (orbfe.MatchPair * len(pairs))(*pairs)
-        for ctx, hs in zip(self.ctxs, self.hip_streams):
-            with torch.cuda.stream(hs):
-                for _ in range(4):  # each slot of the staging ring
-                    ctx.detect_batch_async(frames, stream=hs)
-                    ctx.match_batch_async(pairs, stream=hs)
-        torch.cuda.synchronize()
-        for st in self.streams:
-            for f in st.feats:
-                f.zero_()
-            for m in st.match:
-                m.fill_(-1)
-        torch.cuda.synchronize()
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib().orbfe_server_destroy(self._h)
+            self._h = None
 
-    # -- requests --------------------------------------------------------------
-    def submit(self, stream: int, seq: int, host_frame: torch.Tensor, t_arrival: float):
-        st = self.streams[stream]
-        r = _Request(stream, seq, t_arrival, host_frame, self._order)
-        self._order += 1
-        st.pending.append(r)
-        if len(st.pending) == 1:
-            heapq.heappush(self._ready, (r.order, stream))
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != orbfe.OK:
+            msg = _lib().orbfe_server_last_error(self._h)
+            raise orbfe.OrbfeError(f"liborbfe server error {rc}: {msg.decode() if msg else ''}")
+
+    def clock(self) -> float:
+        return float(_lib().orbfe_server_clock(self._h))
+
+    @property
+    def batches(self) -> int:
+        b, m = C.c_uint64(), C.c_uint32()
+        self._check(_lib().orbfe_server_stats(self._h, C.byref(b), C.byref(m)))
+        return int(b.value)
+
+    @property
+    def max_batch_seen(self) -> int:
+        b, m = C.c_uint64(), C.c_uint32()
+        self._check(_lib().orbfe_server_stats(self._h, C.byref(b), C.byref(m)))
+        return int(m.value)
+
+    @property
+    def batch_log(self) -> List[List[Tuple[int, int, int]]]:
+        """Per launched batch: its frames as (stream, seq, submission number)."""
+        n = C.c_size_t()
+        self._check(_lib().orbfe_server_batch_log(self._h, None, 0, C.byref(n)))
+        buf = (BatchEntry * max(n.value, 1))()
+        self._check(_lib().orbfe_server_batch_log(self._h, buf, n.value, C.byref(n)))
+        out: List[List[Tuple[int, int, int]]] = []
+        for e in buf[:n.value]:
+            while len(out) <= e.batch:
+                out.append([])
+            out[e.batch].append((e.stream, e.seq, e.order))
+        return out
+
+    def submit(self, stream: int, host_frame: torch.Tensor, t_arrival: Optional[float] = None):
+        """Queues a pinned u8 frame of `stream`; returns its seq."""
+        sp = self.specs[stream]
+        assert host_frame.is_pinned() and host_frame.numel() == sp.width * sp.height
+        fo = mo = None
+        if self.keep_outputs:
+            cap = sp.cap or sp.width * sp.height // 250
+            fo = torch.empty(orbfe.feature_array_bytes(LEVELS, cap), dtype=torch.uint8,
+                             pin_memory=True)
+            mo = torch.empty(LEVELS * cap, dtype=torch.int32, pin_memory=True)
+        seq = C.c_uint32()
+        t = self.clock() if t_arrival is None else t_arrival
+        self._check(_lib().orbfe_server_submit(self._h, stream, host_frame.data_ptr(), t,
+                                               None if fo is None else fo.data_ptr(),
+                                               None if mo is None else mo.data_ptr(),
+                                               C.byref(seq)))
+        key = (stream, int(seq.value))
+        self._hold[key] = host_frame
+        if fo is not None:
+            self._pending_out[key] = (fo, mo)
+        return key[1]
+
+    def pump(self) -> int:
+        """Retire finished batches, launch new ones; returns batches launched."""
+        n, launched = C.c_uint32(), C.c_uint32()
+        self._check(_lib().orbfe_server_pump(self._h, self._done_buf, len(self._done_buf),
+                                             C.byref(n), C.byref(launched)))
+        for c in self._done_buf[:n.value]:
+            key = (c.stream, c.seq)
+            self.completed.append(Completed(c.stream, c.seq, c.t_arrival, c.t_done))
+            self._hold.pop(key, None)
+            out = self._pending_out.pop(key, None)
+            if out is not None:
+                self.outputs[key] = (out[0].numpy().copy(),
+                                     out[1].numpy().copy() if c.seq > 0 else None)
+        return int(launched.value)
 
     def idle(self) -> bool:
-        return not self._ready and all(b is None for b in self.busy)
-
-    # -- scheduling -------------------------------------------------------------
-    def poll(self, now_fn=time.perf_counter):
-        """Retire finished batches; record their completion time."""
-        for i, b in enumerate(self.busy):
-            if b is None:
-                continue
-            ev, reqs, nbytes = b
-            if ev.query():
-                t = now_fn()
-                for r in reqs:
-                    self.completed.append(Completed(r.stream, r.seq, r.t_arrival, t))
-                    if self.keep_outputs:
-                        f, m = self._snap.pop((r.stream, r.seq))
-                        self.outputs[(r.stream, r.seq)] = (
-                            f.cpu().numpy(), None if m is None else m.cpu().numpy())
-                self.in_flight_bytes -= nbytes
-                self.busy[i] = None
-
-    def _admit(self) -> Tuple[List[_Request], int]:
-        """FIFO admission: the oldest pending frames, at most frames_per_stream
-        per stream (a stream's frames stay in order), up to max_batch, within
-        the HBM budget (a frame is always admitted when nothing else is in
-        flight)."""
-        chosen: List[_Request] = []
-        nbytes = 0
-        ready = self._ready
-        taken: Dict[int, int] = {}
-        deferred: List[int] = []  # streams at their per-batch limit with frames left
-        while ready and len(chosen) < self.max_batch:
-            s = ready[0][1]
-            st = self.streams[s]
-            if (self.in_flight_bytes + nbytes + st.scratch > self.budget
-                    and (chosen or self.in_flight_bytes > 0)):
-                break
-            heapq.heappop(ready)
-            chosen.append(st.pending.popleft())
-            nbytes += st.scratch
-            taken[s] = taken.get(s, 0) + 1
-            if st.pending:
-                if taken[s] < self.per_stream:
-                    heapq.heappush(ready, (st.pending[0].order, s))
-                else:
-                    deferred.append(s)
-        for s in deferred:  # back in line for later batches
-            heapq.heappush(ready, (self.streams[s].pending[0].order, s))
-        return chosen, nbytes
-
-    def launch(self) -> bool:
-        """Admit pending frames into a free slot; returns True if launched.
-
-        Device order on the slot's HIP stream: each frame's upload (after the
-        batches that last used its stream slot: frames k - slots and
-        k - slots + 1), then a wait for the batches holding the admitted
-        frames' previous frames (their features are this batch's match
-        input), then one batched detect and one batched match.  The detect
-        stage of a batch completes before its match stage, so a stream's
-        consecutive frames can share a batch.  The uploads of a batch overlap
-        the kernels of the batch before it."""
-        if not self._ready:
-            return False
-        try:
-            slot = self.busy.index(None)
-        except ValueError:
-            return False
-        chosen, nbytes = self._admit()
-        if not chosen:
-            return False
-        ctx, hs = self.ctxs[slot], self.hip_streams[slot]
-        n = self.slots
-        frames, pairs = [], []
-        waited = set()
-        in_batch = {(r.stream, r.seq) for r in chosen}
-
-        def wait(e):
-            if e is not None and id(e) not in waited:
-                hs.wait_event(e)
-                waited.add(id(e))
-
-        with torch.cuda.stream(hs):
-            for r in chosen:
-                st = self.streams[r.stream]
-                p = r.seq % n
-                wait(st.events.get(r.seq - n))
-                wait(st.events.get(r.seq - n + 1))
-                st.upload[p].copy_(r.host, non_blocking=True)
-                frames.append(st.frame_c[p])
-                if r.seq > 0:
-                    pairs.append(st.pair_c[p])
-            for r in chosen:
-                if (r.stream, r.seq - 1) not in in_batch:
-                    wait(self.streams[r.stream].events.get(r.seq - 1))
-            ctx.detect_batch_async((orbfe.Frame * len(frames))(*frames), stream=hs)
-            if pairs:
-                ctx.match_batch_async((orbfe.MatchPair * len(pairs))(*pairs), stream=hs)
-            if self.keep_outputs:  # stream-ordered snapshots of the results
-                for r in chosen:
-                    st = self.streams[r.stream]
-                    p = r.seq % n
-                    self._snap[(r.stream, r.seq)] = (
-                        st.feats[p].clone(), st.match[p].clone() if r.seq > 0 else None)
-            ev = torch.cuda.Event()
-            ev.record(hs)
-        for r in chosen:
-            st = self.streams[r.stream]
-            st.events[r.seq] = ev
-            st.events.pop(r.seq - n, None)  # no later frame waits for it
-        self.busy[slot] = (ev, chosen, nbytes)
-        if self.batch_log is not None:
-            self.batch_log.append([(r.stream, r.seq, r.order) for r in chosen])
-        self.in_flight_bytes += nbytes
-        self.max_batch_seen = max(self.max_batch_seen, len(chosen))
-        self.batches += 1
-        return True
+        return bool(_lib().orbfe_server_idle(self._h))
 
 
 def poisson_arrivals(rates: Sequence[float], duration: float, seed: int):
@@ -342,40 +257,37 @@ def poisson_arrivals(rates: Sequence[float], duration: float, seed: int):
     return out
 
 
-def run_schedule(server: Server, arrivals, frame_pool, time_scale: float = 1.0,
-                 spin_s: float = 20e-6):
-    """Replays arrivals in real time (scaled by time_scale) through server.
-    frame_pool[stream] is a list of pinned host frames, used cyclically.
-    Returns (wall seconds, completed list).  Python's cyclic garbage
-    collector is paused for the run (its pauses would land in the latency
-    tail)."""
-    gc_was_enabled = gc.isenabled()
-    gc.collect()
-    gc.disable()
-    try:
-        return _run(server, arrivals, frame_pool, time_scale, spin_s)
-    finally:
-        if gc_was_enabled:
-            gc.enable()
-
-
-def _run(server, arrivals, frame_pool, time_scale, spin_s):
-    t0 = time.perf_counter()
-    i = 0
-    n = len(arrivals)
-    while i < n or not server.idle():
-        now = time.perf_counter() - t0
-        while i < n and arrivals[i][0] * time_scale <= now:
-            ta, s, k = arrivals[i]
-            pool = frame_pool[s]
-            server.submit(s, k, pool[k % len(pool)], t0 + ta * time_scale)
-            i += 1
-        server.poll()
-        launched = server.launch()
-        if not launched and (i < n or not server.idle()):
-            time.sleep(spin_s)
-    wall = time.perf_counter() - t0
-    return wall, server.completed
+def run_schedule(server: Server, arrivals, frame_pool, time_scale: float = 1.0):
+    """Replays arrivals (t, stream, seq) through server in real time (scaled
+    by time_scale; 0 = all waiting at once).  frame_pool[stream] is a list of
+    pinned host frames, used cyclically (every stream the same length).
+    Without keep_outputs the whole loop runs natively (orbfe_server_run);
+    with it, from Python through submit/pump.  Returns (wall seconds,
+    completed list)."""
+    P = len(frame_pool[0])
+    assert all(len(p) == P for p in frame_pool)
+    if server.keep_outputs:
+        t0 = time.perf_counter()
+        i, n = 0, len(arrivals)
+        while i < n or not server.idle():
+            now = time.perf_counter() - t0
+            while i < n and arrivals[i][0] * time_scale <= now:
+                _, s, k = arrivals[i]
+                server.submit(s, frame_pool[s][k % P])
+                i += 1
+            if not server.pump() and (i < n or not server.idle()):
+                time.sleep(20e-6)
+        return time.perf_counter() - t0, server.completed
+    arr = (Arrival * max(len(arrivals), 1))(*[Arrival(t, s, k) for t, s, k in arrivals])
+    pool = (C.c_void_p * (len(frame_pool) * P))(
+        *[f.data_ptr() for frames in frame_pool for f in frames])
+    done = (Completion * max(len(arrivals), 1))()
+    wall = C.c_double()
+    server._check(_lib().orbfe_server_run(server._h, arr, len(arrivals), pool, P, time_scale,
+                                          done, C.byref(wall)))
+    out = [Completed(c.stream, c.seq, c.t_arrival, c.t_done) for c in done[:len(arrivals)]]
+    server.completed.extend(out)
+    return float(wall.value), out
 
 
 def plain_backlog(specs: Sequence[StreamSpec], arrivals, frame_pool, device: int = 0,
