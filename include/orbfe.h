@@ -354,6 +354,21 @@ int orbfe_server_run(struct orbfe_server* srv, const orbfe_arrival* arrivals, ui
                      const uint8_t* const* pool, uint32_t pool_per_stream, double time_scale,
                      orbfe_completion* done, double* wall_s);
 
+/* RANSAC hypothesis scoring, the per-iteration inner loop of
+ * RansacCPU::execute (ClassifyInliers + SampsonDistance,
+ * wslam/src/pose_estimate_cpu/ransac_cpu.cpp:178-209), for k hypotheses at
+ * once.  F: k fundamental matrices, 9 row-major doubles each; prev, curr:
+ * n correspondences as (x, y) double pairs in LOD-0 pixels (16-byte
+ * aligned), prev the first image's point (F's column side).  For each
+ * hypothesis h, counts[h] gets the number of correspondences whose Sampson
+ * distance (f64, the reference's operation order, +inf for a denominator
+ * below 1e-12) is below `threshold`; with mask != NULL, mask[h*n + i] = 1
+ * for each inlier and 0 otherwise.  Device pointers, k <= 65535. */
+int orbfe_ransac_score_async(struct orbfe_ctx* ctx, const double* F, uint32_t k,
+                             const double* prev, const double* curr, uint32_t n,
+                             double threshold, uint32_t* counts, uint8_t* mask,
+                             hipStream_t stream);
+
 /* Kernel timing: when enabled, every kernel launch of the context is
  * bracketed by HIP events on its stream; orbfe_timing_read synchronizes and
  * returns, for kernel `name`, the launch count and summed duration (ms). */

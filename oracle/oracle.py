@@ -59,6 +59,10 @@ def lib():
         L.oracle_hamming_bounded.restype = u32
         L.oracle_find_matches.argtypes = [vp, vp, vp, vp, u32, u32, u32, f32, vp, vp]
         L.oracle_match_bf.argtypes = [vp, u32, vp, u32, vp, vp, vp]
+        L.oracle_sampson_counts.argtypes = [vp, u32, vp, vp, u32, C.c_double, vp, vp]
+        L.oracle_fit_fundamental.argtypes = [vp, vp, u32, vp]
+        L.oracle_fit_fundamental.restype = C.c_int
+        L.oracle_ransac.argtypes = [vp, u32, vp, vp, u32, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -196,5 +200,48 @@ def prev_match_to_padded(prev_match_flat: np.ndarray, counts, cap: int, levels: 
     for l in range(levels):
         n = counts[l]
         out[l * cap:l * cap + n] = prev_match_flat[off:off + n]
+        off += n
+    return out
+
+
+def sampson_counts(F: np.ndarray, prev: np.ndarray, curr: np.ndarray, threshold: float,
+                   with_mask: bool = False):
+    """Inlier counts (and masks) of k fundamental matrices F (k, 9) over
+    correspondences prev, curr (n, 2), float64."""
+    F = np.ascontiguousarray(F, np.float64).reshape(-1, 9)
+    prev = np.ascontiguousarray(prev, np.float64)
+    curr = np.ascontiguousarray(curr, np.float64)
+    k, n = F.shape[0], prev.shape[0]
+    counts = np.zeros(k, np.uint32)
+    mask = np.zeros((k, n), np.uint8) if with_mask else None
+    lib().oracle_sampson_counts(_p(F), k, _p(prev), _p(curr), n, threshold, _p(counts),
+                                None if mask is None else _p(mask))
+    return (counts, mask) if with_mask else counts
+
+
+def fit_fundamental(prev: np.ndarray, curr: np.ndarray):
+    prev = np.ascontiguousarray(prev, np.float64)
+    curr = np.ascontiguousarray(curr, np.float64)
+    F = np.zeros(9, np.float64)
+    ok = lib().oracle_fit_fundamental(_p(prev), _p(curr), prev.shape[0], _p(F))
+    return F.reshape(3, 3) if ok else None
+
+
+def ransac(sets, max_iterations: int = 500):
+    """RansacCPU over consecutive sets [(prev (n, 2), curr (n, 2)), ...] with
+    one generator: per set (stats dict, F (3, 3), inlier mask)."""
+    sizes = np.array([len(p) for p, _ in sets], np.uint32)
+    prev = np.ascontiguousarray(np.concatenate([p for p, _ in sets]), np.float64)
+    curr = np.ascontiguousarray(np.concatenate([c for _, c in sets]), np.float64)
+    stats = np.zeros((len(sets), 4), np.uint64)
+    F = np.zeros((len(sets), 9), np.float64)
+    mask = np.zeros(len(prev), np.uint8)
+    lib().oracle_ransac(_p(sizes), len(sets), _p(prev), _p(curr), max_iterations, _p(stats),
+                        _p(F), _p(mask))
+    out, off = [], 0
+    for i, n in enumerate(sizes):
+        st = {"total": int(stats[i, 0]), "iterations": int(stats[i, 1]),
+              "inliers": int(stats[i, 2]), "found": bool(stats[i, 3])}
+        out.append((st, F[i].reshape(3, 3), mask[off:off + n].copy()))
         off += n
     return out

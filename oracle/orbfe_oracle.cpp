@@ -11,6 +11,8 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <unordered_set>
+#include <random>
 #include <tuple>
 #include <utility>
 #include <vector>
@@ -643,5 +645,241 @@ extern "C" void oracle_match_bf(const uint32_t* q, uint32_t nq,
         idx[i] = best_idx;
         d1[i] = best;
         d2[i] = second;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// RANSAC of the pose-estimate stage (ransac_cpu.cpp).
+// ---------------------------------------------------------------------------
+namespace {
+
+// SampsonDistance (ransac_cpu.cpp:180-195): F x_prev and F^T x_curr as
+// left-to-right sums over the row / column, x_curr . F x_prev, then
+// num^2 / (four squares summed left to right); +inf below 1e-12.
+double sampson(const double* f, double px, double py, double cx, double cy) {
+    const double fx0 = f[0] * px + f[1] * py + f[2] * 1.0;
+    const double fx1 = f[3] * px + f[4] * py + f[5] * 1.0;
+    const double fx2 = f[6] * px + f[7] * py + f[8] * 1.0;
+    const double ft0 = f[0] * cx + f[3] * cy + f[6] * 1.0;
+    const double ft1 = f[1] * cx + f[4] * cy + f[7] * 1.0;
+    const double num = cx * fx0 + cy * fx1 + 1.0 * fx2;
+    const double denom = fx0 * fx0 + fx1 * fx1 + ft0 * ft0 + ft1 * ft1;
+    if (denom < 1e-12) return std::numeric_limits<double>::infinity();
+    return (num * num) / denom;
+}
+
+// Cyclic Jacobi eigendecomposition of the symmetric n x n matrix `a`
+// (destroyed); returns the unit eigenvector of the smallest eigenvalue.
+std::vector<double> smallest_eigenvector(std::vector<double> a, int n) {
+    std::vector<double> v(n * n, 0.0);
+    for (int i = 0; i < n; ++i) v[i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, diag = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) (i == j ? diag : off) += a[i * n + j] * a[i * n + j];
+        if (off <= 1e-32 * diag) break;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = a[p * n + q];
+                if (apq == 0.0) continue;
+                const double theta = (a[q * n + q] - a[p * n + p]) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) /
+                                 (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < n; ++k) {  // A <- A J (columns p, q)
+                    const double akp = a[k * n + p], akq = a[k * n + q];
+                    a[k * n + p] = c * akp - s * akq;
+                    a[k * n + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {  // A <- J^T A (rows p, q)
+                    const double apk = a[p * n + k], aqk = a[q * n + k];
+                    a[p * n + k] = c * apk - s * aqk;
+                    a[q * n + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double vkp = v[k * n + p], vkq = v[k * n + q];
+                    v[k * n + p] = c * vkp - s * vkq;
+                    v[k * n + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    int m = 0;
+    for (int i = 1; i < n; ++i)
+        if (a[i * n + i] < a[m * n + m]) m = i;
+    std::vector<double> out(n);
+    double nn = 0.0;
+    for (int k = 0; k < n; ++k) nn += v[k * n + m] * v[k * n + m];
+    nn = std::sqrt(nn);
+    for (int k = 0; k < n; ++k) out[k] = v[k * n + m] / nn;
+    return out;
+}
+
+// NormalisingTransform (ransac_cpu.cpp:78-109) as (scale, cx, cy).
+void normalising(const double* pts, uint32_t n, double* s, double* cx, double* cy) {
+    double x = 0.0, y = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        x += pts[2 * i];
+        y += pts[2 * i + 1];
+    }
+    x /= n;
+    y /= n;
+    double mean = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const double dx = pts[2 * i] - x, dy = pts[2 * i + 1] - y;
+        mean += std::sqrt(dx * dx + dy * dy);
+    }
+    mean /= n;
+    *s = mean > 1e-12 ? std::sqrt(2.0) / mean : 1.0;
+    *cx = x;
+    *cy = y;
+}
+
+int fit_fundamental(const double* prev, const double* curr, uint32_t n, double* F) {
+    if (n < 8) return 0;
+    double sp, pcx, pcy, sc, ccx, ccy;
+    normalising(prev, n, &sp, &pcx, &pcy);
+    normalising(curr, n, &sc, &ccx, &ccy);
+    // A^T A of the rows (ransac_cpu.cpp:130-145)
+    std::vector<double> ata(81, 0.0);
+    for (uint32_t i = 0; i < n; ++i) {
+        const double xp = sp * prev[2 * i] - sp * pcx, yp = sp * prev[2 * i + 1] - sp * pcy;
+        const double xc = sc * curr[2 * i] - sc * ccx, yc = sc * curr[2 * i + 1] - sc * ccy;
+        const double r[9] = {xc * xp, xc * yp, xc, yc * xp, yc * yp, yc, xp, yp, 1.0};
+        for (int a = 0; a < 9; ++a)
+            for (int b = 0; b < 9; ++b) ata[a * 9 + b] += r[a] * r[b];
+    }
+    const std::vector<double> f = smallest_eigenvector(ata, 9);
+    // rank 2: F (I - v v^T), v the right singular vector of the smallest
+    // singular value of F (eigenvector of F^T F)
+    std::vector<double> ftf(9, 0.0);
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            for (int k = 0; k < 3; ++k) ftf[a * 3 + b] += f[k * 3 + a] * f[k * 3 + b];
+    const std::vector<double> v = smallest_eigenvector(ftf, 3);
+    double f2[9];
+    for (int r = 0; r < 3; ++r) {
+        const double fv = f[r * 3] * v[0] + f[r * 3 + 1] * v[1] + f[r * 3 + 2] * v[2];
+        for (int c = 0; c < 3; ++c) f2[r * 3 + c] = f[r * 3 + c] - fv * v[c];
+    }
+    // t_curr^T F2 t_prev with t = [[s, 0, -s cx], [0, s, -s cy], [0, 0, 1]]
+    const double tp[9] = {sp, 0.0, -sp * pcx, 0.0, sp, -sp * pcy, 0.0, 0.0, 1.0};
+    const double tc[9] = {sc, 0.0, -sc * ccx, 0.0, sc, -sc * ccy, 0.0, 0.0, 1.0};
+    double tmp[9] = {0}, out[9] = {0};
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            for (int k = 0; k < 3; ++k) tmp[r * 3 + c] += f2[r * 3 + k] * tp[k * 3 + c];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            for (int k = 0; k < 3; ++k) out[r * 3 + c] += tc[k * 3 + r] * tmp[k * 3 + c];
+    for (double x : out)
+        if (!std::isfinite(x)) return 0;
+    std::memcpy(F, out, sizeof(out));
+    return 1;
+}
+
+}  // namespace
+
+extern "C" void oracle_sampson_counts(const double* F, uint32_t k, const double* prev,
+                                      const double* curr, uint32_t n, double threshold,
+                                      uint32_t* counts, uint8_t* mask) {
+    for (uint32_t h = 0; h < k; ++h) {
+        uint32_t c = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            const double d = sampson(F + 9 * h, prev[2 * i], prev[2 * i + 1], curr[2 * i],
+                                     curr[2 * i + 1]);
+            const bool in = d < threshold;
+            c += in;
+            if (mask) mask[static_cast<size_t>(h) * n + i] = in;
+        }
+        counts[h] = c;
+    }
+}
+
+extern "C" int oracle_fit_fundamental(const double* prev, const double* curr, uint32_t n,
+                                      double* F) {
+    return fit_fundamental(prev, curr, n, F);
+}
+
+extern "C" void oracle_ransac(const uint32_t* sizes, uint32_t sets, const double* prev,
+                              const double* curr, uint32_t max_iterations, uint64_t* stats,
+                              double* F, uint8_t* mask) {
+    std::mt19937_64 rng(0xC0FFEE1234ULL);  // RansacCPU::Opts::rng_seed
+    const double threshold = 3.84, confidence = 0.99;
+    const size_t min_matches = 8;
+    size_t off = 0;
+    for (uint32_t s = 0; s < sets; ++s) {
+        const uint32_t n = sizes[s];
+        const double* p = prev + 2 * off;
+        const double* c = curr + 2 * off;
+        uint8_t* m = mask + off;
+        std::memset(m, 0, n);
+        std::memset(F + 9 * s, 0, 9 * sizeof(double));
+        uint64_t* st = stats + 4 * s;
+        st[0] = n;
+        st[1] = st[2] = st[3] = 0;
+        off += n;
+        if (n < min_matches) continue;
+        std::vector<uint8_t> best_mask(n, 0), tmask(n);
+        uint32_t best = 0;
+        double best_f[9] = {0};
+        uint32_t iter = 0, needed = max_iterations;
+        for (; iter < needed; ++iter) {
+            // SampleEight (ransac_cpu.cpp:212-225)
+            size_t idx[8];
+            std::unordered_set<size_t> picked;
+            std::uniform_int_distribution<size_t> dist(0, n - 1);
+            size_t filled = 0;
+            while (filled < 8) {
+                const size_t v = dist(rng);
+                if (picked.insert(v).second) idx[filled++] = v;
+            }
+            double sp[16], sc[16], f[9];
+            for (int i = 0; i < 8; ++i) {
+                sp[2 * i] = p[2 * idx[i]];
+                sp[2 * i + 1] = p[2 * idx[i] + 1];
+                sc[2 * i] = c[2 * idx[i]];
+                sc[2 * i + 1] = c[2 * idx[i] + 1];
+            }
+            if (!fit_fundamental(sp, sc, 8, f)) continue;
+            uint32_t cnt = 0;
+            oracle_sampson_counts(f, 1, p, c, n, threshold, &cnt, tmask.data());
+            if (cnt > best) {
+                best = cnt;
+                best_mask = tmask;
+                std::memcpy(best_f, f, sizeof(f));
+                const double ratio = static_cast<double>(best) / static_cast<double>(n);
+                const double pa = std::pow(ratio, 8.0);
+                if (pa >= 1.0 - 1e-12) {
+                    needed = iter + 1;
+                } else if (pa > 0.0) {
+                    const double bound = std::log(1.0 - confidence) / std::log(1.0 - pa);
+                    needed = static_cast<uint32_t>(
+                        std::min<double>(max_iterations, std::ceil(bound)));
+                }
+            }
+        }
+        st[1] = iter;
+        if (best < min_matches) continue;
+        // refit on the inliers, keep it when it classifies at least as many
+        std::vector<double> rp, rc;
+        for (uint32_t i = 0; i < n; ++i)
+            if (best_mask[i]) {
+                rp.insert(rp.end(), {p[2 * i], p[2 * i + 1]});
+                rc.insert(rc.end(), {c[2 * i], c[2 * i + 1]});
+            }
+        double rf[9];
+        if (fit_fundamental(rp.data(), rc.data(), best, rf)) {
+            uint32_t cnt = 0;
+            oracle_sampson_counts(rf, 1, p, c, n, threshold, &cnt, tmask.data());
+            if (cnt >= best) {
+                best = cnt;
+                best_mask = tmask;
+                std::memcpy(best_f, rf, sizeof(rf));
+            }
+        }
+        std::memcpy(m, best_mask.data(), n);
+        std::memcpy(F + 9 * s, best_f, sizeof(best_f));
+        st[2] = best;
+        st[3] = 1;
     }
 }

@@ -139,6 +139,28 @@ int oracle_find_matches(const oracle_feature* prev, const uint32_t* prev_counts,
 void oracle_match_bf(const uint32_t* q, uint32_t nq, const uint32_t* t,
                      uint32_t nt, uint32_t* idx, uint32_t* d1, uint32_t* d2);
 
+/* SampsonDistance + ClassifyInliers (ransac_cpu.cpp:178-209) for k
+ * fundamental matrices (9 row-major doubles each) over n correspondences
+ * (prev, curr as (x, y) double pairs): counts[h] = inliers of hypothesis h;
+ * mask (nullable) gets mask[h*n + i] = 1 for each inlier. */
+void oracle_sampson_counts(const double* F, uint32_t k, const double* prev, const double* curr,
+                           uint32_t n, double threshold, uint32_t* counts, uint8_t* mask);
+
+/* FitFundamentalMatrix (ransac_cpu.cpp:111-175) on n >= 8 correspondences:
+ * writes F (row-major, unit-norm null vector before denormalisation) and
+ * returns 1, or returns 0 when degenerate.  The null vector comes from a
+ * cyclic Jacobi eigendecomposition of A^T A. */
+int oracle_fit_fundamental(const double* prev, const double* curr, uint32_t n, double* F);
+
+/* RansacCPU::execute (ransac_cpu.cpp:262-370) over `sets` consecutive sets
+ * of correspondences (sizes[s] each, prev and curr concatenated), one
+ * generator seeded with 0xC0FFEE1234 carried across the sets, threshold
+ * 3.84, confidence 0.99, min_matches 8.  Per set: stats[4*s..] = total,
+ * iterations run, inliers, model found; F[9*s..]; mask (concatenated like
+ * the sets) = 1 for the final inliers. */
+void oracle_ransac(const uint32_t* sizes, uint32_t sets, const double* prev, const double* curr,
+                   uint32_t max_iterations, uint64_t* stats, double* F, uint8_t* mask);
+
 #ifdef __cplusplus
 }
 #endif

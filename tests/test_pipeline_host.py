@@ -84,17 +84,27 @@ def test_pipeline_driver_matches_oracle(tmp_path, torch_gpu):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("opts", [("--f32",), ("--graph",), ("--f32", "--graph")])
+@pytest.mark.parametrize("opts", [("--f32",), ("--graph",), ("--f32", "--graph", "--ransac")])
 def test_pipeline_driver_held_keyframe_f32_and_graphs(tmp_path, torch_gpu, opts):
     """The reference's f32 FrameBW input, the loader holding the reference
     keyframe on frames without the advance marker, and both stages replayed
     from captured HIP graphs (one per buffer slot combination): the driver's
-    features and MatchResult equal the oracle's on every frame."""
+    features and MatchResult equal the oracle's on every frame.  With
+    --ransac the match stage ends with the RANSAC pass, whose result covers
+    every match of the frame."""
     W, H, N, cap, ratio = 752, 480, 8, 1000, 0.8
     frames, _ = synth.sequence(5, W, H, n_frames=N)
     advance = "10100111"  # frames 1, 3, 4 keep the reference of an earlier frame
     lines = _run_driver(tmp_path, frames, W, H, cap, ratio, *opts, "--advance", advance)
+    ransac = [ln.split() for ln in lines if ln.startswith("ransac ")]
+    lines = [ln for ln in lines if not ln.startswith("ransac ")]
     assert lines == _expected_lines(frames, W, H, cap, ratio, advance)
+    if "--ransac" in opts:
+        assert [int(r[1]) for r in ransac] == list(range(1, N))
+        for r in ransac:
+            t, total, iters, inliers, found = (int(v) for v in r[1:])
+            assert total == sum(1 for ln in lines if ln.startswith(f"m {t} "))
+            assert found == 1 and 8 <= inliers <= total and 1 <= iters <= 500
 
 
 @pytest.mark.gpu

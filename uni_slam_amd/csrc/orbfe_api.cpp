@@ -37,6 +37,9 @@ hipError_t launch_bf_mfma(const uint32_t* q, uint32_t nq, const uint32_t* t, uin
 hipError_t launch_bf_merge(const uint2* part, uint32_t nq, uint32_t splits, uint32_t* idx,
                            uint32_t* d1, uint32_t* d2, hipStream_t s);
 hipError_t bf_scratch_address(void** out);
+hipError_t launch_ransac_score(const double* F, uint32_t k, const double* prev,
+                               const double* curr, uint32_t n, double thr, uint32_t* counts,
+                               uint8_t* mask, hipStream_t s);
 hipError_t launch_export(const uint8_t* feat, uint32_t levels, uint32_t cap, uint32_t stride,
                          uint32_t* out, uint32_t* counts, hipStream_t s);
 hipError_t launch_valu_probe(uint32_t* sink, uint32_t blocks, hipStream_t s,
@@ -1294,6 +1297,23 @@ int orbfe_match_bf_noalloc_async(orbfe_ctx* ctx, const uint32_t* query, uint32_t
             }));
     HIP_TRY(ctx, timed(ctx, "k_bf_merge", stream, [&] {
                 return orbfe::launch_bf_merge(ctx->bf_static, nq, splits, idx, d1, d2, stream);
+            }));
+    return ORBFE_OK;
+}
+
+int orbfe_ransac_score_async(orbfe_ctx* ctx, const double* F, uint32_t k, const double* prev,
+                             const double* curr, uint32_t n, double threshold, uint32_t* counts,
+                             uint8_t* mask, hipStream_t stream) {
+    if (!ctx) return ORBFE_EINVAL;
+    if (k == 0) return ORBFE_OK;
+    if (!F || !counts || (n > 0 && (!prev || !curr)))
+        return fail(ctx, ORBFE_EINVAL, "null pointer");
+    if (k > 65535) return fail(ctx, ORBFE_ERANGE, "%u hypotheses > 65535", k);
+    if ((reinterpret_cast<uintptr_t>(prev) | reinterpret_cast<uintptr_t>(curr)) & 15u)
+        return fail(ctx, ORBFE_EINVAL, "prev and curr must be 16-byte aligned");
+    HIP_TRY(ctx, timed(ctx, "k_ransac_score", stream, [&] {
+                return orbfe::launch_ransac_score(F, k, prev, curr, n, threshold, counts, mask,
+                                                  stream);
             }));
     return ORBFE_OK;
 }

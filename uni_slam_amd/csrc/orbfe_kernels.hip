@@ -2056,6 +2056,59 @@ __global__ __launch_bounds__(kExportThreads) void k_export(const uint8_t* __rest
 }
 
 // ---------------------------------------------------------------------------
+// k_ransac_score: RANSAC hypothesis scoring (RansacCPU::execute's
+// ClassifyInliers with SampsonDistance, wslam/src/pose_estimate_cpu/
+// ransac_cpu.cpp:178-209) for many fundamental matrices at once.  Workgroup
+// h scores hypothesis h against every correspondence: the Sampson distance
+// of (prev, curr) in f64 with the reference's operation order (F x_prev and
+// F^T x_curr as left-to-right row sums, x_curr . F x_prev, the four squares
+// summed left to right; a denominator below 1e-12 gives +inf), compared
+// with the threshold; the inlier count is reduced over the workgroup and
+// written once.  With a mask, mask[h * n + i] = 1 for each inlier.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kRansacThreads = 1024;
+
+__device__ inline bool sampson_inlier(const double* f, double px, double py, double cx,
+                                      double cy, double thr) {
+    const double fx0 = f[0] * px + f[1] * py + f[2] * 1.0;
+    const double fx1 = f[3] * px + f[4] * py + f[5] * 1.0;
+    const double fx2 = f[6] * px + f[7] * py + f[8] * 1.0;
+    const double ft0 = f[0] * cx + f[3] * cy + f[6] * 1.0;
+    const double ft1 = f[1] * cx + f[4] * cy + f[7] * 1.0;
+    const double num = cx * fx0 + cy * fx1 + 1.0 * fx2;
+    const double denom = fx0 * fx0 + fx1 * fx1 + ft0 * ft0 + ft1 * ft1;
+    if (denom < 1e-12) return false;  // distance +inf
+    return (num * num) / denom < thr;
+}
+
+__global__ __launch_bounds__(kRansacThreads) void k_ransac_score(
+    const double* __restrict__ F, const double2* __restrict__ prev,
+    const double2* __restrict__ curr, uint32_t n, double thr, uint32_t* __restrict__ counts,
+    uint8_t* __restrict__ mask) {
+    __shared__ uint32_t wsum[kRansacThreads / 64];
+    const uint32_t h = blockIdx.x;
+    double f[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) f[i] = F[9 * h + i];
+    uint32_t c = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += kRansacThreads) {
+        const double2 p = prev[i], q = curr[i];
+        const bool in = sampson_inlier(f, p.x, p.y, q.x, q.y, thr);
+        c += in ? 1u : 0u;
+        if (mask) mask[static_cast<size_t>(h) * n + i] = in ? 1 : 0;
+    }
+    const uint32_t t = wave_total(c);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < kRansacThreads / 64; ++w) total += wsum[w];
+        counts[h] = total;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Host-side launchers (called from orbfe_api.cpp).
 // ---------------------------------------------------------------------------
 // Gather order of the describe kernel (g_desc_order): for each angle bin,
@@ -2251,6 +2304,15 @@ hipError_t launch_export(const uint8_t* feat, uint32_t levels, uint32_t cap, uin
     const uint32_t chunks = (cap + kExportThreads - 1) / kExportThreads;
     hipLaunchKernelGGL(k_export, dim3(levels * chunks), dim3(kExportThreads), 0, s, feat, levels,
                        cap, stride, chunks, out, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_ransac_score(const double* F, uint32_t k, const double* prev,
+                               const double* curr, uint32_t n, double thr, uint32_t* counts,
+                               uint8_t* mask, hipStream_t s) {
+    hipLaunchKernelGGL(k_ransac_score, dim3(k), dim3(kRansacThreads), 0, s, F,
+                       reinterpret_cast<const double2*>(prev),
+                       reinterpret_cast<const double2*>(curr), n, thr, counts, mask);
     return hipGetLastError();
 }
 

@@ -19,6 +19,7 @@ inline std::string GetFeatureSetName(size_t keyframes_ago) {
 inline const std::string MatchedFeaturesName = "gen:feat:match";
 // Consume-once marker asking the loader to advance the reference feature set.
 inline const std::string FeatureReferenceAdvanceName = "gen:feat:refadv";
+inline const std::string RansacResultName = "gen:feat:ransac";
 }  // namespace ResourceIdentifier
 
 // A grayscale frame kept as u8 end to end (the reference converts to f32 on

@@ -169,8 +169,11 @@ class PublishMatchesPass : public compute::Pass {
 std::unique_ptr<compute::HipStage> CreateFeatureDetectStage(
     std::shared_ptr<compute::Device> device, compute::Store& storage,
     std::shared_ptr<FeatureBuffers> buffers);
+// With `ransac`, the stage ends with RansacHipPass (the reference's RansacCPU,
+// pose_estimate_cpu.cpp:30), its hypotheses scored on the GPU.
 std::unique_ptr<compute::HipStage> CreateMatchStage(std::shared_ptr<compute::Device> device,
                                                     compute::Store& storage,
-                                                    std::shared_ptr<FeatureBuffers> buffers);
+                                                    std::shared_ptr<FeatureBuffers> buffers,
+                                                    bool ransac = false);
 
 }  // namespace wslam

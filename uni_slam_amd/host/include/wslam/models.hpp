@@ -67,4 +67,18 @@ using FeatureSet = std::array<std::vector<Feature>, kLevelsOfDetail>;
 // Per prev-frame level: current-frame feature -> matched prev-frame feature.
 using MatchResult = std::array<std::map<Feature, Feature, FeatureLess>, kLevelsOfDetail>;
 
+// Output of the RANSAC pass (models.hpp:130-145 of the reference).
+struct RansacStats {
+    size_t total_matches = 0;
+    size_t inlier_count = 0;
+    size_t iterations_run = 0;
+    bool model_found = false;
+};
+
+struct RansacResult {
+    MatchResult inliers;
+    std::array<std::array<double, 3>, 3> fundamental_matrix{};
+    RansacStats stats;
+};
+
 }  // namespace wslam

@@ -1,5 +1,6 @@
 // Feature path passes on a HIP device (see feature_detect_hip.hpp).
 #include "wslam/feature_detect_hip.hpp"
+#include "wslam/ransac_hip.hpp"
 
 #include <hip/hip_runtime_api.h>
 
@@ -291,11 +292,14 @@ std::unique_ptr<compute::HipStage> CreateFeatureDetectStage(
 
 std::unique_ptr<compute::HipStage> CreateMatchStage(std::shared_ptr<compute::Device> device,
                                                     compute::Store& storage,
-                                                    std::shared_ptr<FeatureBuffers> buffers) {
+                                                    std::shared_ptr<FeatureBuffers> buffers,
+                                                    bool ransac) {
     auto stage = std::make_unique<compute::HipStage>("Pose estimate", device);
     stage->add_pass(std::unique_ptr<compute::Pass>(new LoadFeaturesHipPass(storage, buffers)));
     stage->add_pass(std::unique_ptr<compute::HIPPass>(new MatchFeaturesHipPass(device, buffers)));
     stage->add_pass(std::unique_ptr<compute::Pass>(new PublishMatchesPass(storage, buffers)));
+    if (ransac)
+        stage->add_pass(std::unique_ptr<compute::Pass>(new RansacHipPass(storage, device, buffers)));
     return stage;
 }
 

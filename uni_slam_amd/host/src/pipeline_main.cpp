@@ -3,13 +3,16 @@
 // frame, the feature counts and the MatchResult in its map order.
 //
 //   wslam_pipeline FRAMES.raw W H N CAP THRESHOLD RATIO OUT.txt
-//                  [--f32] [--graph] [--advance BITS]
+//                  [--f32] [--graph] [--ransac] [--advance BITS]
 //
 // FRAMES.raw holds N frames of W*H u8 pixels back to back.
 //   --f32      publish each frame as the reference's FrameBW (f32 pixels
 //              u8 / 255, data/src/provider_base.cpp:121-124) instead of FrameU8;
 //   --graph    run both stages with HIP graph capture (one graph per buffer
 //              slot combination, replayed);
+//   --ransac   end the match stage with RansacHipPass (hypotheses scored on
+//              the GPU) and print "ransac T total iterations inliers found"
+//              per frame that has a reference set;
 //   --advance  BITS has one character per frame: '1' sets the loader's
 //              advance marker for that frame (the reference set moves to the
 //              previous frame), '0' leaves it unset (the reference keyframe is
@@ -29,12 +32,12 @@ using namespace wslam;
 int main(int argc, char** argv) {
     if (argc < 9) {
         std::cerr << "usage: wslam_pipeline FRAMES.raw W H N CAP THRESHOLD RATIO OUT.txt "
-                     "[--f32] [--graph] [--advance BITS]\n";
+                     "[--f32] [--graph] [--ransac] [--advance BITS]\n";
         return 2;
     }
     const std::string in = argv[1];
     const uint32_t W = std::stoul(argv[2]), H = std::stoul(argv[3]), N = std::stoul(argv[4]);
-    bool f32 = false, graph = false;
+    bool f32 = false, graph = false, ransac = false;
     std::string advance(N, '1');
     for (int i = 9; i < argc; ++i) {
         const std::string a = argv[i];
@@ -42,6 +45,8 @@ int main(int argc, char** argv) {
             f32 = true;
         } else if (a == "--graph") {
             graph = true;
+        } else if (a == "--ransac") {
+            ransac = true;
         } else if (a == "--advance" && i + 1 < argc) {
             advance = argv[++i];
             if (advance.size() != N) {
@@ -73,7 +78,7 @@ int main(int argc, char** argv) {
     compute::Compute comp;
     auto buffers = std::make_shared<FeatureBuffers>(device, opts);
     auto detect = CreateFeatureDetectStage(device, comp.getStorage(), buffers);
-    auto match = CreateMatchStage(device, comp.getStorage(), buffers);
+    auto match = CreateMatchStage(device, comp.getStorage(), buffers, ransac);
     detect->enableGraphCapture(graph);
     match->enableGraphCapture(graph);
     comp.addStage(std::move(detect));
@@ -117,6 +122,18 @@ int main(int argc, char** argv) {
         out << "frame " << t;
         for (const auto& lvl : **cur) out << " " << lvl.size();
         out << "\n";
+        if (auto r = store.take<RansacResult>(ResourceIdentifier::RansacResultName)) {
+            out << "ransac " << t << " " << r->stats.total_matches << " "
+                << r->stats.iterations_run << " " << r->stats.inlier_count << " "
+                << (r->stats.model_found ? 1 : 0) << "\n";
+            size_t kept = 0;
+            for (const auto& lvl : r->inliers) kept += lvl.size();
+            if (kept != r->stats.inlier_count) {
+                std::cerr << "frame " << t << ": RansacResult holds " << kept << " of "
+                          << r->stats.inlier_count << " inliers\n";
+                return 1;
+            }
+        }
         if (auto m = store.take<MatchResult>(ResourceIdentifier::MatchedFeaturesName)) {
             for (uint32_t l = 0; l < kLevelsOfDetail; ++l)
                 for (const auto& [b, a] : (*m)[l])

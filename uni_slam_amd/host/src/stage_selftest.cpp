@@ -2,13 +2,17 @@
 // in insertion order and flushed before each CPU pass, kStageStopExecution
 // ends a stage without error, kComputeStopExecution ends the iteration,
 // other errors carry the pass id.  Exit status 0 when all checks hold.
+#include <algorithm>
+#include <cmath>
 #include <iostream>
+#include <random>
 #include <string>
 #include <vector>
 
 #include "compute/compute.hpp"
 #include "compute/hip_stage.hpp"
 #include "wslam/feature_detect_hip.hpp"
+#include "wslam/ransac_hip.hpp"
 
 using namespace wslam::compute;
 
@@ -125,6 +129,61 @@ int main() {
         FrameU8 u{16, 16, 0, std::vector<uint8_t>(256, 9)};
         s.set("u", u);
         expect(!StageFrameU8(s, "u", 16, 16, out) && out[200] == 9, "u8 frame stages");
+    }
+    {
+        // The 8-point fit of the RANSAC pass recovers a known fundamental
+        // matrix from exact two-view correspondences: cameras K[I|0] and
+        // K[R|t], F = K^-T [t]_x R K^-1, compared up to scale and sign.
+        using namespace wslam;
+        const double f = 900.0, cx = 640.0, cy = 360.0, a = 0.1;
+        const double R[9] = {std::cos(a), 0, std::sin(a), 0, 1, 0, -std::sin(a), 0, std::cos(a)};
+        const double t[3] = {1.0, 0.2, -0.1};
+        std::mt19937_64 g(3);
+        std::uniform_real_distribution<double> ux(-3, 3), uy(-2, 2), uz(4, 12);
+        std::vector<Point2> p, c;
+        for (int i = 0; i < 60; ++i) {
+            const double X[3] = {ux(g), uy(g), uz(g)};
+            p.push_back({f * X[0] / X[2] + cx, f * X[1] / X[2] + cy});
+            double Y[3];
+            for (int r = 0; r < 3; ++r) Y[r] = R[3 * r] * X[0] + R[3 * r + 1] * X[1] + R[3 * r + 2] * X[2] + t[r];
+            c.push_back({f * Y[0] / Y[2] + cx, f * Y[1] / Y[2] + cy});
+        }
+        // E = [t]_x R, F = Ki^T E Ki with Ki = K^-1
+        const double tx[9] = {0, -t[2], t[1], t[2], 0, -t[0], -t[1], t[0], 0};
+        double E[9] = {0}, Ki[9] = {1 / f, 0, -cx / f, 0, 1 / f, -cy / f, 0, 0, 1}, T[9] = {0}, F[9] = {0};
+        for (int r = 0; r < 3; ++r)
+            for (int q = 0; q < 3; ++q)
+                for (int k = 0; k < 3; ++k) E[3 * r + q] += tx[3 * r + k] * R[3 * k + q];
+        for (int r = 0; r < 3; ++r)
+            for (int q = 0; q < 3; ++q)
+                for (int k = 0; k < 3; ++k) T[3 * r + q] += E[3 * r + k] * Ki[3 * k + q];
+        for (int r = 0; r < 3; ++r)
+            for (int q = 0; q < 3; ++q)
+                for (int k = 0; k < 3; ++k) F[3 * r + q] += Ki[3 * k + r] * T[3 * k + q];
+        const auto normed = [](const double* m, double* out) {
+            double n = 0.0, big = 0.0;
+            for (int i = 0; i < 9; ++i) {
+                n += m[i] * m[i];
+                if (std::fabs(m[i]) > std::fabs(big)) big = m[i];
+            }
+            for (int i = 0; i < 9; ++i) out[i] = m[i] / std::sqrt(n) * (big < 0 ? -1.0 : 1.0);
+        };
+        double want[9];
+        normed(F, want);
+        for (size_t n : {size_t(8), size_t(60)}) {
+            const std::vector<Point2> pp(p.begin(), p.begin() + n), cc(c.begin(), c.begin() + n);
+            const auto got = FitFundamental(pp, cc);
+            expect(got.has_value(), "8-point fit on exact data");
+            if (got) {
+                double g2[9], err = 0.0;
+                normed(got->data(), g2);
+                for (int i = 0; i < 9; ++i) err = std::max(err, std::fabs(g2[i] - want[i]));
+                expect(err < 1e-9, "8-point fit recovers F (" + std::to_string(n) + " points)");
+            }
+        }
+        expect(!FitFundamental(std::vector<Point2>(p.begin(), p.begin() + 7),
+                               std::vector<Point2>(c.begin(), c.begin() + 7)),
+               "fewer than 8 correspondences do not fit");
     }
     if (failures == 0) std::cout << "stage selftest ok\n";
     return failures == 0 ? 0 : 1;

@@ -91,6 +91,9 @@ _SIGNATURES = [
     ("orbfe_match_bf_noalloc_async", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
                                                C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
                                                C.c_void_p]),
+    ("orbfe_ransac_score_async", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
+                                           C.c_void_p, C.c_uint32, C.c_double, C.c_void_p,
+                                           C.c_void_p, C.c_void_p]),
     ("orbfe_timing_enable", C.c_int, [C.c_void_p, C.c_int]),
     ("orbfe_timing_read", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_uint32),
                                     C.POINTER(C.c_double)]),
@@ -260,6 +263,14 @@ class Context:
         self._check(lib().orbfe_match_bf_noalloc_async(self._h, _ptr(query), nq, _ptr(train),
                                                        nt, _ptr(idx), _ptr(d1), _ptr(d2),
                                                        _stream(stream)))
+
+    def ransac_score_async(self, F, k: int, prev, curr, n: int, threshold: float, counts,
+                           mask=None, stream=None):
+        """Inlier counts (and optionally masks) of k fundamental matrices over
+        n correspondences (device tensors: F (k, 9) f64, prev/curr (n, 2) f64)."""
+        self._check(lib().orbfe_ransac_score_async(self._h, _ptr(F), k, _ptr(prev), _ptr(curr),
+                                                   n, threshold, _ptr(counts), _ptr(mask),
+                                                   _stream(stream)))
 
     # -- host-memory forms ---------------------------------------------------
     def detect(self, image: np.ndarray, with_pyramid: bool = False):
