@@ -108,6 +108,64 @@ def cpu_baseline(frames, n_sample):
     return n_sample / dt, dt
 
 
+def cpu_info():
+    """Host CPU model, logical CPUs and the CPUs this process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "cpus_available": avail}
+
+
+def median_time(fn, runs=5):
+    fn()  # warm-up
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def cpu_baseline_pair(synth):
+    """The oracle on the 752x480 frame pair (detect both frames at cap 1000,
+    match with ratio 0.8): pairs/s and frames/s, one thread, median of 5."""
+    from oracle import oracle as O
+    a, b, _ = synth.frame_pair(0, 752, 480)
+    p = O.params(752, 480, threshold=thr20(), cap=1000)
+
+    def run():
+        fa, _ = O.detect_frame(a, p)
+        fb, _ = O.detect_frame(b, p)
+        A, ca = O.load_features(fa, LEVELS, 1000)
+        B, cb = O.load_features(fb, LEVELS, 1000)
+        O.find_matches(A, ca, B, cb, 752, 480, RATIO)
+    t = median_time(run)
+    return {"value": round(1.0 / t, 4), "unit": "frame pairs/s", "frames_per_s": round(2.0 / t, 4),
+            "cores": 1, "kind": "port", "runs": 5, "seconds_per_pair": round(t, 4),
+            "sample": "the 752x480 pair (detect both at cap 1000, match at ratio 0.8)"}
+
+
+def cpu_baseline_matcher(synth):
+    """The oracle's brute-force matcher on 1024 queries x 65536 train
+    descriptors (a bounded sample of the 65536^2 workload), one thread,
+    median of 5: pairs/s."""
+    from oracle import oracle as O
+    q, t, _, _ = synth.descriptors(65536, 1024, 65536)
+    tm = median_time(lambda: O.match_bf(q, t))
+    return {"value": round(1024 * 65536 / tm, 1), "unit": "pairs/s", "cores": 1, "kind": "port",
+            "runs": 5, "sample": "1024 queries x 65536 train (%.3f s per run)" % tm}
+
+
 # MI355X published dense FP32 vector rate: 157.3 TFLOP/s = 78.6 T FMA lanes/s,
 # i.e. 2 wave64 VALU instructions per CU per clock at 2.4 GHz.  The integer
 # rooflines use the sustained rate measured on the device itself
@@ -147,6 +205,7 @@ def phase_batched_sequence(torch, orbfe, ctx, d_frames, feats, matches, reps):
                "features": feats[t]} for t in range(N_FRAMES)]
     pairs = [{"prev": feats[t - 1], "curr": feats[t], "prev_match": matches[t - 1],
               "width": W, "height": H, "cap": CAP} for t in range(1, N_FRAMES)]
+    want = [f.clone() for f in feats] + [m.clone() for m in matches]  # the plain path's
     b = ctx.batch_create(frames)
     mb = ctx.match_batch_create(pairs)
 
@@ -154,9 +213,14 @@ def phase_batched_sequence(torch, orbfe, ctx, d_frames, feats, matches, reps):
         b.detect_async(stream=stream)
         mb.match_async(stream=stream)
     g = capture_graph(torch, launch)
+    for x in feats + matches:
+        x.fill_(9)
+    g.replay()
+    torch.cuda.synchronize()
+    exact = all(torch.equal(x, w) for x, w in zip(feats + matches, want))
     dt = timed(torch, g.replay, reps)
     return {"frames_per_s": round(N_FRAMES / dt, 1), "ms_per_step": round(dt * 1e3, 4),
-            "launches_per_step": 5, "graph": True}, (b, mb, g)
+            "launches_per_step": 5, "graph": True, "bit_exact_vs_plain": exact}, (b, mb, g)
 
 
 def phase_mixed(torch, orbfe, synth, reps):
@@ -194,6 +258,7 @@ def phase_mixed(torch, orbfe, synth, reps):
             c.detect_async(db, w, fb, stream=stream)
             c.match_pipeline_async(fa, fb, pm, stream=stream)
     t_plain = timed(torch, plain, reps)
+    want = [f["features"].clone() for f in frames] + [p["prev_match"].clone() for p in pairs]
     tune = autotune.tune_detect(frames, 752, 480, reps=max(3, reps), fast_threshold=thr20(),
                                 lowe_ratio=RATIO)
     mtune = autotune.tune_match(pairs, 752, 480, reps=max(3, reps), fast_threshold=thr20(),
@@ -207,9 +272,15 @@ def phase_mixed(torch, orbfe, synth, reps):
         b.detect_async(stream=st)
         mb.match_async(stream=st)
     g = capture_graph(torch, launch)
+    outs = [f["features"] for f in frames] + [p["prev_match"] for p in pairs]
+    for x in outs:
+        x.fill_(9)
+    g.replay()
+    torch.cuda.synchronize()
+    exact = all(torch.equal(x, w) for x, w in zip(outs, want))
     t_batch = timed(torch, g.replay, reps)
     n = len(frames)
-    return {"frames": n, "pairs": len(pairs),
+    return {"frames": n, "pairs": len(pairs), "bit_exact_vs_plain": exact,
             "plain_frames_per_s": round(n / t_plain, 1),
             "batched_frames_per_s": round(n / t_batch, 1),
             "speedup": round(t_plain / t_batch, 2),
@@ -231,32 +302,35 @@ def valu_peak(ctx):
             "source": "orbfe_probe_valu_rate (8 independent xor/add chains per lane, every CU)"}
 
 
-def phase_matcher(torch, orbfe, synth, n, reps, int_peak_tops):
-    """Brute-force matcher stress: best/second-best over n x n 256-bit descriptors."""
-    q, t, _, _ = synth.descriptors(n, n, n)
+def phase_matcher(torch, orbfe, synth, n, reps, int_peak_tops, nt=None):
+    """Brute-force matcher stress: best/second-best of n query against nt
+    (default n) train 256-bit descriptors."""
+    nq, nt = n, (n if nt is None else nt)
+    q, t, _, _ = synth.descriptors(nq * 7 + nt, nq, nt)
     dev = torch.device("cuda:0")
     dq = torch.from_numpy(q).to(dev)
     dt_ = torch.from_numpy(t).to(dev)
-    idx = torch.empty(n, dtype=torch.int32, device=dev)
+    idx = torch.empty(nq, dtype=torch.int32, device=dev)
     d1 = torch.empty_like(idx)
     d2 = torch.empty_like(idx)
     ctx = orbfe.Context(64, 48)
     stream = torch.cuda.current_stream()
     mfma_probe = ctx.probe_mfma_rate(stream=stream) / 1e15
-    dt = timed(torch, lambda: ctx.match_bf_async(dq, n, dt_, n, idx, d1, d2, stream=stream), reps)
+    dt = timed(torch, lambda: ctx.match_bf_async(dq, nq, dt_, nt, idx, d1, d2, stream=stream),
+               reps)
     ctx.timing(True)
     ctx.timing_reset()
     for _ in range(reps):
-        ctx.match_bf_async(dq, n, dt_, n, idx, d1, d2, stream=stream)
+        ctx.match_bf_async(dq, nq, dt_, nt, idx, d1, d2, stream=stream)
     torch.cuda.synchronize()
     launches, ms = ctx.timing_read("k_bf_mfma")
     ctx.timing(False)
     kern = ms / launches * 1e-3
-    pairs = n * n
+    pairs = nq * nt
     # k_bf_mfma forms every distance as an int8 dot product of two 256-sign
     # vectors on the matrix cores: 256 multiply-adds = 512 int8 ops per pair.
     mfma_pops = 512 * pairs / kern / 1e15
-    return {"n_query": n, "n_train": n, "ms": round(dt * 1e3, 4),
+    return {"n_query": nq, "n_train": nt, "ms": round(dt * 1e3, 4),
             "pairs_per_s": round(pairs / dt, 1),
             "effective_GBps": round(32 * pairs / dt / 1e9, 1),
             "mfma_int8_Pops": round(mfma_pops, 3),
@@ -269,6 +343,83 @@ def phase_matcher(torch, orbfe, synth, n, reps, int_peak_tops):
             "xor_popcount_equiv_Tops": round(16 * pairs / kern / 1e12, 2),
             "int_peak_Tops": round(int_peak_tops, 2),
             "k_bf_mfma_us": round(kern * 1e6, 2)}
+
+
+def phase_matcher_grid(torch, orbfe, synth, reps):
+    """Every N x M combination of the matcher stress workload (N, M in
+    {4096, 16384, 65536}): pairs/s (matches/s) of the whole call."""
+    out = {}
+    for nq in (4096, 16384, 65536):
+        for nt in (4096, 16384, 65536):
+            r = phase_matcher(torch, orbfe, synth, nq, reps, 0.0, nt=nt)
+            out[f"{nq}x{nt}"] = {"ms": r["ms"], "pairs_per_s": r["pairs_per_s"],
+                                 "mfma_int8_Pops": r["mfma_int8_Pops"]}
+    return out
+
+
+def phase_readback(torch, orbfe, ctx, d_frames, feats, reps):
+    """The readback of one detected 1280x720 frame into the host FeatureSet:
+    before, the whole feature slab copied to pinned memory and converted on
+    the CPU (orbfe_load_features, the FromGpuModel loop); after, the export
+    kernel writing the wslam::Feature records straight into pinned memory
+    (orbfe_export_features_async).  Microseconds per frame over the
+    sequence, detect excluded; both give the same records."""
+    stream = torch.cuda.current_stream()
+    slab = torch.empty(ctx.feature_bytes, dtype=torch.uint8).pin_memory()
+    out = torch.empty(LEVELS * CAP * 13, dtype=torch.int32).pin_memory()
+    counts = torch.empty(LEVELS, dtype=torch.int32).pin_memory()
+    for t in range(N_FRAMES):
+        ctx.detect_async(d_frames[t], W, feats[t], stream=stream)
+    torch.cuda.synchronize()
+
+    def before():
+        for t in range(N_FRAMES):
+            slab.copy_(feats[t], non_blocking=True)
+            torch.cuda.synchronize()
+            rec, _ = orbfe.load_features(slab.numpy(), LEVELS, CAP)
+        return rec
+
+    def after():
+        for t in range(N_FRAMES):
+            ctx.export_features_async(feats[t], out, counts, stream=stream)
+            torch.cuda.synchronize()
+        n = int(counts.numpy().sum())
+        return out.numpy().view(np.uint8)[:52 * n].view(orbfe.FEATURE_DTYPE)
+
+    same = np.array_equal(before().view(np.uint8), after().view(np.uint8))
+    tb = median_time(before, reps) / N_FRAMES
+    ta = median_time(after, reps) / N_FRAMES
+    return {"before_us_per_frame": round(tb * 1e6, 2), "after_us_per_frame": round(ta * 1e6, 2),
+            "speedup": round(tb / ta, 2), "same_records": bool(same),
+            "before": "feature slab D2H to pinned memory + CPU FromGpuModel loop",
+            "after": "export kernel writes wslam::Feature records into pinned memory"}
+
+
+def phase_ransac(torch, orbfe, reps):
+    """RANSAC hypothesis scoring: 500 fundamental matrices (the reference's
+    max_iterations) against 8000 correspondences in one launch
+    (orbfe_ransac_score_async), beside the oracle's scalar loop on one
+    thread; hypotheses x correspondences per second."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(5)
+    n, k = 8000, 500
+    prev = rng.uniform(0, 1280, (n, 2))
+    curr = prev + rng.normal(0, 3, (n, 2))
+    F = rng.normal(size=(k, 9)) * 1e-3
+    ctx = orbfe.Context(64, 48)
+    dF = torch.from_numpy(F).cuda()
+    dp = torch.from_numpy(prev).cuda()
+    dc = torch.from_numpy(curr).cuda()
+    counts = torch.empty(k, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+    dt = timed(torch, lambda: ctx.ransac_score_async(dF, k, dp, dc, n, 3.84, counts,
+                                                     stream=stream), reps)
+    same = np.array_equal(counts.cpu().numpy().view(np.uint32),
+                          O.sampson_counts(F, prev, curr, 3.84))
+    tc = median_time(lambda: O.sampson_counts(F[:50], prev, curr, 3.84), 3) * (k / 50)
+    return {"hypotheses": k, "correspondences": n, "gpu_ms": round(dt * 1e3, 4),
+            "gpu_evals_per_s": round(k * n / dt, 1), "cpu_evals_per_s": round(k * n / tc, 1),
+            "cpu_cores": 1, "counts_equal_oracle": bool(same)}
 
 
 def phase_serving(torch, orbfe, synth, seconds, seed=4):
@@ -305,6 +456,32 @@ def phase_serving(torch, orbfe, synth, seconds, seed=4):
     wall_p = serve.plain_backlog(specs, arr2, pool, fast_threshold=thr20(), lowe_ratio=RATIO)
     out["plain_frames_per_s"] = round(len(arr2) / wall_p, 1)
     out["saturated_vs_plain"] = round(wall_p / wall2, 2)
+    # parity: a short served run of all 32 streams (8 frames each, compressed
+    # schedule) with outputs kept, against the plain per-frame path
+    srv3 = serve.Server(specs, fast_threshold=thr20(), lowe_ratio=RATIO, hbm_budget=2 << 30,
+                        keep_outputs=True)
+    arr3 = [a for a in serve.poisson_arrivals(rates, 10.0, seed + 2) if a[2] < 8]
+    serve.run_schedule(srv3, arr3, pool, time_scale=0.0)
+    exact = len(srv3.outputs) == len(arr3)
+    ctxs = {}
+    for s, sp in enumerate(specs):
+        key = (sp.width, sp.height)
+        if key not in ctxs:
+            ctxs[key] = orbfe.Context(sp.width, sp.height, features_per_level=sp.cap,
+                                      fast_threshold=thr20(), lowe_ratio=RATIO)
+        c = ctxs[key]
+        prev = None
+        for k in range(8):
+            feats, _ = c.detect(pool[s][k % len(pool[s])].numpy())
+            got_f, got_m = srv3.outputs[(s, k)]
+            exact &= bool(np.array_equal(got_f, feats))
+            if prev is not None:
+                exact &= bool(np.array_equal(got_m, c.match_pipeline(prev, feats)[0]))
+            prev = feats
+    for c in ctxs.values():
+        c.close()
+    out["bit_exact_vs_plain"] = exact
+    out["parity_sample"] = f"{len(arr3)} served frames of the 32 streams vs the plain path"
     return out
 
 
@@ -318,11 +495,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the batched / mixed / matcher / serving phases")
-    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-seconds", type=float, default=60.0,
+                    help="length of the real-time serving schedule (the workload's 60 s)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device-to-device copy (counter calibration)")
     ap.add_argument("--workload", default="plain",
-                    choices=["plain", "batched", "mixed", "matcher", "serving"],
+                    choices=["plain", "batched", "mixed", "matcher", "matcher_grid", "serving",
+                             "readback", "ransac"],
                     help="plain: the full report; the others run only that phase and "
                          "print its JSON (for profiling one path)")
     args = ap.parse_args()
@@ -364,6 +543,12 @@ def main():
             peak = valu_peak(ctx)
             out = phase_matcher(torch, orbfe, synth, 65536, max(2, args.steps), peak["int_Tops"])
             out["valu_peak"] = peak
+        elif args.workload == "matcher_grid":
+            out = phase_matcher_grid(torch, orbfe, synth, max(2, args.steps))
+        elif args.workload == "readback":
+            out = phase_readback(torch, orbfe, ctx, d_frames, feats, 5)
+        elif args.workload == "ransac":
+            out = phase_ransac(torch, orbfe, max(2, args.steps))
         else:
             out = phase_serving(torch, orbfe, synth, args.serve_seconds)
         print(json.dumps({"workload": args.workload, **out}), flush=True)
@@ -418,9 +603,11 @@ def main():
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
                 "traffic": measured_traffic(dom),
                 "valu": valu_roofline(dom, per_kernel[dom]["avg_us"], peak["wave_instr_per_s"]),
-                "limiter": "per-tile latency of barrier-separated phases; VALU issue about 0.58 "
-                           "of the probed peak in 32-frame batches, 0.36 on one frame (neither "
-                           "HBM- nor VALU-bound; profiles/valu_1280x720.md)",
+                "limiter": "priced against HBM (the roofline of its algorithmic bytes); what "
+                           "limits it is instruction issue: f32 bilinear and 9-tap smoothing "
+                           "arithmetic, VALU issue about 0.58 of the probed peak in 32-frame "
+                           "batches and 0.36 on one frame, whose 2709 tiles fill the device "
+                           "1.3 times (profiles/valu_1280x720.md)",
                 "avg_us": per_kernel[dom]["avg_us"], "bytes_per_launch": kb[dom]}
 
     extra = {}
@@ -435,6 +622,12 @@ def main():
         extra["valu_peak"] = peak
         extra["matcher"] = phase_matcher(torch, orbfe, synth, 65536, 3, peak["int_Tops"])
         log("matcher:", json.dumps(extra["matcher"]))
+        extra["matcher_grid"] = phase_matcher_grid(torch, orbfe, synth, 3)
+        log("matcher grid:", json.dumps(extra["matcher_grid"]))
+        extra["readback"] = phase_readback(torch, orbfe, ctx, d_frames, feats, 5)
+        log("readback:", json.dumps(extra["readback"]))
+        extra["ransac"] = phase_ransac(torch, orbfe, 5)
+        log("ransac:", json.dumps(extra["ransac"]))
         extra["serving"] = phase_serving(torch, orbfe, synth, args.serve_seconds)
         log("serving:", json.dumps(extra["serving"]))
 
@@ -443,8 +636,13 @@ def main():
         cfps, cdt = cpu_baseline(frames, args.cpu_sample)
         cpu = {"value": round(cfps, 4), "unit": "frames/s", "cores": 1, "kind": "port",
                "sample": f"{args.cpu_sample} frames of the 1280x720 sequence (cycled): detect "
-                         f"each, match {args.cpu_sample - 1} consecutive pairs ({cdt:.1f} s)"}
+                         f"each, match {args.cpu_sample - 1} consecutive pairs ({cdt:.1f} s)",
+               "threads": 1, **cpu_info()}
         log(f"cpu baseline: {cfps:.3f} frames/s")
+        extra["cpu_baseline_pair_752x480"] = cpu_baseline_pair(synth)
+        log("cpu pair:", json.dumps(extra["cpu_baseline_pair_752x480"]))
+        extra["cpu_baseline_matcher"] = cpu_baseline_matcher(synth)
+        log("cpu matcher:", json.dumps(extra["cpu_baseline_matcher"]))
 
     out = {
         "metric": "frames/s (plain per-frame HIP path: detect + match vs previous frame)",
