@@ -306,30 +306,52 @@ struct SetupItems {
     static constexpr uint32_t kMask = kSrow + TH;
 };
 
-// Loads of a tile's set-up item e (raw dwords) and of its first kPF source
-// rows per wave, issued together before any of them is stored.
-template <int TH, uint32_t kPF>
-__device__ inline void prefetch_tile(const DetectTile& T, const Params& prm, uint32_t tid,
-                                     uint32_t (&pf)[kPF], uint2& item) {
-    using SI = SetupItems<TH>;
+// Direct-to-LDS loads (global_load_lds_dword) of a tile's u8 source
+// rectangle: dword e of the row-major rectangle (nd dwords per row) goes to
+// src[e], lane e % 64 of round e / 64, rounds dealt to the waves.  They run
+// in the background; the consumer waits for them (vmcnt) and a barrier.
+// Tiles whose rectangle does not fit the buffer load nothing (their texels
+// are read from the frame directly).
+template <int TH>
+__device__ inline void issue_source(const DetectTile& T, uint32_t* src, uint32_t tid) {
+    if (!T.use_src) return;  // block-uniform
     constexpr uint32_t kWaves = kDetectThreads / 64;
     const uint32_t lane = tid & 63, wave = tid >> 6;
-    // Unconditional loads from clamped (in-range) addresses; the stores of
-    // the staging step pick the rows and lanes that belong to the rectangle.
-    const uint32_t* base = reinterpret_cast<const uint32_t*>(T.J.img);
-    uint32_t dw = 0, dpitch = 0, rmax = 0;
-    if (T.use_src) {
-        dw = static_cast<uint32_t>((T.cs + T.rowmin * T.J.pitch) / 4) + umin(lane, T.nd - 1);
-        dpitch = static_cast<uint32_t>(T.J.pitch) / 4;
-        rmax = T.nrows - 1;
+    const uint32_t total = T.nd * T.nrows;  // <= stage_words<TH>()
+    // row = floor(e / nd) through a reciprocal: e < 2^13 and nd <= 45 keep
+    // (e + 0.5) / nd at least 1/90 from an integer, far above the relative
+    // rounding error (~2^-23) of the product, so the floor is exact.
+    const float inv_nd = 1.0f / static_cast<float>(T.nd);
+    const ORBFE_GLOBAL uint8_t* img = reinterpret_cast<const ORBFE_GLOBAL uint8_t*>(
+        reinterpret_cast<uintptr_t>(T.J.img));
+    for (uint32_t r = wave; r * 64 < total; r += kWaves) {
+        const uint32_t e = r * 64 + lane;
+        if (e < total) {
+            const uint32_t row = static_cast<uint32_t>((static_cast<float>(e) + 0.5f) * inv_nd);
+            const uint32_t col = e - row * T.nd;
+            const ORBFE_GLOBAL uint8_t* g =
+                img + (static_cast<uint64_t>(T.rowmin + row) * T.J.pitch + T.cs + 4u * col);
+            __builtin_amdgcn_global_load_lds(
+                reinterpret_cast<const ORBFE_GLOBAL void*>(g),
+                reinterpret_cast<__attribute__((address_space(3))) void*>(
+                    reinterpret_cast<uintptr_t>(src + r * 64)),
+                4, 0, 0);
+        }
     }
-#pragma unroll
-    for (uint32_t k = 0; k < kPF; ++k)
-        pf[k] = load_u32(base, dw + umin(wave + k * kWaves, rmax) * dpitch);
+}
+
+// Waits for this wave's direct-to-LDS loads.
+__device__ inline void wait_lds_dma() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// The set-up item e = tid of a tile (raw dwords): a column or row level tap,
+// a row tap of the 9-tap Gaussian or a cull-mask word.
+template <int TH>
+__device__ inline uint2 setup_item(const DetectTile& T, const Params& prm, uint32_t tid) {
+    using SI = SetupItems<TH>;
     ConstPlan P = as_const(T.J.plan);
     const uint32_t e = tid;
     const uint32_t w = T.G.w, h = T.G.h;
-    item = make_uint2(0u, 0u);
+    uint2 item = make_uint2(0u, 0u);
     if (e < SI::kTap) {
         const bool is_col = e < kLdsW;
         const uint32_t c = is_col ? e : e - kLdsW;
@@ -361,6 +383,7 @@ __device__ inline void prefetch_tile(const DetectTile& T, const Params& prm, uin
             item = make_uint2(static_cast<uint32_t>(m), static_cast<uint32_t>(m >> 32));
         }
     }
+    return item;
 }
 
 // One workgroup per tile.
@@ -371,8 +394,6 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     constexpr int RPT = TH / (kDetectThreads / 64);        // rows per thread
     constexpr uint32_t kSrcWords = stage_words<TH>();
     constexpr uint32_t kWaves = kDetectThreads / 64;
-    // first source rows per wave loaded ahead (level scales up to 2.5)
-    constexpr uint32_t kPF = (LH * 5 / 2 + 4 + kWaves - 1) / kWaves;
     using SI = SetupItems<TH>;
     static_assert(SI::kMask <= kDetectThreads, "one set-up item per thread");
     __shared__ float lut[256];
@@ -402,9 +423,8 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
 
     const DetectTile T =
         decode_tile<TH>(bv, job0, prm, bv.n ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x);
-    uint32_t pf[kPF];
-    uint2 item;
-    prefetch_tile<TH, kPF>(T, prm, tid, pf, item);
+    issue_source<TH>(T, src, tid);
+    const uint2 item = setup_item<TH>(T, prm, tid);
     const Job& J = T.J;
     ConstPlan P = as_const(J.plan);
     const LevelGeom& G = T.G;
@@ -416,9 +436,8 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     const TapEntry* taps = P.taps;
     const uint32_t c_first = T.c_first, r_first = T.r_first;
     const uint32_t c_last = T.c_last, r_last = T.r_last;
-    const uint32_t rowmin = T.rowmin, cs = T.cs, nd = T.nd, nrows = T.nrows;
+    const uint32_t rowmin = T.rowmin, cs = T.cs, nd = T.nd;
     const bool use_src = T.use_src;
-    const uint8_t* col0 = J.img + cs;
 
     // Set-up items and the row-0 neighbour texels into LDS.
     const bool has_row0 = y0 == 0 && tile_x < G.wpr;
@@ -470,34 +489,9 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
             row0_ring[k][i] = v;
         }
     }
-    // Source rows loaded above, then the rows past the first kPF per wave
-    // (taller rectangles): batches of kBatch loads, each issued before its
-    // LDS stores.
-    {
-        const bool src_lane = use_src && lane < nd;
-#pragma unroll
-        for (uint32_t k = 0; k < kPF; ++k) {
-            const uint32_t rr = wave + k * kWaves;
-            if (src_lane && rr < nrows) src[rr * nd + lane] = pf[k];
-        }
-    }
-    if (use_src) {
-        constexpr uint32_t kBatch = 8;
-        for (uint32_t r0 = wave + kPF * kWaves; r0 < nrows; r0 += kWaves * kBatch) {
-            uint32_t v[kBatch];
-#pragma unroll
-            for (uint32_t k = 0; k < kBatch; ++k) {
-                const uint32_t rr = r0 + k * kWaves;
-                if (rr < nrows && lane < nd)
-                    v[k] = reinterpret_cast<const uint32_t*>(col0 + (rowmin + rr) * J.pitch)[lane];
-            }
-#pragma unroll
-            for (uint32_t k = 0; k < kBatch; ++k) {
-                const uint32_t rr = r0 + k * kWaves;
-                if (rr < nrows && lane < nd) src[rr * nd + lane] = v[k];
-            }
-        }
-    }
+    // The source rectangle's direct-to-LDS loads land: this wave's, then
+    // everybody's.
+    wait_lds_dma();
     __syncthreads();
     const uint32_t spitch = nd * 4;
     const uint8_t* srcb = reinterpret_cast<const uint8_t*>(src);
