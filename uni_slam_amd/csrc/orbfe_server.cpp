@@ -344,6 +344,7 @@ void orbfe_server_destroy(orbfe_server* s) {
         for (auto* p : st.match) (void)hipFree(p);
     }
     for (orbfe_ctx* c : s->ctxs) orbfe_destroy(c);
+    for (hipStream_t h : s->hstreams) (void)hipStreamDestroy(h);
     for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
     s->event_pool.clear();
     delete s;
