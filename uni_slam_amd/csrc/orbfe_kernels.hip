@@ -121,6 +121,15 @@ __device__ inline float load_f32(const uint8_t* base, uint32_t off) {
     return *reinterpret_cast<const ORBFE_GLOBAL float*>(g + off);
 }
 
+// The global-address-space view of a device pointer.  Accesses through it
+// are global_* instructions (uniform base in SGPRs, 32-bit VGPR offsets)
+// instead of flat ones, which need 64-bit addresses per lane and also count
+// against the counter LDS reads wait on.
+template <typename T>
+__device__ inline ORBFE_GLOBAL T* gptr(T* p) {
+    return reinterpret_cast<ORBFE_GLOBAL T*>(reinterpret_cast<uintptr_t>(p));
+}
+
 // Element i of a global array with a uniform base (32-bit offsets).
 __device__ inline uint32_t load_u32(const uint32_t* base, uint32_t i) {
     return reinterpret_cast<const ORBFE_GLOBAL uint32_t*>(reinterpret_cast<uintptr_t>(base))[i];
@@ -159,13 +168,14 @@ __device__ inline float level_texel_from_frame(const Job& J, ConstPlan P,
                                                const LevelGeom& G, uint32_t l,
                                                const float* lut, uint32_t x,
                                                uint32_t y) {
-    if (l == 0) return lut[J.img[y * J.pitch + x]];
+    const ORBFE_GLOBAL uint8_t* img = gptr(J.img);
+    if (l == 0) return lut[img[y * J.pitch + x]];
     const float u = (static_cast<float>(x) + 0.5f) / static_cast<float>(G.w);
     const float v = (static_cast<float>(y) + 0.5f) / static_cast<float>(G.h);
     const AxisTaps tx = axis_taps(u, P.W0);
     const AxisTaps ty = axis_taps(v, P.H0);
-    const uint8_t* r0 = J.img + ty.i0 * J.pitch;
-    const uint8_t* r1 = J.img + ty.i1 * J.pitch;
+    const ORBFE_GLOBAL uint8_t* r0 = img + ty.i0 * J.pitch;
+    const ORBFE_GLOBAL uint8_t* r1 = img + ty.i1 * J.pitch;
     return lerp2(lut[r0[tx.i0]], lut[r0[tx.i1]], lut[r1[tx.i0]], lut[r1[tx.i1]],
                  tx.frac, ty.frac);
 }
@@ -176,13 +186,14 @@ __device__ inline float u8_unit(uint32_t v) { return static_cast<float>(v) / 255
 // level_texel_from_frame without the LDS table (usable before it is filled).
 __device__ inline float level_texel_direct(const Job& J, ConstPlan P, const LevelGeom& G,
                                            uint32_t l, uint32_t x, uint32_t y) {
-    if (l == 0) return u8_unit(J.img[y * J.pitch + x]);
+    const ORBFE_GLOBAL uint8_t* img = gptr(J.img);
+    if (l == 0) return u8_unit(img[y * J.pitch + x]);
     const float u = (static_cast<float>(x) + 0.5f) / static_cast<float>(G.w);
     const float v = (static_cast<float>(y) + 0.5f) / static_cast<float>(G.h);
     const AxisTaps tx = axis_taps(u, P.W0);
     const AxisTaps ty = axis_taps(v, P.H0);
-    const uint8_t* r0 = J.img + ty.i0 * J.pitch;
-    const uint8_t* r1 = J.img + ty.i1 * J.pitch;
+    const ORBFE_GLOBAL uint8_t* r0 = img + ty.i0 * J.pitch;
+    const ORBFE_GLOBAL uint8_t* r1 = img + ty.i1 * J.pitch;
     return lerp2(u8_unit(r0[tx.i0]), u8_unit(r0[tx.i1]), u8_unit(r1[tx.i0]),
                  u8_unit(r1[tx.i1]), tx.frac, ty.frac);
 }
@@ -509,9 +520,9 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
             return lerp2(lut[srcb[o0 + a0]], lut[srcb[o0 + a1]], lut[srcb[o1 + a0]],
                          lut[srcb[o1 + a1]], ca, rb);
         }
-        const uint8_t* s0 = J.img + ri0 * J.pitch;
+        const ORBFE_GLOBAL uint8_t* s0 = gptr(J.img) + ri0 * J.pitch;
         if (l == 0) return lut[s0[ci0]];
-        const uint8_t* s1 = J.img + ri1 * J.pitch;
+        const ORBFE_GLOBAL uint8_t* s1 = gptr(J.img) + ri1 * J.pitch;
         return lerp2(lut[s0[ci0]], lut[s0[ci1]], lut[s1[ci0]], lut[s1[ci1]], ca, rb);
     };
 
@@ -705,7 +716,7 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
                             sum = sum + sv * wts[i] * wts[jj];
                         }
                     }
-                    J.smooth[G.sm_off + y * sw + x] = sum;
+                    gptr(J.smooth)[G.sm_off + y * sw + x] = sum;
                 }
             } else {
                 const float frs[3] = {fr0, fr1, fr2};
@@ -734,14 +745,14 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
                             sum = sum + sv * wts[i] * wts[jj];
                         }
                     }
-                    J.smooth[G.sm_off + y * sw + x] = sum;
+                    gptr(J.smooth)[G.sm_off + y * sw + x] = sum;
                 }
             }
         }
     } else if (x < sw) {
         for (int k = 0; k < RPT; ++k) {
             const uint32_t y = static_cast<uint32_t>(y0) + wave * RPT + k;
-            if (y < sh) J.smooth[G.sm_off + y * sw + x] = smooth_direct(J, P, G, l, lut, x, y);
+            if (y < sh) gptr(J.smooth)[G.sm_off + y * sw + x] = smooth_direct(J, P, G, l, lut, x, y);
         }
     }
 
@@ -777,7 +788,7 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
         const uint32_t y = static_cast<uint32_t>(y0) + r;
         wscore[k * 64 + lane] = 0;
         const bool inb = x < w && y < h;
-        if (inb && l > 0 && J.pyr) J.pyr[G.pyr_off + y * w + x] = cen[k];
+        if (inb && l > 0 && J.pyr) gptr(J.pyr)[G.pyr_off + y * w + x] = cen[k];
         bool need = inb && tile_x < G.wpr &&
                     (y == 0 ? (x % 9) < 3 : ((mrow[k] >> lane) & 1ull) != 0);
         if (quick)
@@ -859,7 +870,7 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
             }
         }
         const uint64_t bits = __ballot(keep);
-        if (lane == 0) J.surv[G.word_off + y * G.wpr + tile_x] = bits;
+        if (lane == 0) gptr(J.surv)[G.word_off + y * G.wpr + tile_x] = bits;
     }
     }
 }
@@ -943,7 +954,7 @@ __device__ inline uint32_t wave_total(uint32_t v) {
 }
 
 __device__ inline void emit_word(uint64_t v, uint32_t k, uint32_t rank, uint32_t cap,
-                                 uint32_t wpr, uint32_t* kp) {
+                                 uint32_t wpr, ORBFE_GLOBAL uint32_t* kp) {
     if (!v || rank >= cap) return;
     const uint32_t y = k / wpr;
     const uint32_t xb = (k - y * wpr) * 64;
@@ -976,8 +987,8 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(BatchView bv, Job j
     const uint32_t part_chunks = (nchunks + parts - 1) / parts;
     const uint32_t c0 = umin(part * part_chunks, nchunks);
     const uint32_t c1 = umin(c0 + part_chunks, nchunks);
-    const uint64_t* surv = J.surv + G.word_off;
-    uint32_t* kp = J.kp + G.kp_off;
+    const ORBFE_GLOBAL uint64_t* surv = gptr(J.surv) + G.word_off;
+    ORBFE_GLOBAL uint32_t* kp = gptr(J.kp) + G.kp_off;
     const uint32_t cap = P.cap;
     const bool last = part == parts - 1;
 
@@ -1053,7 +1064,7 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(BatchView bv, Job j
         __syncthreads();  // cbase and round_total are rewritten by the next round
     }
     if (last && tid == 0) {
-        uint32_t* hdr = reinterpret_cast<uint32_t*>(J.feat + G.feat_off);
+        ORBFE_GLOBAL uint32_t* hdr = reinterpret_cast<ORBFE_GLOBAL uint32_t*>(gptr(J.feat) + G.feat_off);
         hdr[0] = carry;
         hdr[1] = 0;
     }
@@ -1065,9 +1076,9 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(BatchView bv, Job j
 namespace {
 
 struct LevelSrc {
-    const uint8_t* img;  // level 0
+    const ORBFE_GLOBAL uint8_t* img;  // level 0
     uint64_t pitch;
-    const float* lvl;    // levels >= 1
+    const ORBFE_GLOBAL float* lvl;    // levels >= 1
     const float* lut;
     uint32_t w, h;
     bool base;
@@ -1133,9 +1144,9 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
     const uint32_t gbase = lane & ~(LANES - 1);  // first lane of the group
     const uint32_t slot = g * KPB + tid / LANES;
     const bool in_cap = slot < J.cap;
-    uint8_t* block = J.feat + static_cast<size_t>(l) * J.feat_stride;  // the level's FeatureArray
-    const uint32_t count = reinterpret_cast<const uint32_t*>(block)[0];
-    const uint32_t kp_entry = in_cap ? J.kp[l * J.cap + slot] : 0u;
+    ORBFE_GLOBAL uint8_t* block = gptr(J.feat) + static_cast<size_t>(l) * J.feat_stride;  // the level's FeatureArray
+    const uint32_t count = reinterpret_cast<const ORBFE_GLOBAL uint32_t*>(block)[0];
+    const uint32_t kp_entry = in_cap ? gptr(J.kp)[l * J.cap + slot] : 0u;
     ConstPlan P = as_const(J.plan);
     const LevelGeom G = copy_const(&P.lv[l]);  // by value: stores below cannot alias it
     lut[tid] = static_cast<float>(tid) / 255.0f;
@@ -1148,7 +1159,7 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
 
     const uint32_t kept = count < J.cap ? count : J.cap;
     const bool active = slot < kept;
-    uint32_t* out = reinterpret_cast<uint32_t*>(block + 8 + 48ull * slot);
+    ORBFE_GLOBAL uint32_t* out = reinterpret_cast<ORBFE_GLOBAL uint32_t*>(block + 8 + 48ull * slot);
     if (__ballot(active) == 0) {  // whole wave empty: zero its slots
 #pragma unroll
         for (uint32_t o = 0; o < OUTS; ++o) {
@@ -1162,9 +1173,9 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
     const uint32_t y = kpv >> 16;
 
     LevelSrc S;
-    S.img = J.img;
+    S.img = gptr(J.img);
     S.pitch = J.pitch;
-    S.lvl = J.pyr + G.pyr_off;
+    S.lvl = gptr(J.pyr) + G.pyr_off;
     S.lut = lut;
     S.w = G.w;
     S.h = G.h;
@@ -1360,11 +1371,13 @@ namespace {
 struct FeatView {
     const uint8_t* base;
     uint32_t stride;  // feature array block stride
-    __device__ const uint32_t* rec(uint32_t lod, uint32_t i) const {
-        return reinterpret_cast<const uint32_t*>(base + stride * lod + 8 + 48ull * i);
+    __device__ const ORBFE_GLOBAL uint32_t* rec(uint32_t lod, uint32_t i) const {
+        return reinterpret_cast<const ORBFE_GLOBAL uint32_t*>(gptr(base) + stride * lod + 8 +
+                                                              48ull * i);
     }
     __device__ uint32_t count(uint32_t lod, uint32_t cap) const {
-        const uint32_t c = *reinterpret_cast<const uint32_t*>(base + stride * lod);
+        const uint32_t c =
+            *reinterpret_cast<const ORBFE_GLOBAL uint32_t*>(gptr(base) + stride * lod);
         return c < cap ? c : cap;
     }
 };
@@ -1377,6 +1390,16 @@ constexpr uint32_t kMatchThreads = 256;
 constexpr uint32_t kCandPerLane = 2;
 // Native vector types for loads through the global address space.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// 16-byte global load / store (native vectors: the HIP vector classes have
+// no operators for objects outside the generic address space).
+__device__ inline uint4 gload_u4(const ORBFE_GLOBAL uint32_t* p) {
+    const u32x4 v = *reinterpret_cast<const ORBFE_GLOBAL u32x4*>(p);
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ inline void gstore_u4(ORBFE_GLOBAL uint32_t* p, uint4 v) {
+    *reinterpret_cast<ORBFE_GLOBAL u32x4*>(p) = u32x4{v.x, v.y, v.z, v.w};
+}
 
 // LOD0 position of a record (toLod0, match_features_cpu.cpp:58-64).
 __device__ inline void rec_lod0(const uint32_t* r, float scale, float* x, float* y) {
@@ -1417,8 +1440,8 @@ __global__ __launch_bounds__(kIndexThreads) void k_match_index(MatchBatchView bv
     const MatchJob J = bv.n ? copy_const(const_ptr(bv.jobs + j)) : job0;
     ConstPlan P = as_const(J.plan);
     const FeatView A{side ? J.curr : J.prev, P.feat_stride};
-    uint32_t* base = side ? J.index_curr : J.index_prev;
-    uint32_t* starts = base + l * (kCells + 1);
+    ORBFE_GLOBAL uint32_t* base = gptr(side ? J.index_curr : J.index_prev);
+    ORBFE_GLOBAL uint32_t* starts = base + l * (kCells + 1);
     const uint32_t n = A.count(l, P.cap);
     const float sc = P.lv[l].lod0_scale;
     const float inv_x = P.cell_inv_x, inv_y = P.cell_inv_y;
@@ -1433,10 +1456,10 @@ __global__ __launch_bounds__(kIndexThreads) void k_match_index(MatchBatchView bv
     const auto load = [&](uint32_t i, uint32_t lim) -> Held {
         Held h{};
         if (i < lim) {
-            const uint32_t* r = A.rec(l, i);
-            h.head = *reinterpret_cast<const uint4*>(r);
-            h.lo = *reinterpret_cast<const uint4*>(r + 4);
-            h.hi = *reinterpret_cast<const uint4*>(r + 8);
+            const auto* r = A.rec(l, i);
+            h.head = gload_u4(r);
+            h.lo = gload_u4(r + 4);
+            h.hi = gload_u4(r + 8);
         }
         return h;
     };
@@ -1458,8 +1481,8 @@ __global__ __launch_bounds__(kIndexThreads) void k_match_index(MatchBatchView bv
     // The mutual pass writes only the matched entries of prev_match; the
     // direction pass appends to the query list.
     if (side == 0)
-        for (uint32_t i = tid; i < P.cap; i += kIndexThreads) J.prev_match[l * P.cap + i] = -1;
-    if (side == 0 && l == 0 && tid < kListSegs) J.q_list[tid * kListStride] = 0;
+        for (uint32_t i = tid; i < P.cap; i += kIndexThreads) gptr(J.prev_match)[l * P.cap + i] = -1;
+    if (side == 0 && l == 0 && tid < kListSegs) gptr(J.q_list)[tid * kListStride] = 0;
     for (uint32_t c = tid; c <= kCells; c += kIndexThreads) hist[c] = 0;
     // LOD0 position (toLod0, match_features_cpu.cpp:58-64) and cell
     const auto pos_of = [&](uint32_t rx, uint32_t ry, float* x, float* y) -> uint32_t {
@@ -1483,7 +1506,7 @@ __global__ __launch_bounds__(kIndexThreads) void k_match_index(MatchBatchView bv
     h2 = count(h2, tid + 2 * kIndexThreads);
     h3 = count(h3, tid + 3 * kIndexThreads);
     for (uint32_t i = tid + kIndexHeld * kIndexThreads; i < n; i += kIndexThreads) {
-        const uint32_t* r = A.rec(l, i);
+        const auto* r = A.rec(l, i);
         float x, y;
         atomicAdd(&hist[pos_of(r[0], r[1], &x, &y)], 1u);
     }
@@ -1509,15 +1532,16 @@ __global__ __launch_bounds__(kIndexThreads) void k_match_index(MatchBatchView bv
     }
     __syncthreads();
     const size_t g0 = static_cast<size_t>(l) * P.cap;
-    uint4* shead = reinterpret_cast<uint4*>(base + P.idx_head) + g0;
-    uint4* sdlo = reinterpret_cast<uint4*>(base + P.idx_dlo) + g0;
-    uint4* sdhi = reinterpret_cast<uint4*>(base + P.idx_dhi) + g0;
+    // 16-byte entries, addressed in dwords
+    ORBFE_GLOBAL uint32_t* shead = base + P.idx_head + 4 * g0;
+    ORBFE_GLOBAL uint32_t* sdlo = base + P.idx_dlo + 4 * g0;
+    ORBFE_GLOBAL uint32_t* sdhi = base + P.idx_dhi + 4 * g0;
     const auto scatter = [&](Held h, uint32_t i) {
         if (i < n) {
             const uint32_t k = atomicAdd(&hist[h.cell], 1u);
-            shead[k] = make_uint4(__float_as_uint(h.x), __float_as_uint(h.y), i, 0u);
-            sdlo[k] = h.lo;
-            sdhi[k] = h.hi;
+            gstore_u4(shead + 4 * k, make_uint4(__float_as_uint(h.x), __float_as_uint(h.y), i, 0u));
+            gstore_u4(sdlo + 4 * k, h.lo);
+            gstore_u4(sdhi + 4 * k, h.hi);
         }
     };
     scatter(h0, tid);
@@ -1525,18 +1549,18 @@ __global__ __launch_bounds__(kIndexThreads) void k_match_index(MatchBatchView bv
     scatter(h2, tid + 2 * kIndexThreads);
     scatter(h3, tid + 3 * kIndexThreads);
     for (uint32_t i = tid + kIndexHeld * kIndexThreads; i < n; i += kIndexThreads) {
-        const uint32_t* r = A.rec(l, i);
+        const auto* r = A.rec(l, i);
         float x, y;
         const uint32_t k = atomicAdd(&hist[pos_of(r[0], r[1], &x, &y)], 1u);
-        shead[k] = make_uint4(__float_as_uint(x), __float_as_uint(y), i, 0u);
-        sdlo[k] = *reinterpret_cast<const uint4*>(r + 4);
-        sdhi[k] = *reinterpret_cast<const uint4*>(r + 8);
+        gstore_u4(shead + 4 * k, make_uint4(__float_as_uint(x), __float_as_uint(y), i, 0u));
+        gstore_u4(sdlo + 4 * k, gload_u4(r + 4));
+        gstore_u4(sdhi + 4 * k, gload_u4(r + 8));
     }
 }
 
 namespace {
 
-__device__ inline uint4 load_u4(const uint4* base, uint32_t i) {
+__device__ inline uint4 load_u4(const ORBFE_GLOBAL uint32_t* base, uint32_t i) {
     const u32x4 v = reinterpret_cast<const ORBFE_GLOBAL u32x4*>(reinterpret_cast<uintptr_t>(base))[i];
     return make_uint4(v.x, v.y, v.z, v.w);
 }
@@ -1582,7 +1606,7 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
     ConstPlan P = as_const(M.plan);
     const FeatView Q{kMutual ? M.prev : M.curr, M.feat_stride};
     const FeatView C{kMutual ? M.curr : M.prev, M.feat_stride};
-    const uint32_t* cidx = kMutual ? M.index_curr : M.index_prev;
+    const ORBFE_GLOBAL uint32_t* cidx = gptr(kMutual ? M.index_curr : M.index_prev);
     uint32_t l, slot;  // the curr feature b
     if (!kMutual) {
         const uint32_t groups = (M.cap + QPB - 1) / QPB;  // blocks per level
@@ -1596,16 +1620,16 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
         // Entry t of the segment (t < the segment's capacity for every block
         // of the grid) is loaded together with the segment's count, which
         // the direction pass wrote.
-        const uint32_t e = M.q_list[kListSegs * kListStride +
+        const uint32_t e = gptr(M.q_list)[kListSegs * kListStride +
                                     seg * query_segment_capacity(blocks, QPB) + t];
-        if (t >= M.q_list[seg * kListStride]) return;
+        if (t >= gptr(M.q_list)[seg * kListStride]) return;
         l = e >> 24;
         slot = e & 0xFFFFFFu;
     }
     // Query: feature (ql, qs) of the query side.
     uint32_t ql = l, qs = slot;
     if (kMutual) {
-        const int32_t ab = M.b_to_a[l * M.cap + slot];
+        const int32_t ab = gptr(M.b_to_a)[l * M.cap + slot];
         ql = static_cast<uint32_t>(ab) >> 24;
         qs = static_cast<uint32_t>(ab) & 0xFFFFFFu;
     }
@@ -1617,15 +1641,15 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
     // The query's record (qs < cap: inside its level block), the pool
     // counts and, in the direction pass, the query level's own count are
     // loaded together before any of them is used.
-    const uint32_t* qr = Q.rec(ql, qs);
-    const uint4 qw0 = *reinterpret_cast<const uint4*>(qr);
-    const uint4 qw1 = *reinterpret_cast<const uint4*>(qr + 4);
-    const uint4 qw2 = *reinterpret_cast<const uint4*>(qr + 8);
+    const auto* qr = Q.rec(ql, qs);
+    const uint4 qw0 = gload_u4(qr);
+    const uint4 qw1 = gload_u4(qr + 4);
+    const uint4 qw2 = gload_u4(qr + 8);
     const uint32_t c0 = C.count(ql, M.cap);
     const uint32_t c1 = C.count(ql > 0 ? ql - 1 : ql, M.cap);
     const uint32_t c2 = C.count(ql + 1 < M.levels ? ql + 1 : ql, M.cap);
     if (!kMutual && slot >= FeatView{M.curr, M.feat_stride}.count(l, M.cap)) {
-        if (sub == 0) M.b_to_a[l * M.cap + slot] = -1;
+        if (sub == 0) gptr(M.b_to_a)[l * M.cap + slot] = -1;
         return;
     }
     uint32_t npool = 1, total_pool = c0;
@@ -1662,7 +1686,7 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
             if (k < nranges) {
                 const uint32_t sk = k / rows;
                 const uint32_t lv = pool_level(sk);
-                const uint32_t* st =
+                const ORBFE_GLOBAL uint32_t* st =
                     cidx + lv * (kCells + 1) + (cy0 + (k - sk * rows)) * kCellsX;
                 const uint32_t a = st[cx0];
                 len = st[cx1 + 1] - a;
@@ -1677,9 +1701,9 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
         }
         wave_sync();
 
-        const uint4* chead = reinterpret_cast<const uint4*>(cidx + P.idx_head);
-        const uint4* cdlo = reinterpret_cast<const uint4*>(cidx + P.idx_dlo);
-        const uint4* cdhi = reinterpret_cast<const uint4*>(cidx + P.idx_dhi);
+        const ORBFE_GLOBAL uint32_t* chead = cidx + P.idx_head;
+        const ORBFE_GLOBAL uint32_t* cdlo = cidx + P.idx_dlo;
+        const ORBFE_GLOBAL uint32_t* cdhi = cidx + P.idx_dhi;
         uint32_t min1 = 0xFFFFFFFFu, min2 = 0xFFFFFFFFu;
         uint32_t k = 0;
         uint32_t kend = total ? r_end[grp][0] : 0u, kgb = total ? r_gb[grp][0] : 0u;
@@ -1750,7 +1774,7 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
         }
     }
     if (!kMutual) {
-        if (sub == 0) M.b_to_a[l * M.cap + slot] = result;
+        if (sub == 0) gptr(M.b_to_a)[l * M.cap + slot] = result;
         // append b to the query list: one atomic per wave
         const uint32_t lane = tid & 63;
         const bool claim = sub == 0 && result >= 0;
@@ -1764,11 +1788,11 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
                 base = atomicAdd(M.q_list + seg * kListStride, static_cast<uint32_t>(__popcll(m)));
             base = __builtin_amdgcn_readlane(base, leader);
             if (claim)
-                M.q_list[kListSegs * kListStride + seg * query_segment_capacity(blocks, QPB) +
-                         base + __popcll(m & ((1ull << lane) - 1ull))] = (l << 24) | slot;
+                gptr(M.q_list)[kListSegs * kListStride + seg * query_segment_capacity(blocks, QPB) +
+                               base + __popcll(m & ((1ull << lane) - 1ull))] = (l << 24) | slot;
         }
     } else if (sub == 0 && result == static_cast<int32_t>((l << 24) | slot)) {
-        M.prev_match[ql * M.cap + qs] = result;
+        gptr(M.prev_match)[ql * M.cap + qs] = result;
     }
 }
 
