@@ -213,11 +213,12 @@ void build_level_tiles(const Plan& P, const LevelGeom& G, uint32_t l,
             const uint32_t nrows = rowmax >= rowmin ? rowmax - rowmin + 1 : 0;
             const bool fits = colmax >= colmin && rowmax >= rowmin && nd <= 64 &&
                               nd * nrows <= orbfe::detect_src_words(P.tile_h);
+            const bool fits_small = fits && nd * nrows <= orbfe::detect_src_words_small(P.tile_h);
             orbfe::TileDesc& d = out[ty * G.tiles_x + tx];
             d.lxy = l | (tx << 4) | (ty << 16);
             d.cs = cs;
             d.rowmin = rowmin;
-            d.nd_rows = fits ? (nd | (nrows << 16) | (1u << 31)) : 0u;
+            d.nd_rows = fits ? (nd | (nrows << 16) | (fits_small ? 1u << 30 : 0u) | (1u << 31)) : 0u;
         }
     }
 }

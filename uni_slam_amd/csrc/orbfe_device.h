@@ -146,12 +146,13 @@ __host__ __device__ inline uint32_t xcd_block(uint32_t i, uint32_t n) {
 // kernel would otherwise repeat: level | tile_x << 4 | tile_y << 16, then the
 // source rectangle of level 0 its staged texels sample (first dword column
 // cs, first row rowmin, nd dwords per row, nrows rows) and whether it fits
-// the LDS source buffer (bit 31 of nd_rows).
+// the LDS source buffer of the kernel's two forms (bit 31: the full buffer;
+// bit 30: the small one of single-frame launches).
 struct TileDesc {
     uint32_t lxy;
     uint32_t cs;
     uint32_t rowmin;
-    uint32_t nd_rows;  // nd | nrows << 16 | fits << 31
+    uint32_t nd_rows;  // nd | nrows << 16 | fits_small << 30 | fits << 31
 };
 
 // Dwords of the detect kernel's LDS source buffer for tiles of th rows: at
@@ -160,6 +161,15 @@ struct TileDesc {
 // bilinear neighbour and the dword alignment).
 __host__ __device__ constexpr uint32_t detect_src_words(uint32_t th) {
     return ((th + 6) * 5u / 2u + 4u) * 45u;
+}
+
+// The small source buffer of single-frame launches: level scales up to 1.75
+// (the first four levels of 1.2), (th+6)*1.75 + 4 rows of 33 dwords; tiles
+// of higher levels read level 0 from memory.  A 16-row tile's workgroup then
+// stays under 16 KB of LDS, so 10 of them fit on a CU, and a single frame's
+// tiles fill the device in about one round instead of 1.3.
+__host__ __device__ constexpr uint32_t detect_src_words_small(uint32_t th) {
+    return ((th + 6) * 7u / 4u + 4u) * 33u;
 }
 
 struct Job {

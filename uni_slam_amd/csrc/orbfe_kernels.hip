@@ -208,10 +208,13 @@ __device__ inline float level_texel_direct(const Job& J, ConstPlan P, const Leve
 // of the FAST pass.  At level scales up to 2.5 (six levels of 1.2) the region
 // spans at most (TH+6)*2.5 + 4 rows of 45 dwords (70 texels * 2.5 plus the
 // bilinear neighbour and the dword alignment): 7.0 KB, 10.6 KB, 17.8 KB for
-// TH = 8, 16, 32.  Regions that do not fit (levels past the sixth) fall back
-// to per-texel gathers.
-template <int TH>
-constexpr uint32_t stage_words() { return detect_src_words(TH); }
+// TH = 8, 16, 32.  Regions that do not fit (levels past the sixth; with the
+// small buffer of single-frame launches, levels past the fourth) fall back to
+// per-texel gathers.
+template <int TH, bool kSmallSrc>
+constexpr uint32_t stage_words() {
+    return kSmallSrc ? detect_src_words_small(TH) : detect_src_words(TH);
+}
 
 // Byte i of the staged source region, read as a dword and shifted, so the
 // load stays a 32-bit LDS read.
@@ -263,7 +266,7 @@ struct DetectTile {
     bool use_src;
 };
 
-template <int TH>
+template <int TH, bool kSmallSrc>
 __device__ inline DetectTile decode_tile(const BatchView& bv, const Job& job0, const Params& prm,
                                          uint32_t b) {
     constexpr int LH = TH + 2 * kHalo;
@@ -282,8 +285,8 @@ __device__ inline DetectTile decode_tile(const BatchView& bv, const Job& job0, c
     T.rowmin = td.rowmin;
     T.cs = td.cs;
     T.nd = td.nd_rows & 0xFFFFu;
-    T.nrows = (td.nd_rows >> 16) & 0x7FFFu;
-    T.use_src = (td.nd_rows >> 31) != 0 && (T.J.pitch & 3u) == 0 &&
+    T.nrows = (td.nd_rows >> 16) & 0x3FFFu;
+    T.use_src = ((td.nd_rows >> (kSmallSrc ? 30 : 31)) & 1u) != 0 && (T.J.pitch & 3u) == 0 &&
                 (reinterpret_cast<uintptr_t>(T.J.img) & 3u) == 0;
     ConstPlan P = as_const(T.J.plan);
     T.G = copy_const(&P.lv[l]);  // by value: stores cannot alias it
@@ -328,7 +331,7 @@ __device__ inline void issue_source(const DetectTile& T, uint32_t* src, uint32_t
     if (!T.use_src) return;  // block-uniform
     constexpr uint32_t kWaves = kDetectThreads / 64;
     const uint32_t lane = tid & 63, wave = tid >> 6;
-    const uint32_t total = T.nd * T.nrows;  // <= stage_words<TH>()
+    const uint32_t total = T.nd * T.nrows;  // <= the buffer's words (the tile's fit bit)
     // row = floor(e / nd) through a reciprocal: e < 2^13 and nd <= 45 keep
     // (e + 0.5) / nd at least 1/90 from an integer, far above the relative
     // rounding error (~2^-23) of the product, so the floor is exact.
@@ -397,13 +400,14 @@ __device__ inline uint2 setup_item(const DetectTile& T, const Params& prm, uint3
     return item;
 }
 
-// One workgroup per tile.
-template <int TH>
+// One workgroup per tile.  kSmallSrc: the small LDS source buffer
+// (single-frame launches).
+template <int TH, bool kSmallSrc = false>
 __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job0,
                                                             Params prm) {
     constexpr int LH = TH + 2 * kHalo;                     // staged rows
     constexpr int RPT = TH / (kDetectThreads / 64);        // rows per thread
-    constexpr uint32_t kSrcWords = stage_words<TH>();
+    constexpr uint32_t kSrcWords = stage_words<TH, kSmallSrc>();
     constexpr uint32_t kWaves = kDetectThreads / 64;
     using SI = SetupItems<TH>;
     static_assert(SI::kMask <= kDetectThreads, "one set-up item per thread");
@@ -428,12 +432,13 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63;
-    const uint32_t wave = tid >> 6;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
     const float wts[3] = {0.3125f, 0.375f, 0.3125f};
     for (uint32_t i = tid; i < 256; i += kDetectThreads) lut[i] = static_cast<float>(i) / 255.0f;
 
     const DetectTile T =
-        decode_tile<TH>(bv, job0, prm, bv.n ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x);
+        decode_tile<TH, kSmallSrc>(bv, job0, prm,
+                                   bv.n ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x);
     issue_source<TH>(T, src, tid);
     const uint2 item = setup_item<TH>(T, prm, tid);
     const Job& J = T.J;
@@ -972,7 +977,7 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(BatchView bv, Job j
     __shared__ uint32_t round_total;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63;
-    const uint32_t wave = tid >> 6;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
     const uint32_t per_frame = levels * parts;
     const uint32_t blk = bv.n ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint32_t j = blk / per_frame;
@@ -1595,7 +1600,8 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
     __shared__ uint32_t r_gb[QPB][kMaxRanges];
     const uint32_t tid = threadIdx.x;
     const uint32_t sub = tid & (LANES - 1);
-    const uint32_t grp = tid / LANES;
+    // a query per wave at 64 lanes: its slot arithmetic stays scalar
+    const uint32_t grp = LANES == 64 ? __builtin_amdgcn_readfirstlane(tid / LANES) : tid / LANES;
     uint32_t b = bv.n ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     uint32_t j = 0;
     if (bv.n) {
@@ -1950,7 +1956,7 @@ __global__ __launch_bounds__(kBmThreads) void k_bf_mfma(const uint32_t* __restri
                                                          uint32_t splits,
                                                          uint2* __restrict__ part) {
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
     const uint32_t col = lane & 31;
     const uint32_t half = lane >> 5;
     const uint32_t split = blockIdx.x % splits;
@@ -2176,7 +2182,16 @@ hipError_t launch_detect(uint32_t tile_h, uint32_t blocks, BatchView bv, const J
                          const Params& prm, hipStream_t s) {
     switch (tile_h) {
         case 8: hipLaunchKernelGGL(k_detect<8>, dim3(blocks), dim3(kDetectThreads), 0, s, bv, job0, prm); break;
-        case 16: hipLaunchKernelGGL(k_detect<16>, dim3(blocks), dim3(kDetectThreads), 0, s, bv, job0, prm); break;
+        case 16:
+            // single frames: 10 workgroups per CU (small source buffer);
+            // batches: the full buffer (no tile reads level 0 from memory)
+            if (bv.n == 0)
+                hipLaunchKernelGGL((k_detect<16, true>), dim3(blocks), dim3(kDetectThreads), 0, s,
+                                   bv, job0, prm);
+            else
+                hipLaunchKernelGGL((k_detect<16, false>), dim3(blocks), dim3(kDetectThreads), 0, s,
+                                   bv, job0, prm);
+            break;
         case 32: hipLaunchKernelGGL(k_detect<32>, dim3(blocks), dim3(kDetectThreads), 0, s, bv, job0, prm); break;
         default: return hipErrorInvalidValue;
     }
