@@ -1004,12 +1004,14 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(BatchView bv, Job j
         uint32_t pc = 0;
         constexpr uint32_t kU = 4;  // loads in flight per thread
         for (uint32_t k0 = 0; k0 < w0; k0 += kU * kCompactThreads) {
+            // loads at clamped indices, unconditional, so all kU are in
+            // flight at once (a load under a branch is waited for inside it)
             uint64_t v[kU];
 #pragma unroll
-            for (uint32_t u = 0; u < kU; ++u) {
-                const uint32_t k = k0 + u * kCompactThreads + tid;
-                v[u] = k < w0 ? surv[k] : 0ull;
-            }
+            for (uint32_t u = 0; u < kU; ++u) v[u] = surv[umin(k0 + u * kCompactThreads + tid, w0 - 1)];
+#pragma unroll
+            for (uint32_t u = 0; u < kU; ++u)
+                if (k0 + u * kCompactThreads + tid >= w0) v[u] = 0ull;
 #pragma unroll
             for (uint32_t u = 0; u < kU; ++u) pc += __popcll(v[u]);
         }
@@ -1025,10 +1027,13 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(BatchView bv, Job j
         uint64_t wv[kCompactChunks];
         uint32_t cc[kCompactChunks];
 #pragma unroll
+        for (uint32_t c = 0; c < kCompactChunks; ++c)
+            wv[c] = surv[umin((r0 + wave + kCompactWaves * c) * 64 + lane, words - 1)];
+#pragma unroll
         for (uint32_t c = 0; c < kCompactChunks; ++c) {
             const uint32_t chunk = r0 + wave + kCompactWaves * c;
             const uint32_t k = chunk * 64 + lane;
-            wv[c] = (chunk < c1 && k < words) ? surv[k] : 0ull;
+            if (!(chunk < c1 && k < words)) wv[c] = 0ull;
             cc[c] = __popcll(wv[c]);
         }
 #pragma unroll
@@ -1402,6 +1407,11 @@ __device__ inline uint4 gload_u4(const ORBFE_GLOBAL uint32_t* p) {
     const u32x4 v = *reinterpret_cast<const ORBFE_GLOBAL u32x4*>(p);
     return make_uint4(v.x, v.y, v.z, v.w);
 }
+__device__ inline uint2 gload_u2(const ORBFE_GLOBAL uint32_t* p) {
+    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+    const u32x2 v = *reinterpret_cast<const ORBFE_GLOBAL u32x2*>(p);
+    return make_uint2(v.x, v.y);
+}
 __device__ inline void gstore_u4(ORBFE_GLOBAL uint32_t* p, uint4 v) {
     *reinterpret_cast<ORBFE_GLOBAL u32x4*>(p) = u32x4{v.x, v.y, v.z, v.w};
 }
@@ -1453,19 +1463,21 @@ __global__ __launch_bounds__(kIndexThreads) void k_match_index(MatchBatchView bv
     // Held records as named variables (an array indexed in a loop would be
     // placed in scratch or LDS before the loop is unrolled).
     struct Held {
-        uint4 head, lo, hi;
+        uint2 head;  // the record's x, y
+        uint4 lo, hi;
         uint32_t cell;
         float x, y;
     };
     static_assert(kIndexHeld == 4, "four held records");
+    // Unconditional loads at a clamped slot (slot 0 of an empty level is
+    // still inside the level's record block), so all held records' loads
+    // are in flight together; slots past lim are never used.
     const auto load = [&](uint32_t i, uint32_t lim) -> Held {
         Held h{};
-        if (i < lim) {
-            const auto* r = A.rec(l, i);
-            h.head = gload_u4(r);
-            h.lo = gload_u4(r + 4);
-            h.hi = gload_u4(r + 8);
-        }
+        const auto* r = A.rec(l, umin(i, (lim ? lim : 1u) - 1u));
+        h.head = gload_u2(r);  // only x and y (no dead halves to be reused early)
+        h.lo = gload_u4(r + 4);
+        h.hi = gload_u4(r + 8);
         return h;
     };
     // A single pair (latency-bound) loads the level's slots up to the cap
@@ -1734,23 +1746,25 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
                     sl[u] = kslot;
                 }
             }
+            // Branch-free: every lane loads (record 0 past the runs' end) and
+            // scores its candidates, and one outside the window or past the
+            // end offers the neutral key; with no loads or waits under
+            // branches, the loads of a round stay in flight together.
             uint4 head[kCandPerLane], dlo[kCandPerLane], dhi[kCandPerLane];
 #pragma unroll
             for (uint32_t u = 0; u < kCandPerLane; ++u) {
-                if (ok[u]) {
-                    head[u] = load_u4(chead, g[u]);
-                    dlo[u] = load_u4(cdlo, g[u]);
-                    dhi[u] = load_u4(cdhi, g[u]);
-                }
+                head[u] = load_u4(chead, g[u]);
+                dlo[u] = load_u4(cdlo, g[u]);
+                dhi[u] = load_u4(cdhi, g[u]);
             }
 #pragma unroll
             for (uint32_t u = 0; u < kCandPerLane; ++u) {
-                if (!ok[u]) continue;
                 const float cx = __uint_as_float(head[u].x), cy = __uint_as_float(head[u].y);
-                if (cx < lo_x || cx > hi_x || __builtin_fabsf(qy - cy) > dy) continue;
+                const bool in =
+                    ok[u] && !(cx < lo_x || cx > hi_x || __builtin_fabsf(qy - cy) > dy);
                 const uint32_t d = hamming256(qd, dlo[u], dhi[u]);
                 const uint32_t key = (d << kMatchKeyShift) | (sl[u] << kMatchIdxBits) | head[u].z;
-                keep_two_smallest(min1, min2, key);
+                keep_two_smallest(min1, min2, in ? key : 0xFFFFFFFFu);
             }
         }
         // The two smallest keys of the group (each lane holds the two
