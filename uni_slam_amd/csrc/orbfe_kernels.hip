@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "brief_pattern.h"
 #include "orbfe_device.h"
@@ -1103,6 +1104,22 @@ struct LevelSrc {
         }
         return at(tint_clamp(x, w), tint_clamp(y, h));
     }
+    // load() in two steps, so that several loads can be in flight: raw()
+    // reads the texel at the clamped coordinates (the u8 for level 0, the f32
+    // bits otherwise), value() turns it into load()'s result.  In range both
+    // clamps are the identity; outside, border_zero gives 0.
+    template <bool kBase>
+    __device__ uint32_t raw(int32_t x, int32_t y) const {
+        const uint32_t cx = tint_clamp(x, w), cy = tint_clamp(y, h);
+        if constexpr (kBase) return img[cy * pitch + cx];
+        return __float_as_uint(lvl[cy * w + cx]);
+    }
+    __device__ float value(uint32_t r, int32_t x, int32_t y, uint32_t border_zero) const {
+        const bool out =
+            x < 0 || y < 0 || x >= static_cast<int32_t>(w) || y >= static_cast<int32_t>(h);
+        if (border_zero && out) return 0.0f;
+        return base ? lut[r] : __uint_as_float(r);
+    }
 };
 
 }  // namespace
@@ -1197,12 +1214,41 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
     const int32_t r = prm.orient_r;
     const bool boxed = r <= kBoxR;
     float* bx = box[tid / LANES];
-    if (boxed) {
+    if (boxed && kAllLoads) {  // single frames: one entry at a time measured faster
         for (uint32_t e = sub; e < kBoxN; e += LANES) {
             const int32_t dy = static_cast<int32_t>(e / kBoxSide) - kBoxR;
             const int32_t dx = static_cast<int32_t>(e % kBoxSide) - kBoxR;
             bx[e] = S.load(static_cast<int32_t>(x) + dx, static_cast<int32_t>(y) + dy,
                            prm.border_zero);
+        }
+        wave_sync();
+    } else if (boxed) {
+        // every lane's loads first (entries past the box repeat the last),
+        // then the values
+        constexpr uint32_t kPer = (kBoxN + LANES - 1) / LANES;
+        uint32_t raw[kPer];
+        const auto load_all = [&](auto base) {
+#pragma unroll
+            for (uint32_t k = 0; k < kPer; ++k) {
+                const uint32_t e = umin(sub + k * LANES, kBoxN - 1);
+                const int32_t dy = static_cast<int32_t>(e / kBoxSide) - kBoxR;
+                const int32_t dx = static_cast<int32_t>(e % kBoxSide) - kBoxR;
+                raw[k] = S.template raw<decltype(base)::value>(static_cast<int32_t>(x) + dx,
+                                                               static_cast<int32_t>(y) + dy);
+            }
+        };
+        if (S.base)  // block-uniform: one loop of loads either way
+            load_all(std::true_type{});
+        else
+            load_all(std::false_type{});
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) {
+            const uint32_t e = sub + k * LANES;
+            const int32_t dy = static_cast<int32_t>(e / kBoxSide) - kBoxR;
+            const int32_t dx = static_cast<int32_t>(e % kBoxSide) - kBoxR;
+            if (e < kBoxN)
+                bx[e] = S.value(raw[k], static_cast<int32_t>(x) + dx,
+                                static_cast<int32_t>(y) + dy, prm.border_zero);
         }
         wave_sync();
     }
