@@ -131,6 +131,24 @@ __device__ inline ORBFE_GLOBAL T* gptr(T* p) {
     return reinterpret_cast<ORBFE_GLOBAL T*>(reinterpret_cast<uintptr_t>(p));
 }
 
+// Native vector types for loads and stores through the global address space.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// 16-byte global load / store (native vectors: the HIP vector classes have
+// no operators for objects outside the generic address space).
+__device__ inline uint4 gload_u4(const ORBFE_GLOBAL uint32_t* p) {
+    const u32x4 v = *reinterpret_cast<const ORBFE_GLOBAL u32x4*>(p);
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ inline uint2 gload_u2(const ORBFE_GLOBAL uint32_t* p) {
+    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+    const u32x2 v = *reinterpret_cast<const ORBFE_GLOBAL u32x2*>(p);
+    return make_uint2(v.x, v.y);
+}
+__device__ inline void gstore_u4(ORBFE_GLOBAL uint32_t* p, uint4 v) {
+    *reinterpret_cast<ORBFE_GLOBAL u32x4*>(p) = u32x4{v.x, v.y, v.z, v.w};
+}
+
 // Element i of a global array with a uniform base (32-bit offsets).
 __device__ inline uint32_t load_u32(const uint32_t* base, uint32_t i) {
     return reinterpret_cast<const ORBFE_GLOBAL uint32_t*>(reinterpret_cast<uintptr_t>(base))[i];
@@ -366,6 +384,14 @@ __device__ inline uint2 setup_item(const DetectTile& T, const Params& prm, uint3
     ConstPlan P = as_const(T.J.plan);
     const uint32_t e = tid;
     const uint32_t w = T.G.w, h = T.G.h;
+    // Every item is one 8-byte entry (a tap entry or a cull-mask word) or a
+    // value computed here; the branches below only pick the entry, and one
+    // load per thread follows (entry 0 of the tap table for threads with
+    // nothing to load), so a wave holding several kinds of item makes one
+    // round trip, not one per kind.
+    const ORBFE_GLOBAL uint2* base = reinterpret_cast<const ORBFE_GLOBAL uint2*>(gptr(P.taps));
+    uint32_t idx = 0;
+    bool load = false;
     uint2 item = make_uint2(0u, 0u);
     if (e < SI::kTap) {
         const bool is_col = e < kLdsW;
@@ -376,8 +402,8 @@ __device__ inline uint2 setup_item(const DetectTile& T, const Params& prm, uint3
         const uint32_t cl = prm.border_zero ? (inside ? static_cast<uint32_t>(g) : 0u)
                                             : tint_clamp(g, n);
         if (T.l > 0) {
-            const TapEntry tp = load_tap(P.taps, (is_col ? T.t_lc : T.t_lr) + cl);
-            item = make_uint2(tp.i01, __float_as_uint(tp.frac));
+            idx = (is_col ? T.t_lc : T.t_lr) + cl;
+            load = true;
         } else {
             item = make_uint2(cl | (cl << 16), 0u);
         }
@@ -387,18 +413,20 @@ __device__ inline uint2 setup_item(const DetectTile& T, const Params& prm, uint3
         const uint32_t y = static_cast<uint32_t>(T.y0) + rr;
         const uint32_t sh = h + kSmoothExt;
         if (y < sh) {  // rows past the extended grid are not smoothed
-            const TapEntry tp = load_tap(P.taps, T.t_sr + jj * sh + y);
-            item = make_uint2(tp.i01, __float_as_uint(tp.frac));
+            idx = T.t_sr + jj * sh + y;
+            load = true;
         }
     } else if (e < SI::kMask) {
         const uint32_t rr = e - SI::kSrow;
         const uint32_t y = static_cast<uint32_t>(T.y0) + rr;
         if (T.tile_x < T.G.wpr && y < h) {
-            const uint64_t m = P.mask[T.G.word_off + y * T.G.wpr + T.tile_x];
-            item = make_uint2(static_cast<uint32_t>(m), static_cast<uint32_t>(m >> 32));
+            base = reinterpret_cast<const ORBFE_GLOBAL uint2*>(gptr(P.mask));
+            idx = T.G.word_off + y * T.G.wpr + T.tile_x;
+            load = true;
         }
     }
-    return item;
+    const uint2 v = gload_u2(reinterpret_cast<const ORBFE_GLOBAL uint32_t*>(base + idx));
+    return load ? v : item;
 }
 
 // One workgroup per tile.  kSmallSrc: the small LDS source buffer
@@ -1444,23 +1472,6 @@ constexpr uint32_t kMatchThreads = 256;
 // Candidates a lane loads per round (their loads are issued together; more
 // would cost occupancy).
 constexpr uint32_t kCandPerLane = 2;
-// Native vector types for loads through the global address space.
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-
-// 16-byte global load / store (native vectors: the HIP vector classes have
-// no operators for objects outside the generic address space).
-__device__ inline uint4 gload_u4(const ORBFE_GLOBAL uint32_t* p) {
-    const u32x4 v = *reinterpret_cast<const ORBFE_GLOBAL u32x4*>(p);
-    return make_uint4(v.x, v.y, v.z, v.w);
-}
-__device__ inline uint2 gload_u2(const ORBFE_GLOBAL uint32_t* p) {
-    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
-    const u32x2 v = *reinterpret_cast<const ORBFE_GLOBAL u32x2*>(p);
-    return make_uint2(v.x, v.y);
-}
-__device__ inline void gstore_u4(ORBFE_GLOBAL uint32_t* p, uint4 v) {
-    *reinterpret_cast<ORBFE_GLOBAL u32x4*>(p) = u32x4{v.x, v.y, v.z, v.w};
-}
 
 // LOD0 position of a record (toLod0, match_features_cpu.cpp:58-64).
 __device__ inline void rec_lod0(const uint32_t* r, float scale, float* x, float* y) {
