@@ -604,10 +604,11 @@ def main():
                 "traffic": measured_traffic(dom),
                 "valu": valu_roofline(dom, per_kernel[dom]["avg_us"], peak["wave_instr_per_s"]),
                 "limiter": "priced against HBM (the roofline of its algorithmic bytes); what "
-                           "limits it is instruction issue: f32 bilinear and 9-tap smoothing "
-                           "arithmetic, VALU issue about 0.58 of the probed peak in 32-frame "
-                           "batches and 0.36 on one frame, whose 2709 tiles fill the device "
-                           "1.3 times (profiles/valu_1280x720.md)",
+                           "limits it is instruction issue and per-tile latency: f32 bilinear "
+                           "and 9-tap smoothing arithmetic, VALU issue about 0.55 of the probed "
+                           "peak in 32-frame batches and 0.33 on one frame, whose 2709 tiles "
+                           "fill the device about once at 10 workgroups per CU "
+                           "(profiles/valu_1280x720.md)",
                 "avg_us": per_kernel[dom]["avg_us"], "bytes_per_launch": kb[dom]}
 
     extra = {}
