@@ -52,9 +52,10 @@ def kernel_bytes(dims, cap, counts):
         "k_describe": frame + 4 * pyr + 4 * kept + feat_array,
         # two feature arrays' positions in, two cell indices out
         "k_match_index": 2 * 8 * kept + 2 * 4 * LEVELS * (577 + cap),
-        # two feature arrays in, one int32 per slot out
-        "k_match_dir": 2 * feat_array + 4 * LEVELS * cap,
-        "k_match_mutual": 2 * feat_array + 8 * LEVELS * cap,
+        # two feature arrays in, at most one 8-byte list entry per slot out
+        "k_match_dir": 2 * feat_array + 8 * LEVELS * cap,
+        # two feature arrays and the list entries in, prev_match out
+        "k_match_mutual": 2 * feat_array + 12 * LEVELS * cap,
     }
 
 

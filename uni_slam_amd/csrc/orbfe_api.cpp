@@ -233,7 +233,7 @@ struct PlanEntry {
     // per-frame scratch sizes (bytes), 256-aligned
     size_t surv_bytes = 0, kp_bytes = 0, pyr_bytes = 0, feat_bytes = 0, match_bytes = 0;
     size_t sm_bytes = 0, index_bytes = 0;
-    size_t pair_bytes = 0;  // per pair matcher scratch: b_to_a, then the query list
+    size_t pair_bytes = 0;  // per pair matcher scratch: the query list
 
     PlanEntry() = default;
     PlanEntry(const PlanEntry&) = delete;
@@ -280,7 +280,7 @@ struct orbfe_ctx {
     uint32_t* kp = nullptr;
     float* pyr_scratch = nullptr;
     float* smooth = nullptr;
-    int32_t* b_to_a = nullptr;
+    uint32_t* q_list = nullptr;  // the pipeline matcher's query list
     uint32_t* index[2] = {nullptr, nullptr};
     // host-memory entry points: device staging
     uint8_t* d_img = nullptr;
@@ -527,7 +527,7 @@ int get_plan(orbfe_ctx* ctx, uint32_t W0, uint32_t H0, uint32_t cap, PlanEntry**
     pe->match_bytes = align_up(static_cast<size_t>(L) * cap * 4);
     pe->sm_bytes = align_up(static_cast<size_t>(sm) * 4);
     pe->index_bytes = align_up(static_cast<size_t>(P.idx_words) * 4);
-    pe->pair_bytes = pe->match_bytes + align_up(static_cast<size_t>(orbfe::query_list_words(L, cap)) * 4);
+    pe->pair_bytes = align_up(static_cast<size_t>(orbfe::query_list_words(L, cap)) * 4);
     *out = pe.get();
     ctx->plans.emplace(key, std::move(pe));
     return ORBFE_OK;
@@ -710,8 +710,7 @@ int layout_match(orbfe_ctx* ctx, const orbfe_match_pair* pairs, uint32_t n, Matc
         M.prev = static_cast<const uint8_t*>(p.prev_features);
         M.curr = static_cast<const uint8_t*>(p.curr_features);
         M.prev_match = p.prev_match;
-        M.b_to_a = reinterpret_cast<int32_t*>(arena + off);
-        M.q_list = reinterpret_cast<uint32_t*>(arena + off + pe->match_bytes);
+        M.q_list = reinterpret_cast<uint32_t*>(arena + off);
         M.cap = pe->host.cap;
         M.feat_stride = pe->host.feat_stride;
         M.levels = pe->host.L;
@@ -826,7 +825,7 @@ void orbfe_destroy(orbfe_ctx* ctx) {
     dfree(ctx->kp);
     dfree(ctx->pyr_scratch);
     dfree(ctx->smooth);
-    dfree(ctx->b_to_a);
+    dfree(ctx->q_list);
     dfree(ctx->index[0]);
     dfree(ctx->index[1]);
     dfree(ctx->d_img);
@@ -907,7 +906,7 @@ int orbfe_create(const orbfe_config* cfg, int device, orbfe_ctx** out) {
     c->main = pe;
     if (dalloc(c, &c->surv, pe->surv_bytes) || dalloc(c, &c->kp, pe->kp_bytes) ||
         dalloc(c, &c->pyr_scratch, pe->pyr_bytes) || dalloc(c, &c->smooth, pe->sm_bytes) ||
-        dalloc(c, &c->b_to_a, pe->pair_bytes) || dalloc(c, &c->index[0], pe->index_bytes) ||
+        dalloc(c, &c->q_list, pe->pair_bytes) || dalloc(c, &c->index[0], pe->index_bytes) ||
         dalloc(c, &c->index[1], pe->index_bytes) ||
         dalloc(c, &c->d_img, align_up(static_cast<size_t>(cfg->frame_width) * cfg->frame_height)) ||
         dalloc(c, &c->d_pyr, pe->pyr_bytes) || dalloc(c, &c->d_feat[0], pe->feat_bytes) ||
@@ -1130,9 +1129,7 @@ int orbfe_match_pipeline_async(orbfe_ctx* ctx, const void* prev_features,
     job0.prev = static_cast<const uint8_t*>(prev_features);
     job0.curr = static_cast<const uint8_t*>(curr_features);
     job0.prev_match = prev_match;
-    job0.b_to_a = ctx->b_to_a;
-    job0.q_list = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(ctx->b_to_a) +
-                                              ctx->main->match_bytes);
+    job0.q_list = ctx->q_list;
     job0.cap = P.cap;
     job0.feat_stride = P.feat_stride;
     job0.levels = P.L;

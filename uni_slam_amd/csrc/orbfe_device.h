@@ -47,7 +47,7 @@ constexpr uint32_t kMatchLanesPair = 64;
 constexpr uint32_t kListSegs = 64;
 constexpr uint32_t kListStride = 32;
 __host__ __device__ constexpr uint32_t query_list_words(uint32_t L, uint32_t cap) {
-    return kListSegs * kListStride + L * cap + 64u * L + 64u * kListSegs;
+    return kListSegs * kListStride + 2u * (L * cap + 64u * L + 64u * kListSegs);
 }
 __host__ __device__ inline uint32_t query_segment_capacity(uint32_t blocks, uint32_t qpb) {
     return (blocks + kListSegs - 1) / kListSegs * qpb;
@@ -192,9 +192,9 @@ struct MatchJob {
     const uint8_t* prev;  // feature arrays
     const uint8_t* curr;
     int32_t* prev_match;  // L*cap
-    int32_t* b_to_a;      // scratch L*cap
-    uint32_t* q_list;     // scratch, query_list_words(L, cap): the curr
-                          // features (level << 24 | slot) with b_to_a >= 0
+    uint32_t* q_list;     // scratch, query_list_words(L, cap): per segment a
+                          // count, then entries (b, a): the curr features
+                          // b = level << 24 | slot with a direction match a
     // copies of plan fields, so a matcher block reaches its feature counts
     // and query record without waiting for the plan
     uint32_t cap, feat_stride, levels, pad_;

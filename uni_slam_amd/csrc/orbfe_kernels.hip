@@ -140,8 +140,8 @@ __device__ inline uint4 gload_u4(const ORBFE_GLOBAL uint32_t* p) {
     const u32x4 v = *reinterpret_cast<const ORBFE_GLOBAL u32x4*>(p);
     return make_uint4(v.x, v.y, v.z, v.w);
 }
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 __device__ inline uint2 gload_u2(const ORBFE_GLOBAL uint32_t* p) {
-    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
     const u32x2 v = *reinterpret_cast<const ORBFE_GLOBAL u32x2*>(p);
     return make_uint2(v.x, v.y);
 }
@@ -1642,11 +1642,12 @@ __device__ inline uint4 load_u4(const ORBFE_GLOBAL uint32_t* base, uint32_t i) {
 }  // namespace
 
 // Pipeline matcher, LANES lanes per query (256 / LANES queries per block).
-// kMutual = false: slot = curr feature b; query b, pool from prev -> b_to_a[b];
-//   every b with a match is appended to segment (block % kListSegs) of the
-//   query list (one atomic per wave on that segment's counter).
+// kMutual = false: slot = curr feature b; query b, pool from prev: its match
+//   a; every b with a match is appended as the entry (b, a) to segment
+//   (block % kListSegs) of the query list (one atomic per wave on that
+//   segment's counter).
 // kMutual = true:  block b takes entries (b / kListSegs) * QPB + group of
-//   segment b % kListSegs; an entry names b, a = b_to_a[b]; query a,
+//   segment b % kListSegs; an entry names b and its match a; query a,
 //   pool from curr; prev_match[a] = b when a's best match is b.  This is the
 //   reference's mutual check (match_features_cpu.cpp:196-205: a matches
 //   iff its best match b has b_to_a[b] == a) evaluated only for the prev
@@ -1683,6 +1684,7 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
     const FeatView C{kMutual ? M.curr : M.prev, M.feat_stride};
     const ORBFE_GLOBAL uint32_t* cidx = gptr(kMutual ? M.index_curr : M.index_prev);
     uint32_t l, slot;  // the curr feature b
+    uint32_t ab_entry = 0;  // mutual: b's direction match a, from its list entry
     if (!kMutual) {
         const uint32_t groups = (M.cap + QPB - 1) / QPB;  // blocks per level
         l = b / groups;
@@ -1695,18 +1697,18 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
         // Entry t of the segment (t < the segment's capacity for every block
         // of the grid) is loaded together with the segment's count, which
         // the direction pass wrote.
-        const uint32_t e = gptr(M.q_list)[kListSegs * kListStride +
-                                    seg * query_segment_capacity(blocks, QPB) + t];
+        const uint2 e = gload_u2(gptr(M.q_list) + kListSegs * kListStride +
+                                 2 * (seg * query_segment_capacity(blocks, QPB) + t));
         if (t >= gptr(M.q_list)[seg * kListStride]) return;
-        l = e >> 24;
-        slot = e & 0xFFFFFFu;
+        l = e.x >> 24;
+        slot = e.x & 0xFFFFFFu;
+        ab_entry = e.y;
     }
     // Query: feature (ql, qs) of the query side.
     uint32_t ql = l, qs = slot;
     if (kMutual) {
-        const int32_t ab = gptr(M.b_to_a)[l * M.cap + slot];
-        ql = static_cast<uint32_t>(ab) >> 24;
-        qs = static_cast<uint32_t>(ab) & 0xFFFFFFu;
+        ql = ab_entry >> 24;
+        qs = ab_entry & 0xFFFFFFu;
     }
 
     // Pool levels in the reference's concatenation order: ql, ql-1, ql+1.
@@ -1723,10 +1725,7 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
     const uint32_t c0 = C.count(ql, M.cap);
     const uint32_t c1 = C.count(ql > 0 ? ql - 1 : ql, M.cap);
     const uint32_t c2 = C.count(ql + 1 < M.levels ? ql + 1 : ql, M.cap);
-    if (!kMutual && slot >= FeatView{M.curr, M.feat_stride}.count(l, M.cap)) {
-        if (sub == 0) gptr(M.b_to_a)[l * M.cap + slot] = -1;
-        return;
-    }
+    if (!kMutual && slot >= FeatView{M.curr, M.feat_stride}.count(l, M.cap)) return;
     uint32_t npool = 1, total_pool = c0;
     if (ql > 0) { total_pool += c1; ++npool; }
     if (ql + 1 < M.levels) { total_pool += c2; ++npool; }
@@ -1851,7 +1850,6 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
         }
     }
     if (!kMutual) {
-        if (sub == 0) gptr(M.b_to_a)[l * M.cap + slot] = result;
         // append b to the query list: one atomic per wave
         const uint32_t lane = tid & 63;
         const bool claim = sub == 0 && result >= 0;
@@ -1864,9 +1862,14 @@ __global__ __launch_bounds__(kMatchThreads) void k_match_dir(MatchBatchView bv, 
             if (lane == leader)
                 base = atomicAdd(M.q_list + seg * kListStride, static_cast<uint32_t>(__popcll(m)));
             base = __builtin_amdgcn_readlane(base, leader);
-            if (claim)
-                gptr(M.q_list)[kListSegs * kListStride + seg * query_segment_capacity(blocks, QPB) +
-                               base + __popcll(m & ((1ull << lane) - 1ull))] = (l << 24) | slot;
+            if (claim) {  // the entry: b and its match a
+                ORBFE_GLOBAL uint32_t* ent =
+                    gptr(M.q_list) + kListSegs * kListStride +
+                    2 * (seg * query_segment_capacity(blocks, QPB) + base +
+                         __popcll(m & ((1ull << lane) - 1ull)));
+                *reinterpret_cast<ORBFE_GLOBAL u32x2*>(ent) =
+                    u32x2{(l << 24) | slot, static_cast<uint32_t>(result)};
+            }
         }
     } else if (sub == 0 && result == static_cast<int32_t>((l << 24) | slot)) {
         gptr(M.prev_match)[ql * M.cap + qs] = result;
