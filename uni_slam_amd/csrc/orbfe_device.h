@@ -166,8 +166,9 @@ __host__ __device__ constexpr uint32_t detect_src_words(uint32_t th) {
 // The small source buffer of single-frame launches: level scales up to 1.75
 // (the first four levels of 1.2), (th+6)*1.75 + 4 rows of 33 dwords; tiles
 // of higher levels read level 0 from memory.  A 16-row tile's workgroup then
-// stays under 16 KB of LDS, so 10 of them fit on a CU, and a single frame's
-// tiles fill the device in about one round instead of 1.3.
+// takes 14.9 KB of LDS instead of 20 KB.  Occupancy is the same (the wave
+// slots, 8 per SIMD, cap both forms at 8 workgroups per CU), but a single
+// 1280x720 frame's detect runs about 1.2 us faster with it.
 __host__ __device__ constexpr uint32_t detect_src_words_small(uint32_t th) {
     return ((th + 6) * 7u / 4u + 4u) * 33u;
 }

@@ -2257,7 +2257,7 @@ hipError_t launch_detect(uint32_t tile_h, uint32_t blocks, BatchView bv, const J
     switch (tile_h) {
         case 8: hipLaunchKernelGGL(k_detect<8>, dim3(blocks), dim3(kDetectThreads), 0, s, bv, job0, prm); break;
         case 16:
-            // single frames: 10 workgroups per CU (small source buffer);
+            // single frames: the small source buffer (measured faster there);
             // batches: the full buffer (no tile reads level 0 from memory)
             if (bv.n == 0)
                 hipLaunchKernelGGL((k_detect<16, true>), dim3(blocks), dim3(kDetectThreads), 0, s,
