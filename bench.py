@@ -536,6 +536,13 @@ def main():
 
     if args.workload != "plain":
         if args.workload == "batched":
+            # the plain path's results, which the batched form must reproduce
+            for t in range(N_FRAMES):
+                ctx.detect_async(d_frames[t], W, feats[t], stream=stream)
+                if t > 0:
+                    ctx.match_pipeline_async(feats[t - 1], feats[t], matches[t - 1],
+                                             stream=stream)
+            torch.cuda.synchronize()
             out, _keep = phase_batched_sequence(torch, orbfe, ctx, d_frames, feats, matches,
                                                 max(2, args.steps))
         elif args.workload == "mixed":

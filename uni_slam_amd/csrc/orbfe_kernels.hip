@@ -29,17 +29,37 @@
 
 namespace orbfe {
 
-// BRIEF test pattern (brief_pattern.h), uploaded once per device.
-__constant__ int8_t c_pattern[256][4];
-
-// Gather order of the describe kernel's sample points, per lanes-per-keypoint
-// variant (8, 16, 32), angle bin and half: for lane s of a keypoint group,
-// its PPL = 256 / LANES point ids (u8) in load order.  Built on the host
-// (build_desc_order) and uploaded once per device.
+// The describe kernel evaluates the 256 BRIEF tests (brief_pattern.h) in two
+// groups of 128.  The 512 test points name only 375 distinct pattern
+// positions, and a test's two points always rotate together, so each group
+// gathers every distinct position of its tests once (at most kDescPts, ids
+// past a group's count repeat a position no test reads).  The groups are
+// chosen on the host (build_desc_tables) so that every aligned run of 8
+// tests puts 4 in each (each lane of a keypoint group evaluates TPL / 2
+// tests of either group) and the larger group has as few positions as a
+// local search finds: 189 and 189, against 220 and 221 for the tests of
+// each parity, so a keypoint gathers 378 values instead of 512.
+//   c_desc_pts[g][p]     pattern position (x, y) of point id p of group g;
+//   c_desc_tests[g][m]   the group's tests in increasing order:
+//                        point id of A | point id of B << 8 | test << 16;
+//   g_desc_order[v][..]  per lanes-per-keypoint variant v (8, 16, 32), angle
+//                        bin, group and lane: its PPL = kDescPts / LANES
+//                        point ids (u8, in rows of desc_order_row bytes) in
+//                        load order.
+constexpr uint32_t kDescPts = 192;
 constexpr uint32_t kAngleBins = 64;
 constexpr float kAngleBinScale = 10.185916357881302f;  // kAngleBins / (2 pi)
-constexpr size_t kDescOrderBytes = kAngleBins * 2 * 256;
-__device__ uint8_t g_desc_order[3][kAngleBins * 2 * 256];
+
+// Bytes per lane row of the gather order (PPL rounded up to whole u32).
+template <uint32_t LANES>
+constexpr uint32_t desc_order_row() {
+    return ((kDescPts / LANES) + 3u) & ~3u;
+}
+
+constexpr size_t kDescOrderBytes = kAngleBins * 2 * 32 * desc_order_row<32>();  // the largest
+__constant__ int8_t c_desc_pts[2][kDescPts][2];
+__constant__ uint32_t c_desc_tests[2][128];
+__device__ uint8_t g_desc_order[3][kDescOrderBytes];
 
 template <uint32_t LANES>
 __device__ inline const uint8_t* desc_order() {
@@ -1172,13 +1192,14 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
     // orientation radii up to 3, the moment disc.
     constexpr int32_t kBoxR = 3;
     constexpr uint32_t kBoxSide = 2 * kBoxR + 1, kBoxN = kBoxSide * kBoxSide;
-    // Sample points of one half of the tests (the tests of one parity): point
-    // q is A (q even) or B (q odd) of test 2 * (q >> 1) + half.
-    constexpr uint32_t kHalfPts = 256;
-    constexpr uint32_t PPL = kHalfPts / LANES;         // points per lane and half
-    constexpr uint32_t kSlots = kHalfPts + kHalfPts / 16;  // point q at q + q / 16
+    // Sample points of one test group (c_desc_pts): point id q of group
+    // `half` is the pattern position c_desc_pts[half][q].
+    constexpr uint32_t PPL = kDescPts / LANES;          // points per lane and group
+    constexpr uint32_t ROW = desc_order_row<LANES>();   // gather-order bytes per lane
+    constexpr uint32_t kSlots = kDescPts + kDescPts / 16;  // point q at q + q / 16
+    static_assert(PPL <= ROW && TPL % 2 == 0, "gather rows hold PPL ids; TPL / 2 tests per group");
     __shared__ float lut[256];
-    __shared__ float2 ppt[2][kHalfPts];  // BRIEF pattern point of each half and q
+    __shared__ float2 ppt[2][kDescPts];  // pattern position of each group's point ids
     __shared__ float box[KPB][kBoxN];    // per keypoint: level texels around it
     __shared__ float samp[KPB][kSlots];  // per keypoint: smoothed value of each point
     const uint32_t tid = threadIdx.x;
@@ -1204,11 +1225,16 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
     const uint32_t kp_entry = in_cap ? gptr(J.kp)[l * J.cap + slot] : 0u;
     ConstPlan P = as_const(J.plan);
     const LevelGeom G = copy_const(&P.lv[l]);  // by value: stores below cannot alias it
+    // This lane's TPL / 2 tests of each group (tests TPL * sub .. + TPL - 1).
+    uint32_t tst[2][TPL / 2];
+#pragma unroll
+    for (uint32_t half = 0; half < 2; ++half)
+#pragma unroll
+        for (uint32_t m = 0; m < TPL / 2; ++m) tst[half][m] = c_desc_tests[half][TPL / 2 * sub + m];
     lut[tid] = static_cast<float>(tid) / 255.0f;
-    {
-        const uint32_t q = tid, i = 2 * (q >> 1), c = (q & 1u) * 2;
-        ppt[0][q] = make_float2(c_pattern[i][c], c_pattern[i][c + 1]);
-        ppt[1][q] = make_float2(c_pattern[i + 1][c], c_pattern[i + 1][c + 1]);
+    if (tid < kDescPts) {
+        ppt[0][tid] = make_float2(c_desc_pts[0][tid][0], c_desc_pts[0][tid][1]);
+        ppt[1][tid] = make_float2(c_desc_pts[1][tid][0], c_desc_pts[1][tid][1]);
     }
     __syncthreads();
 
@@ -1326,9 +1352,11 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
     // Descriptor (generate_features.wgsl:101-144): G at the wrapped u32
     // coordinates (:116-129) is looked up in the smoothed image, where
     // coordinates past the extension (including wrapped negatives) share the
-    // value at w+3 / h+3.  The 512 sample points are gathered in two halves
-    // (the tests of each parity) into LDS, then lane s evaluates tests
-    // TPL*s .. TPL*s + TPL - 1 from there.  The gather order comes from a
+    // value at w+3 / h+3.  The tests run in two groups: each group's
+    // distinct sample points (c_desc_pts, at most kDescPts) are gathered into
+    // LDS, then every lane evaluates its tests of the group from there.  Two
+    // tests that name the same pattern position read the same rotated point,
+    // so gathering it once changes no value.  The gather order comes from a
     // table per angle bin that sorts the points by their position rotated to
     // the bin's centre, so the lanes of one load touch few rows; it only
     // orders the loads, every value is the one at the point's own exact
@@ -1351,11 +1379,11 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
     const uint32_t bin = umin(static_cast<uint32_t>((theta + fm::kPi) * kAngleBinScale),
                               kAngleBins - 1);
     const uint32_t* order = reinterpret_cast<const uint32_t*>(
-        desc_order<LANES>() + (static_cast<size_t>(bin) * 2 * LANES + sub) * PPL);
+        desc_order<LANES>() + (static_cast<size_t>(bin) * 2 * LANES + sub) * ROW);
     float* ks = samp[tid / LANES];
     uint32_t bits = 0;
-    // Byte offset in the smoothed image of BRIEF point q of a half, rotated
-    // by the keypoint's angle (offsets < 4 * sw * sh: 24-bit factors).
+    // Byte offset in the smoothed image of point id q of a group, rotated by
+    // the keypoint's angle (offsets < 4 * sw * sh: 24-bit factors).
     const auto point_offset = [&](uint32_t half, uint32_t q) -> uint32_t {
         const float2 pp = ppt[half][q];
         const uint32_t ax4 =
@@ -1363,30 +1391,35 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
         const uint32_t ay = umin(__float_as_uint((pp.x * sn + pp.y * cs) + kRound) + yb, sh - 1);
         return __umul24(ay, sw4) + ax4;
     };
-    // The tests of a half, once its samples are in ks.
-    const auto half_tests = [&](uint32_t half) {
+    // The lane's tests of a group, once the group's samples are in ks: test
+    // t sets bit t - TPL * sub of the lane's bits.
+    const auto group_tests = [&](uint32_t half) {
         wave_sync();
 #pragma unroll
-        for (uint32_t k = half; k < TPL; k += 2) {
-            const uint32_t qa = 2 * ((TPL * sub + k) >> 1);
-            const float sa = ks[qa + (qa >> 4)], sb = ks[qa + 1 + ((qa + 1) >> 4)];
-            bits |= (sa > sb ? 1u : 0u) << k;
+        for (uint32_t m = 0; m < TPL / 2; ++m) {
+            const uint32_t e = tst[half][m];
+            const uint32_t qa = e & 0xFFu, qb = (e >> 8) & 0xFFu;
+            const float sa = ks[qa + (qa >> 4)], sb = ks[qb + (qb >> 4)];
+            bits |= (sa > sb ? 1u : 0u) << ((e >> 16) & (TPL - 1));
         }
         wave_sync();
     };
+    const auto point_id = [&](const uint32_t* ids, uint32_t k) -> uint32_t {
+        return (ids[k / 4] >> (8 * (k % 4))) & 0xFFu;
+    };
     if constexpr (kAllLoads) {
-        // Single-frame launches (latency-bound): both halves' loads in flight
-        // at once, then each half through LDS.
+        // Single-frame launches (latency-bound): both groups' loads in flight
+        // at once, then each group through LDS.
         float v[2][PPL];
         uint32_t qs[2][PPL];
 #pragma unroll
         for (uint32_t half = 0; half < 2; ++half) {
-            uint32_t ids[PPL / 4];
+            uint32_t ids[ROW / 4];
 #pragma unroll
-            for (uint32_t k = 0; k < PPL / 4; ++k) ids[k] = order[half * LANES * PPL / 4 + k];
+            for (uint32_t k = 0; k < ROW / 4; ++k) ids[k] = order[half * LANES * ROW / 4 + k];
 #pragma unroll
             for (uint32_t k = 0; k < PPL; ++k) {
-                qs[half][k] = (ids[k / 4] >> (8 * (k % 4))) & 0xFFu;
+                qs[half][k] = point_id(ids, k);
                 v[half][k] = load_f32(sm, point_offset(half, qs[half][k]));
             }
         }
@@ -1394,29 +1427,30 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
         for (uint32_t half = 0; half < 2; ++half) {
 #pragma unroll
             for (uint32_t k = 0; k < PPL; ++k) ks[qs[half][k] + (qs[half][k] >> 4)] = v[half][k];
-            half_tests(half);
+            group_tests(half);
         }
     } else {
 #pragma unroll
         for (uint32_t half = 0; half < 2; ++half) {
-            uint32_t ids[PPL / 4];
+            uint32_t ids[ROW / 4];
 #pragma unroll
-            for (uint32_t k = 0; k < PPL / 4; ++k) ids[k] = order[half * LANES * PPL / 4 + k];
-            constexpr uint32_t kB = PPL < 8 ? PPL : 8;  // loads in flight per batch
+            for (uint32_t k = 0; k < ROW / 4; ++k) ids[k] = order[half * LANES * ROW / 4 + k];
+            // loads in flight per batch (a divisor of PPL)
+            constexpr uint32_t kB = PPL <= 12 ? PPL : (PPL % 8 == 0 ? 8 : PPL / 2);
+            static_assert(PPL % kB == 0, "whole batches");
 #pragma unroll
             for (uint32_t k0 = 0; k0 < PPL; k0 += kB) {
                 float v[kB];
                 uint32_t qs[kB];
 #pragma unroll
                 for (uint32_t u = 0; u < kB; ++u) {
-                    const uint32_t k = k0 + u;
-                    qs[u] = (ids[k / 4] >> (8 * (k % 4))) & 0xFFu;
+                    qs[u] = point_id(ids, k0 + u);
                     v[u] = load_f32(sm, point_offset(half, qs[u]));
                 }
 #pragma unroll
                 for (uint32_t u = 0; u < kB; ++u) ks[qs[u] + (qs[u] >> 4)] = v[u];
             }
-            half_tests(half);
+            group_tests(half);
         }
     }
     // Record word e: x, y, strength, angle, then descriptor word e - 4 =
@@ -2209,47 +2243,146 @@ __global__ __launch_bounds__(kRansacThreads) void k_ransac_score(
 // ---------------------------------------------------------------------------
 // Host-side launchers (called from orbfe_api.cpp).
 // ---------------------------------------------------------------------------
-// Gather order of the describe kernel (g_desc_order): for each angle bin,
-// the 256 points of each half sorted by (row, column) of their position
-// rotated to the bin's centre angle; the lanes of a group take consecutive
-// points of that order in each load.  The order only affects which lane
-// loads which point, never a value.
-static void build_desc_order(uint8_t* out, uint32_t lanes) {
-    const uint32_t ppl = 256 / lanes;
-    for (uint32_t bin = 0; bin < kAngleBins; ++bin) {
-        const double th = -3.14159265358979323846 + (bin + 0.5) * (2.0 * 3.14159265358979323846 / kAngleBins);
-        const double c = std::cos(th), sn = std::sin(th);
-        for (uint32_t half = 0; half < 2; ++half) {
-            uint32_t seq[256];
-            long key[256];
-            for (uint32_t q = 0; q < 256; ++q) {
-                const uint32_t i = 2 * (q >> 1) + half, o = (q & 1u) * 2;
-                const double px = kOrbfeBriefPattern[i][o], py = kOrbfeBriefPattern[i][o + 1];
-                const long rx = std::lround(px * c - py * sn), ry = std::lround(px * sn + py * c);
-                key[q] = (ry + 64) * 256 + (rx + 64);
-                seq[q] = q;
-            }
-            std::stable_sort(seq, seq + 256, [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
-            uint8_t* dst = out + (static_cast<size_t>(bin) * 2 + half) * 256;
-            for (uint32_t s = 0; s < lanes; ++s)
-                for (uint32_t k = 0; k < ppl; ++k) dst[s * ppl + k] = static_cast<uint8_t>(seq[lanes * k + s]);
+// Tables of the describe kernel (c_desc_pts, c_desc_tests, g_desc_order).
+struct DescTables {
+    int8_t pts[2][kDescPts][2];
+    uint32_t tests[2][128];
+    uint8_t order[3][kDescOrderBytes];
+};
+
+// Split of the 256 tests into two groups of 128, 4 of every aligned run of 8
+// in each, with few distinct pattern positions per group: starting from the
+// tests of each parity, swap a random test of group 0 with one of group 1 in
+// the same run of 8 whenever that does not raise (larger group's positions,
+// total positions); a fixed xorshift seed makes the split the same on every
+// build.  group[t] is the group of test t; returns each group's position
+// count.
+static void split_tests(uint8_t group[256], uint32_t npos[2]) {
+    // pattern position (x, y in [-13, 13]) -> key in [0, 27 * 27)
+    const auto key = [](int x, int y) { return static_cast<uint32_t>((y + 13) * 27 + x + 13); };
+    uint32_t pa[256], pb[256];
+    for (uint32_t t = 0; t < 256; ++t) {
+        pa[t] = key(kOrbfeBriefPattern[t][0], kOrbfeBriefPattern[t][1]);
+        pb[t] = key(kOrbfeBriefPattern[t][2], kOrbfeBriefPattern[t][3]);
+    }
+    static uint16_t cnt[2][27 * 27];
+    std::fill(&cnt[0][0], &cnt[0][0] + 2 * 27 * 27, 0);
+    npos[0] = npos[1] = 0;
+    const auto add = [&](uint32_t g, uint32_t k, int d) {
+        if (d > 0 && cnt[g][k]++ == 0) ++npos[g];
+        if (d < 0 && --cnt[g][k] == 0) --npos[g];
+    };
+    const auto put = [&](uint32_t t, uint32_t g, int d) {
+        add(g, pa[t], d);
+        add(g, pb[t], d);
+    };
+    for (uint32_t t = 0; t < 256; ++t) {
+        group[t] = static_cast<uint8_t>(t & 1u);
+        put(t, group[t], 1);
+    }
+    const auto cost = [&]() { return std::max(npos[0], npos[1]) * 1024u + npos[0] + npos[1]; };
+    uint64_t st = 0x9E3779B97F4A7C15ull;
+    const auto rnd = [&]() {
+        st ^= st << 13;
+        st ^= st >> 7;
+        st ^= st << 17;
+        return st;
+    };
+    for (uint32_t it = 0; it < 300000; ++it) {
+        const uint32_t run = static_cast<uint32_t>(rnd() % 32) * 8;
+        uint32_t in0[4], in1[4], n0 = 0, n1 = 0;
+        for (uint32_t t = run; t < run + 8; ++t) (group[t] ? in1[n1++] : in0[n0++]) = t;
+        const uint32_t a = in0[rnd() % 4], b = in1[rnd() % 4];
+        const uint32_t before = cost();
+        put(a, 0, -1); put(a, 1, 1); put(b, 1, -1); put(b, 0, 1);
+        if (cost() > before) {
+            put(a, 1, -1); put(a, 0, 1); put(b, 0, -1); put(b, 1, 1);
+        } else {
+            group[a] = 1;
+            group[b] = 0;
         }
     }
 }
 
-hipError_t upload_pattern() {
-    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_pattern), kOrbfeBriefPattern,
-                                     sizeof(kOrbfeBriefPattern));
-    if (e != hipSuccess) return e;
-    static uint8_t order[3][kDescOrderBytes];
-    static bool built = false;
-    if (!built) {
-        build_desc_order(order[0], 8);
-        build_desc_order(order[1], 16);
-        build_desc_order(order[2], 32);
-        built = true;
+// Gather order of one lanes-per-keypoint variant: for each angle bin and
+// group, the group's kDescPts point ids sorted by (row, column) of their
+// position rotated to the bin's centre angle; lane s takes ids s, s + LANES,
+// ... of that order (one load of the group's lanes: consecutive ids).  The
+// order only decides which lane loads which point, never a value.
+static void build_desc_order(const int8_t (*pts)[kDescPts][2], uint8_t* out, uint32_t lanes,
+                             uint32_t row) {
+    const uint32_t ppl = kDescPts / lanes;
+    for (uint32_t bin = 0; bin < kAngleBins; ++bin) {
+        const double th = -3.14159265358979323846 + (bin + 0.5) * (2.0 * 3.14159265358979323846 / kAngleBins);
+        const double c = std::cos(th), sn = std::sin(th);
+        for (uint32_t g = 0; g < 2; ++g) {
+            uint32_t seq[kDescPts];
+            long key[kDescPts];
+            for (uint32_t q = 0; q < kDescPts; ++q) {
+                const double px = pts[g][q][0], py = pts[g][q][1];
+                const long rx = std::lround(px * c - py * sn), ry = std::lround(px * sn + py * c);
+                key[q] = (ry + 64) * 256 + (rx + 64);
+                seq[q] = q;
+            }
+            std::stable_sort(seq, seq + kDescPts, [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+            uint8_t* dst = out + (static_cast<size_t>(bin) * 2 + g) * lanes * row;
+            for (uint32_t s = 0; s < lanes; ++s)
+                for (uint32_t k = 0; k < row; ++k)
+                    dst[s * row + k] = k < ppl ? static_cast<uint8_t>(seq[lanes * k + s]) : 0;
+        }
     }
-    return hipMemcpyToSymbol(HIP_SYMBOL(g_desc_order), order, sizeof(order));
+}
+
+// Returns false if a group needs more than kDescPts points (the pattern in
+// brief_pattern.h needs 189 each).
+static bool build_desc_tables(DescTables* d) {
+    uint8_t group[256];
+    uint32_t npos[2];
+    split_tests(group, npos);
+    if (npos[0] > kDescPts || npos[1] > kDescPts) return false;
+    for (uint32_t g = 0; g < 2; ++g) {
+        int id_of[27 * 27];
+        std::fill(id_of, id_of + 27 * 27, -1);
+        uint32_t n = 0, m = 0;
+        const auto id = [&](int x, int y) -> uint32_t {
+            int& e = id_of[(y + 13) * 27 + x + 13];
+            if (e < 0) {
+                e = static_cast<int>(n);
+                d->pts[g][n][0] = static_cast<int8_t>(x);
+                d->pts[g][n][1] = static_cast<int8_t>(y);
+                ++n;
+            }
+            return static_cast<uint32_t>(e);
+        };
+        for (uint32_t t = 0; t < 256; ++t) {
+            if (group[t] != g) continue;
+            const int8_t* pt = kOrbfeBriefPattern[t];
+            const uint32_t qa = id(pt[0], pt[1]), qb = id(pt[2], pt[3]);
+            d->tests[g][m++] = qa | qb << 8 | t << 16;
+        }
+        if (m != 128) return false;
+        for (uint32_t q = n; q < kDescPts; ++q) {  // unread slots: a valid position
+            d->pts[g][q][0] = d->pts[g][0][0];
+            d->pts[g][q][1] = d->pts[g][0][1];
+        }
+    }
+    build_desc_order(d->pts, d->order[0], 8, desc_order_row<8>());
+    build_desc_order(d->pts, d->order[1], 16, desc_order_row<16>());
+    build_desc_order(d->pts, d->order[2], 32, desc_order_row<32>());
+    return true;
+}
+
+hipError_t upload_pattern() {
+    static DescTables tables;
+    static int built = 0;  // 1 built, -1 failed
+    if (!built) built = build_desc_tables(&tables) ? 1 : -1;
+    if (built < 0) return hipErrorInvalidValue;
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_desc_pts), tables.pts, sizeof(tables.pts));
+    if (e == hipSuccess)
+        e = hipMemcpyToSymbol(HIP_SYMBOL(c_desc_tests), tables.tests, sizeof(tables.tests));
+    if (e == hipSuccess)
+        e = hipMemcpyToSymbol(HIP_SYMBOL(g_desc_order), tables.order, sizeof(tables.order));
+    return e;
 }
 
 hipError_t launch_detect(uint32_t tile_h, uint32_t blocks, BatchView bv, const Job& job0,
