@@ -46,6 +46,10 @@ namespace orbfe {
 //                        bin, group and lane: its PPL = kDescPts / LANES
 //                        point ids (u8, in rows of desc_order_row bytes) in
 //                        load order.
+// f32(v) / 255 for every byte v (provider_base.cpp:121-124), correctly
+// rounded on the host; workgroups copy it into their LDS lookup tables.
+__constant__ float c_u8_unit[256];
+
 constexpr uint32_t kDescPts = 192;
 constexpr uint32_t kAngleBins = 64;
 constexpr float kAngleBinScale = 10.185916357881302f;  // kAngleBins / (2 pi)
@@ -140,6 +144,13 @@ __device__ inline float load_f32(const uint8_t* base, uint32_t off) {
     const ORBFE_GLOBAL uint8_t* g =
         reinterpret_cast<const ORBFE_GLOBAL uint8_t*>(reinterpret_cast<uintptr_t>(base));
     return *reinterpret_cast<const ORBFE_GLOBAL float*>(g + off);
+}
+
+// Stores an f32 at byte offset `off` of a global buffer whose base is
+// uniform (SGPR base, one VGPR offset; no 64-bit address per lane).
+__device__ inline void store_f32(float* base, uint32_t off, float v) {
+    ORBFE_GLOBAL uint8_t* g = reinterpret_cast<ORBFE_GLOBAL uint8_t*>(reinterpret_cast<uintptr_t>(base));
+    *reinterpret_cast<ORBFE_GLOBAL float*>(g + off) = v;
 }
 
 // The global-address-space view of a device pointer.  Accesses through it
@@ -483,7 +494,7 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     const uint32_t lane = tid & 63;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
     const float wts[3] = {0.3125f, 0.375f, 0.3125f};
-    for (uint32_t i = tid; i < 256; i += kDetectThreads) lut[i] = static_cast<float>(i) / 255.0f;
+    for (uint32_t i = tid; i < 256; i += kDetectThreads) lut[i] = c_u8_unit[i];
 
     const DetectTile T =
         decode_tile<TH, kSmallSrc>(bv, job0, prm,
@@ -740,6 +751,9 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
                     const uint32_t y = static_cast<uint32_t>(y0) + rr;
                     if (y >= sh) continue;
                     const bool b1 = (row_b1 >> k) & 1u;
+                    // The shader's sum starts at 0; every term is >= +0 (a
+                    // lerp of non-negative texels times positive weights,
+                    // never -0), so 0 + t is t and the first term starts it.
                     float sum = 0.0f;
 #pragma unroll
                     for (int jj = 0; jj < 3; ++jj) {
@@ -767,10 +781,11 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
                                 bot = hv[i][k + 4];
                             }
                             const float sv = top + bb * (bot - top);
-                            sum = sum + sv * wts[i] * wts[jj];
+                            const float t = sv * wts[i] * wts[jj];
+                            sum = jj == 0 && i == 0 ? t : sum + t;  // 0 + t == t
                         }
                     }
-                    gptr(J.smooth)[G.sm_off + y * sw + x] = sum;
+                    store_f32(J.smooth, 4 * (G.sm_off + y * sw + x), sum);
                 }
             } else {
                 const float frs[3] = {fr0, fr1, fr2};
@@ -796,17 +811,19 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
                             const float t1 = tile[rb][c0];
                             const float bot = t1 + frs[i] * (tile[rb][c1] - t1);
                             const float sv = top + bb * (bot - top);
-                            sum = sum + sv * wts[i] * wts[jj];
+                            const float t = sv * wts[i] * wts[jj];
+                            sum = jj == 0 && i == 0 ? t : sum + t;  // 0 + t == t, as above
                         }
                     }
-                    gptr(J.smooth)[G.sm_off + y * sw + x] = sum;
+                    store_f32(J.smooth, 4 * (G.sm_off + y * sw + x), sum);
                 }
             }
         }
     } else if (x < sw) {
         for (int k = 0; k < RPT; ++k) {
             const uint32_t y = static_cast<uint32_t>(y0) + wave * RPT + k;
-            if (y < sh) gptr(J.smooth)[G.sm_off + y * sw + x] = smooth_direct(J, P, G, l, lut, x, y);
+            if (y < sh)
+                store_f32(J.smooth, 4 * (G.sm_off + y * sw + x), smooth_direct(J, P, G, l, lut, x, y));
         }
     }
 
@@ -842,7 +859,7 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
         const uint32_t y = static_cast<uint32_t>(y0) + r;
         wscore[k * 64 + lane] = 0;
         const bool inb = x < w && y < h;
-        if (inb && l > 0 && J.pyr) gptr(J.pyr)[G.pyr_off + y * w + x] = cen[k];
+        if (inb && l > 0 && J.pyr) store_f32(J.pyr, 4 * (G.pyr_off + y * w + x), cen[k]);
         bool need = inb && tile_x < G.wpr &&
                     (y == 0 ? (x % 9) < 3 : ((mrow[k] >> lane) & 1ull) != 0);
         if (quick)
@@ -1231,7 +1248,7 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
     for (uint32_t half = 0; half < 2; ++half)
 #pragma unroll
         for (uint32_t m = 0; m < TPL / 2; ++m) tst[half][m] = c_desc_tests[half][TPL / 2 * sub + m];
-    lut[tid] = static_cast<float>(tid) / 255.0f;
+    lut[tid] = c_u8_unit[tid];
     if (tid < kDescPts) {
         ppt[0][tid] = make_float2(c_desc_pts[0][tid][0], c_desc_pts[0][tid][1]);
         ppt[1][tid] = make_float2(c_desc_pts[1][tid][0], c_desc_pts[1][tid][1]);
@@ -2377,7 +2394,11 @@ hipError_t upload_pattern() {
     static int built = 0;  // 1 built, -1 failed
     if (!built) built = build_desc_tables(&tables) ? 1 : -1;
     if (built < 0) return hipErrorInvalidValue;
-    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_desc_pts), tables.pts, sizeof(tables.pts));
+    float unit[256];
+    for (uint32_t v = 0; v < 256; ++v) unit[v] = static_cast<float>(v) / 255.0f;
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_u8_unit), unit, sizeof(unit));
+    if (e == hipSuccess)
+        e = hipMemcpyToSymbol(HIP_SYMBOL(c_desc_pts), tables.pts, sizeof(tables.pts));
     if (e == hipSuccess)
         e = hipMemcpyToSymbol(HIP_SYMBOL(c_desc_tests), tables.tests, sizeof(tables.tests));
     if (e == hipSuccess)
