@@ -154,13 +154,7 @@ void build_level_taps(const Plan& P, const LevelGeom& G, orbfe::TapEntry* out) {
             out[sr + i * sh + y] = tap_entry(orbfe::axis_taps(uv + offs[i] * G.texel_y, G.h));
         }
     }
-    uint32_t cc, rc;
-    orbfe::level_code_offsets(0, G.w, G.h, &cc, &rc);
-    for (uint32_t x = 0; x < sw; ++x)
-        out[cc + x] = {orbfe::smooth_tap_code(out[sc + x].i01, out[sc + sw + x].i01,
-                                              out[sc + 2 * sw + x].i01,
-                                              static_cast<int32_t>(x), G.w),
-                       0.0f};
+    const uint32_t rc = orbfe::level_row_code_offset(0, G.w, G.h);
     for (uint32_t y = 0; y < sh; ++y)
         out[rc + y] = {orbfe::smooth_tap_code(out[sr + y].i01, out[sr + sh + y].i01,
                                               out[sr + 2 * sh + y].i01,

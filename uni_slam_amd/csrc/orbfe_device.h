@@ -103,7 +103,7 @@ struct LevelGeom {
     // column cx (w entries) and row cy (h entries) sampled from level 0 at
     // (c + 0.5) / n, then the 9-tap Gaussian's column taps [3][w + 4] and
     // row taps [3][h + 4] at offsets -1.2, 0, 1.2 texels, then the smoothing
-    // tap codes (smooth_tap_code) of every column [w + 4] and row [h + 4].
+    // tap codes (smooth_tap_code) of every row [h + 4].
     uint32_t tap_off;
     uint32_t pad_;
 };
@@ -342,15 +342,13 @@ __host__ __device__ inline void level_tap_offsets(uint32_t base, uint32_t w, uin
 }
 
 __host__ __device__ inline uint32_t level_tap_entries(uint32_t w, uint32_t h) {
-    return w + h + 4 * (w + kSmoothExt) + 4 * (h + kSmoothExt);
+    return w + h + 3 * (w + kSmoothExt) + 4 * (h + kSmoothExt);
 }
 
-// Entry offsets of the smoothing tap codes of a level (after its four
-// tables): one per column of the extended grid, then one per row.
-__host__ __device__ inline void level_code_offsets(uint32_t base, uint32_t w, uint32_t h,
-                                                   uint32_t* col_code, uint32_t* row_code) {
-    *col_code = base + w + h + 3 * (w + kSmoothExt) + 3 * (h + kSmoothExt);
-    *row_code = *col_code + w + kSmoothExt;
+// Entry offset of the smoothing tap codes of a level (after its four
+// tables): one per row of the extended grid.
+__host__ __device__ inline uint32_t level_row_code_offset(uint32_t base, uint32_t w, uint32_t h) {
+    return base + w + h + 3 * (w + kSmoothExt) + 3 * (h + kSmoothExt);
 }
 
 
@@ -367,14 +365,14 @@ __host__ __device__ inline uint32_t tint_clamp(int32_t c, uint32_t n) {
     return u < n - 1 ? u : n - 1;
 }
 
-// Smoothing tap code of point p along an axis of n texels, from the table
-// entries (i0 | i1 << 16) of its three taps at offsets -1.2, 0, 1.2: the
-// kernels' tile staging holds the texel of tint_clamp(g) at staged
-// coordinate g (0 outside the level with border_zero), so a tap whose entry
-// equals the pattern for sample floor f reads exactly the staged texels f,
-// f + 1.  Bit 0: the floors are p - 2, p or p - 1, p + 1 under tint_clamp;
-// bit 1: the middle tap's floor is p - 1 (bit 0 set); bits 2, 3: the same
-// with border_zero (floors inside the level).
+// Smoothing tap code of row p of a level of n rows, from the table entries
+// (i0 | i1 << 16) of its three taps at offsets -1.2, 0, 1.2: the kernels'
+// tile staging holds the texel of tint_clamp(g) at staged row g (0 outside
+// the level with border_zero), so a tap whose entry equals the pattern for
+// sample floor f reads exactly the staged rows f, f + 1.  Bit 0: the floors
+// are p - 2, p or p - 1, p + 1 under tint_clamp; bit 1: the middle tap's
+// floor is p - 1 (bit 0 set); bits 2, 3: the same with border_zero (floors
+// inside the level).
 enum : uint32_t { kCodeFast = 1u, kCodeMidLow = 2u, kCodeFastBz = 4u, kCodeMidLowBz = 8u };
 
 __host__ __device__ inline uint32_t smooth_tap_code(uint32_t e0, uint32_t e1, uint32_t e2,

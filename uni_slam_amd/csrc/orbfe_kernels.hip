@@ -29,6 +29,10 @@
 
 namespace orbfe {
 
+// f32(v) / 255 for every byte v (provider_base.cpp:121-124), correctly
+// rounded on the host; workgroups copy it into their LDS lookup tables.
+__constant__ float c_u8_unit[256];
+
 // The describe kernel evaluates the 256 BRIEF tests (brief_pattern.h) in two
 // groups of 128.  The 512 test points name only 375 distinct pattern
 // positions, and a test's two points always rotate together, so each group
@@ -39,18 +43,21 @@ namespace orbfe {
 // tests of either group) and the larger group has as few positions as a
 // local search finds: 189 and 189, against 220 and 221 for the tests of
 // each parity, so a keypoint gathers 378 values instead of 512.
+// A keypoint's sample of point id p goes to slot desc_slot(p) = p + p / 16
+// of its LDS buffer (one pad word per 16 against bank conflicts).
 //   c_desc_pts[g][p]     pattern position (x, y) of point id p of group g;
-//   c_desc_tests[g][m]   the group's tests in increasing order:
-//                        point id of A | point id of B << 8 | test << 16;
+//   c_desc_tests[g][m]   the group's tests in increasing order: byte offset
+//                        of A's slot | byte offset of B's slot << 16 |
+//                        (test mod 32) << 26;
 //   g_desc_order[v][..]  per lanes-per-keypoint variant v (8, 16, 32), angle
-//                        bin, group and lane: its PPL = kDescPts / LANES
-//                        point ids (u8, in rows of desc_order_row bytes) in
-//                        load order.
-// f32(v) / 255 for every byte v (provider_base.cpp:121-124), correctly
-// rounded on the host; workgroups copy it into their LDS lookup tables.
-__constant__ float c_u8_unit[256];
-
+//                        bin, group and lane: the slots of its PPL =
+//                        kDescPts / LANES points (u8, in rows of
+//                        desc_order_row bytes) in load order.
 constexpr uint32_t kDescPts = 192;
+constexpr uint32_t kDescSlots = kDescPts + kDescPts / 16;
+
+__host__ __device__ constexpr uint32_t desc_slot(uint32_t p) { return p + p / 16; }
+
 constexpr uint32_t kAngleBins = 64;
 constexpr float kAngleBinScale = 10.185916357881302f;  // kAngleBins / (2 pi)
 
@@ -697,31 +704,38 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     // A sample is the vertical lerp of two horizontal lerps of staged texels.
     //
     // Staged column (row) g - x0 + kHalo holds the level texel at
-    // tint_clamp(g) (0 outside the level when border_zero), so a tap whose
-    // table entry (i0, i1) equals (tint_clamp(f), tint_clamp(f + 1)) for a
-    // sample floor f (border_zero: (f, f + 1), inside the level) reads
-    // exactly the staged texels f, f + 1.  Where the plan's tap codes
-    // (smooth_tap_code) confirm the floors x - 2, x or x - 1, and x + 1 (all
-    // but the first two columns and rows of a level at the sizes in use),
-    // the horizontal lerps of a wave's rows y - 2 .. y + RPT + 1 sit in a
-    // register window shared by its RPT outputs.  Other lanes and waves
-    // evaluate each sample from the staged texels the table names.  A
-    // middle row tap with fraction 0 (most rows) yields its first operand
-    // exactly (no value here is -0, infinite or NaN), so its lerp is skipped.
+    // tint_clamp(g) (0 outside the level when border_zero).  A sampler tap
+    // names texels inside the level (clamp to edge), and for such a texel g
+    // that staged column holds g itself, so each lane reads its three
+    // column taps at the staged columns their table entries name, at the
+    // level's edges too.  Rows: where the plan's row tap codes
+    // (smooth_tap_code) confirm the floors y - 2, y or y - 1, and y + 1 for
+    // all of a wave's output rows (all but the first two rows of a level),
+    // the horizontal lerps of its rows y - 2 .. y + RPT + 1 sit in a
+    // register window shared by its RPT outputs; other waves evaluate each
+    // sample from the staged texels the tables name.  A middle row tap with
+    // fraction 0 (most rows) yields its first operand exactly (no value here
+    // is -0, infinite or NaN), so its lerp is skipped.
     const bool slow = x0 > static_cast<int32_t>(w) + 2 || y0 > static_cast<int32_t>(h) + 2;
     if (!slow) {
         if (x < sw) {
-            uint32_t t_cc, t_rc;
-            level_code_offsets(G.tap_off, w, h, &t_cc, &t_rc);
+            const uint32_t t_rc = level_row_code_offset(G.tap_off, w, h);
             TapEntry tp[3];
 #pragma unroll
             for (int i = 0; i < 3; ++i) tp[i] = load_tap(taps, t_sc + i * sw + x);
-            const uint32_t ccode = load_tap(taps, t_cc + x).i01;
             const bool bz = prm.border_zero != 0;
             const uint32_t fast_bit = bz ? kCodeFastBz : kCodeFast;
             const uint32_t low_bit = bz ? kCodeMidLowBz : kCodeMidLow;
-            const bool col_fast = (ccode & fast_bit) != 0;
-            const bool col_a1 = (ccode & low_bit) == 0;
+            // Staged columns of the lane's column taps: a tap names level
+            // texels g in [0, w) (the sampler clamps to the edge), and staged
+            // column g - x0 + kHalo holds texel g for every such g the tile's
+            // halo reaches, in both border modes, at the level's edges too.
+            uint32_t ca[3], cb[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                ca[i] = (tp[i].i01 & 0xFFFFu) - x0 + kHalo;
+                cb[i] = (tp[i].i01 >> 16) - x0 + kHalo;
+            }
             // wave-uniform (scalar loads): rows of this wave's outputs
             bool row_fast = true;
             uint32_t row_b1 = 0;  // bit k: middle tap of output k has floor y - 1
@@ -734,16 +748,17 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
                 if (rcode & low_bit) row_b1 |= 1u << k;
             }
             const float fr0 = tp[0].frac, fr1 = tp[1].frac, fr2 = tp[2].frac;
-            if (row_fast && col_fast) {
+            if (row_fast) {  // wave-uniform
+                const float frs[3] = {fr0, fr1, fr2};
                 float hv[3][RPT + 4];
 #pragma unroll
                 for (int m = 0; m < RPT + 4; ++m) {
-                    const float* tr = &tile[wave * RPT + 1 + m][lane + 1];  // row y - 2 + m
-                    const float v0 = tr[0], v1 = tr[1], v2 = tr[2], v3 = tr[3], v4 = tr[4];
-                    hv[0][m] = v0 + fr0 * (v1 - v0);
-                    const float a = col_a1 ? v2 : v1, b = col_a1 ? v3 : v2;
-                    hv[1][m] = a + fr1 * (b - a);
-                    hv[2][m] = v3 + fr2 * (v4 - v3);
+                    const float* tr = tile[wave * RPT + 1 + m];  // row y - 2 + m
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) {
+                        const float t0 = tr[ca[i]];
+                        hv[i][m] = t0 + frs[i] * (tr[cb[i]] - t0);
+                    }
                 }
 #pragma unroll
                 for (int k = 0; k < RPT; ++k) {
@@ -1197,7 +1212,8 @@ struct LevelSrc {
 constexpr uint32_t kDescribeThreads = 256;
 
 template <uint32_t LANES, bool kAllLoads>
-__global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job job0,
+__global__ __launch_bounds__(kDescribeThreads)
+__attribute__((amdgpu_waves_per_eu(LANES == 16 ? 8 : 1))) void k_describe(BatchView bv, Job job0,
                                                                 Params prm) {
     constexpr uint32_t KPB = kDescribeThreads / LANES;  // keypoints per block
     constexpr uint32_t TPL = 256 / LANES;               // tests per lane
@@ -1210,13 +1226,14 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
     constexpr int32_t kBoxR = 3;
     constexpr uint32_t kBoxSide = 2 * kBoxR + 1, kBoxN = kBoxSide * kBoxSide;
     // Sample points of one test group (c_desc_pts): point id q of group
-    // `half` is the pattern position c_desc_pts[half][q].
+    // `half` is the pattern position c_desc_pts[half][q]; its sample sits in
+    // slot desc_slot(q) of the keypoint's buffer (tables hold slots).
     constexpr uint32_t PPL = kDescPts / LANES;          // points per lane and group
     constexpr uint32_t ROW = desc_order_row<LANES>();   // gather-order bytes per lane
-    constexpr uint32_t kSlots = kDescPts + kDescPts / 16;  // point q at q + q / 16
+    constexpr uint32_t kSlots = kDescSlots;
     static_assert(PPL <= ROW && TPL % 2 == 0, "gather rows hold PPL ids; TPL / 2 tests per group");
     __shared__ float lut[256];
-    __shared__ float2 ppt[2][kDescPts];  // pattern position of each group's point ids
+    __shared__ float2 ppt[2][kSlots];    // pattern position of each group's slots
     __shared__ float box[KPB][kBoxN];    // per keypoint: level texels around it
     __shared__ float samp[KPB][kSlots];  // per keypoint: smoothed value of each point
     const uint32_t tid = threadIdx.x;
@@ -1250,8 +1267,8 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
         for (uint32_t m = 0; m < TPL / 2; ++m) tst[half][m] = c_desc_tests[half][TPL / 2 * sub + m];
     lut[tid] = c_u8_unit[tid];
     if (tid < kDescPts) {
-        ppt[0][tid] = make_float2(c_desc_pts[0][tid][0], c_desc_pts[0][tid][1]);
-        ppt[1][tid] = make_float2(c_desc_pts[1][tid][0], c_desc_pts[1][tid][1]);
+        ppt[0][desc_slot(tid)] = make_float2(c_desc_pts[0][tid][0], c_desc_pts[0][tid][1]);
+        ppt[1][desc_slot(tid)] = make_float2(c_desc_pts[1][tid][0], c_desc_pts[1][tid][1]);
     }
     __syncthreads();
 
@@ -1399,8 +1416,9 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
         desc_order<LANES>() + (static_cast<size_t>(bin) * 2 * LANES + sub) * ROW);
     float* ks = samp[tid / LANES];
     uint32_t bits = 0;
-    // Byte offset in the smoothed image of point id q of a group, rotated by
-    // the keypoint's angle (offsets < 4 * sw * sh: 24-bit factors).
+    // Byte offset in the smoothed image of the point in slot q of a group,
+    // rotated by the keypoint's angle (offsets < 4 * sw * sh: 24-bit
+    // factors).
     const auto point_offset = [&](uint32_t half, uint32_t q) -> uint32_t {
         const float2 pp = ppt[half][q];
         const uint32_t ax4 =
@@ -1408,20 +1426,22 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
         const uint32_t ay = umin(__float_as_uint((pp.x * sn + pp.y * cs) + kRound) + yb, sh - 1);
         return __umul24(ay, sw4) + ax4;
     };
-    // The lane's tests of a group, once the group's samples are in ks: test
-    // t sets bit t - TPL * sub of the lane's bits.
+    // The lane's tests of a group, once the group's samples are in ks: entry
+    // e names the byte offsets of A's and B's samples in ks and the test t,
+    // which sets bit t - TPL * sub = t mod TPL of the lane's bits.
+    const char* ksb = reinterpret_cast<const char*>(ks);
     const auto group_tests = [&](uint32_t half) {
         wave_sync();
 #pragma unroll
         for (uint32_t m = 0; m < TPL / 2; ++m) {
             const uint32_t e = tst[half][m];
-            const uint32_t qa = e & 0xFFu, qb = (e >> 8) & 0xFFu;
-            const float sa = ks[qa + (qa >> 4)], sb = ks[qb + (qb >> 4)];
-            bits |= (sa > sb ? 1u : 0u) << ((e >> 16) & (TPL - 1));
+            const float sa = *reinterpret_cast<const float*>(ksb + (e & 0xFFFFu));
+            const float sb = *reinterpret_cast<const float*>(ksb + ((e >> 16) & 0x3FFu));
+            bits |= (sa > sb ? 1u : 0u) << ((e >> 26) & (TPL - 1));
         }
         wave_sync();
     };
-    const auto point_id = [&](const uint32_t* ids, uint32_t k) -> uint32_t {
+    const auto point_slot = [&](const uint32_t* ids, uint32_t k) -> uint32_t {
         return (ids[k / 4] >> (8 * (k % 4))) & 0xFFu;
     };
     if constexpr (kAllLoads) {
@@ -1436,14 +1456,14 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
             for (uint32_t k = 0; k < ROW / 4; ++k) ids[k] = order[half * LANES * ROW / 4 + k];
 #pragma unroll
             for (uint32_t k = 0; k < PPL; ++k) {
-                qs[half][k] = point_id(ids, k);
+                qs[half][k] = point_slot(ids, k);
                 v[half][k] = load_f32(sm, point_offset(half, qs[half][k]));
             }
         }
 #pragma unroll
         for (uint32_t half = 0; half < 2; ++half) {
 #pragma unroll
-            for (uint32_t k = 0; k < PPL; ++k) ks[qs[half][k] + (qs[half][k] >> 4)] = v[half][k];
+            for (uint32_t k = 0; k < PPL; ++k) ks[qs[half][k]] = v[half][k];
             group_tests(half);
         }
     } else {
@@ -1461,11 +1481,11 @@ __global__ __launch_bounds__(kDescribeThreads) void k_describe(BatchView bv, Job
                 uint32_t qs[kB];
 #pragma unroll
                 for (uint32_t u = 0; u < kB; ++u) {
-                    qs[u] = point_id(ids, k0 + u);
+                    qs[u] = point_slot(ids, k0 + u);
                     v[u] = load_f32(sm, point_offset(half, qs[u]));
                 }
 #pragma unroll
-                for (uint32_t u = 0; u < kB; ++u) ks[qs[u] + (qs[u] >> 4)] = v[u];
+                for (uint32_t u = 0; u < kB; ++u) ks[qs[u]] = v[u];
             }
             group_tests(half);
         }
@@ -2323,9 +2343,10 @@ static void split_tests(uint8_t group[256], uint32_t npos[2]) {
 
 // Gather order of one lanes-per-keypoint variant: for each angle bin and
 // group, the group's kDescPts point ids sorted by (row, column) of their
-// position rotated to the bin's centre angle; lane s takes ids s, s + LANES,
-// ... of that order (one load of the group's lanes: consecutive ids).  The
-// order only decides which lane loads which point, never a value.
+// position rotated to the bin's centre angle; lane s takes points s,
+// s + LANES, ... of that order (one load of the group's lanes: consecutive
+// points), stored as their slots.  The order only decides which lane loads
+// which point, never a value.
 static void build_desc_order(const int8_t (*pts)[kDescPts][2], uint8_t* out, uint32_t lanes,
                              uint32_t row) {
     const uint32_t ppl = kDescPts / lanes;
@@ -2345,7 +2366,7 @@ static void build_desc_order(const int8_t (*pts)[kDescPts][2], uint8_t* out, uin
             uint8_t* dst = out + (static_cast<size_t>(bin) * 2 + g) * lanes * row;
             for (uint32_t s = 0; s < lanes; ++s)
                 for (uint32_t k = 0; k < row; ++k)
-                    dst[s * row + k] = k < ppl ? static_cast<uint8_t>(seq[lanes * k + s]) : 0;
+                    dst[s * row + k] = k < ppl ? static_cast<uint8_t>(desc_slot(seq[lanes * k + s])) : 0;
         }
     }
 }
@@ -2375,7 +2396,7 @@ static bool build_desc_tables(DescTables* d) {
             if (group[t] != g) continue;
             const int8_t* pt = kOrbfeBriefPattern[t];
             const uint32_t qa = id(pt[0], pt[1]), qb = id(pt[2], pt[3]);
-            d->tests[g][m++] = qa | qb << 8 | t << 16;
+            d->tests[g][m++] = 4 * desc_slot(qa) | 4 * desc_slot(qb) << 16 | (t & 31u) << 26;
         }
         if (m != 128) return false;
         for (uint32_t q = n; q < kDescPts; ++q) {  // unread slots: a valid position
