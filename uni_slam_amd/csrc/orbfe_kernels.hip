@@ -1368,15 +1368,34 @@ __attribute__((amdgpu_waves_per_eu(LANES == 16 ? 8 : 1))) void k_describe(BatchV
 
     // Orientation (generate_features.wgsl:80-99): sequential moments.
     float m10 = 0.0f, m01 = 0.0f;
-    for (int32_t dy = -r; dy <= r; ++dy) {
-        for (int32_t dx = -r; dx <= r; ++dx) {
-            if (dx * dx + dy * dy > r * r) continue;
-            const float pixel =
-                boxed ? bx[(kBoxR + dy) * static_cast<int32_t>(kBoxSide) + kBoxR + dx]
-                      : S.load(static_cast<int32_t>(x) + dx, static_cast<int32_t>(y) + dy,
-                               prm.border_zero);
-            m10 += static_cast<float>(dx) * pixel;
-            m01 += static_cast<float>(dy) * pixel;
+    if (r == kBoxR) {
+        // The shader's radius: its 29-point disc unrolled, in the shader's
+        // order, texels at constant offsets in the box.  A zero weight's
+        // term is +0 (texels are finite and >= 0), and neither sum is ever
+        // -0 (both start at +0, and a round-to-nearest sum is -0 only when
+        // both operands are), so adding that term changes nothing and it is
+        // left out.
+#pragma unroll
+        for (int32_t dy = -kBoxR; dy <= kBoxR; ++dy) {
+#pragma unroll
+            for (int32_t dx = -kBoxR; dx <= kBoxR; ++dx) {
+                if (dx * dx + dy * dy > kBoxR * kBoxR) continue;
+                const float pixel = bx[(kBoxR + dy) * static_cast<int32_t>(kBoxSide) + kBoxR + dx];
+                if (dx != 0) m10 += static_cast<float>(dx) * pixel;
+                if (dy != 0) m01 += static_cast<float>(dy) * pixel;
+            }
+        }
+    } else {
+        for (int32_t dy = -r; dy <= r; ++dy) {
+            for (int32_t dx = -r; dx <= r; ++dx) {
+                if (dx * dx + dy * dy > r * r) continue;
+                const float pixel =
+                    boxed ? bx[(kBoxR + dy) * static_cast<int32_t>(kBoxSide) + kBoxR + dx]
+                          : S.load(static_cast<int32_t>(x) + dx, static_cast<int32_t>(y) + dy,
+                                   prm.border_zero);
+                m10 += static_cast<float>(dx) * pixel;
+                m01 += static_cast<float>(dy) * pixel;
+            }
         }
     }
     const float theta = atan2_own(m01, m10);
