@@ -270,14 +270,15 @@ __host__ __device__ inline float atan2_own(float y, float x) {
     const bool swap = ay > ax;
     const float mx = swap ? ay : ax;
     const float mn = swap ? ax : ay;
-    const float t = mx == 0.0f ? 0.0f : mn / mx;
-    float a;
-    if (t > fm::kTanPi8) {
-        const float u = (t - 1.0f) / (t + 1.0f);
-        a = fm::kPio4 + atan_taylor(u);
-    } else {
-        a = atan_taylor(t);
-    }
+    // Both divisions and one polynomial on every lane, no divergent branch
+    // (lanes of one wave take either side); each lane's operations are the
+    // ones of its side.  mx == 0 gives t = 0 whatever the quotient.
+    const float q = mn / (mx == 0.0f ? 1.0f : mx);
+    const float t = mx == 0.0f ? 0.0f : q;
+    const bool reduce = t > fm::kTanPi8;
+    const float u = (t - 1.0f) / (t + 1.0f);
+    float a = atan_taylor(reduce ? u : t);
+    if (reduce) a = fm::kPio4 + a;
     if (swap) a = fm::kPio2Hi - a;
     if (__builtin_signbit(x)) a = fm::kPi - a;
     if (__builtin_signbit(y)) a = -a;
@@ -307,6 +308,28 @@ __host__ __device__ inline void sincos_own(float x, float* s, float* c) {
     const float c0 = (q & 1) ? sn : cs;
     *s = (q & 2) ? -s0 : s0;
     *c = ((q + 1) & 2) ? -c0 : c0;
+}
+
+// 1 + the longest run of set bits of a 16-bit ring mask (no wrap): the
+// `streak` of detect_corners.wgsl:67-79 (curr = 1, streak = 1, then per
+// ring test curr = diff ? curr + 1 : 1, streak = max(streak, curr)).
+// r_k marks the positions where k set bits start (r_2k = r_k & r_k >> k);
+// the run is built greedily from 8, 4, 2 and 1: a run of n bits from i
+// followed by one of k bits from i + n is a run of n + k bits.
+__host__ __device__ inline uint32_t ring_streak(uint32_t ring) {
+    const uint32_t r1 = ring & 0xFFFFu;
+    const uint32_t r2 = r1 & (r1 >> 1), r4 = r2 & (r2 >> 2), r8 = r4 & (r4 >> 4);
+    uint32_t cur = 0xFFFFu, n = 0;  // cur: starts of runs of n set bits
+    const uint32_t rk[4] = {r8, r4, r2, r1};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t c = cur & (rk[j] >> n);
+        if (c != 0u) {
+            cur = c;
+            n += 8u >> j;
+        }
+    }
+    return 1u + (r1 == 0xFFFFu ? 16u : n);
 }
 
 // One axis of the linear clamp-to-edge sampler.

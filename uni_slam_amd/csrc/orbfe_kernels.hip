@@ -1358,12 +1358,7 @@ __attribute__((amdgpu_waves_per_eu(LANES == 16 ? 8 : 1))) void k_describe(BatchV
         }
         ring |= static_cast<uint32_t>((__ballot(diff) >> gbase) & GMASK) << (q * LANES);
     }
-    ring &= 0xFFFFu;
-    uint32_t curr = 1, streak = 1;
-    for (int i = 0; i < 16; ++i) {
-        curr = ((ring >> i) & 1u) ? curr + 1 : 1;
-        streak = curr > streak ? curr : streak;
-    }
+    const uint32_t streak = ring_streak(ring);
     const uint32_t strength = streak > prm.min_run ? streak : 0u;
 
     // Orientation (generate_features.wgsl:80-99): sequential moments.
