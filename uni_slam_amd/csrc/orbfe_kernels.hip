@@ -45,7 +45,9 @@ __constant__ float c_u8_unit[256];
 // each parity, so a keypoint gathers 378 values instead of 512.
 // A keypoint's sample of point id p goes to slot desc_slot(p) = p + p / 16
 // of its LDS buffer (one pad word per 16 against bank conflicts).
-//   c_desc_pts[g][p]     pattern position (x, y) of point id p of group g;
+//   c_desc_ppt8[s]       pattern positions in slot s of both groups, int8 x
+//                        and y of group 0 in bytes 0-1, of group 1 in 2-3
+//                        (0 in pad slots);
 //   c_desc_tests[g][m]   the group's tests in increasing order: byte offset
 //                        of A's slot | byte offset of B's slot << 16 |
 //                        (test mod 32) << 26;
@@ -68,7 +70,7 @@ constexpr uint32_t desc_order_row() {
 }
 
 constexpr size_t kDescOrderBytes = kAngleBins * 2 * 32 * desc_order_row<32>();  // the largest
-__constant__ int8_t c_desc_pts[2][kDescPts][2];
+__constant__ uint32_t c_desc_ppt8[kDescSlots];
 __constant__ uint32_t c_desc_tests[2][128];
 __device__ uint8_t g_desc_order[3][kDescOrderBytes];
 
@@ -1225,15 +1227,24 @@ __attribute__((amdgpu_waves_per_eu(LANES == 16 ? 8 : 1))) void k_describe(BatchV
     // orientation radii up to 3, the moment disc.
     constexpr int32_t kBoxR = 3;
     constexpr uint32_t kBoxSide = 2 * kBoxR + 1, kBoxN = kBoxSide * kBoxSide;
-    // Sample points of one test group (c_desc_pts): point id q of group
-    // `half` is the pattern position c_desc_pts[half][q]; its sample sits in
-    // slot desc_slot(q) of the keypoint's buffer (tables hold slots).
+    // Sample points of one test group: the point in slot q of group `half`
+    // is at the pattern position in bytes 2 * half, 2 * half + 1 of
+    // c_desc_ppt8[q]; its sample sits in slot q of the keypoint's buffer.
+    // The tables sit in LDS (one load per thread to fill them), so the
+    // texture-address path, which bounds this kernel in batches, serves only
+    // the frame's gathers.
     constexpr uint32_t PPL = kDescPts / LANES;          // points per lane and group
     constexpr uint32_t ROW = desc_order_row<LANES>();   // gather-order bytes per lane
     constexpr uint32_t kSlots = kDescSlots;
     static_assert(PPL <= ROW && TPL % 2 == 0, "gather rows hold PPL ids; TPL / 2 tests per group");
     __shared__ float lut[256];
-    __shared__ float2 ppt[2][kSlots];    // pattern position of each group's slots
+    // Pattern positions of both groups' slots: packed int8 (4 B per slot)
+    // where LDS is tight (8 and 16 lanes: 16 or 32 keypoints per block),
+    // float pairs per group (no conversion before the gathers) at 32 lanes.
+    constexpr bool kPackedPts = LANES < 32;
+    __shared__ uint32_t ppt[kPackedPts ? kSlots : 4 * kSlots];
+    float2 (*const pptf)[kSlots] = reinterpret_cast<float2 (*)[kSlots]>(ppt);
+    __shared__ uint32_t tests[2][128];   // c_desc_tests
     __shared__ float box[KPB][kBoxN];    // per keypoint: level texels around it
     __shared__ float samp[KPB][kSlots];  // per keypoint: smoothed value of each point
     const uint32_t tid = threadIdx.x;
@@ -1260,15 +1271,20 @@ __attribute__((amdgpu_waves_per_eu(LANES == 16 ? 8 : 1))) void k_describe(BatchV
     ConstPlan P = as_const(J.plan);
     const LevelGeom G = copy_const(&P.lv[l]);  // by value: stores below cannot alias it
     // This lane's TPL / 2 tests of each group (tests TPL * sub .. + TPL - 1).
-    uint32_t tst[2][TPL / 2];
-#pragma unroll
-    for (uint32_t half = 0; half < 2; ++half)
-#pragma unroll
-        for (uint32_t m = 0; m < TPL / 2; ++m) tst[half][m] = c_desc_tests[half][TPL / 2 * sub + m];
+    static_assert(kDescribeThreads == 256, "one lookup entry and one test entry per thread");
     lut[tid] = c_u8_unit[tid];
-    if (tid < kDescPts) {
-        ppt[0][desc_slot(tid)] = make_float2(c_desc_pts[0][tid][0], c_desc_pts[0][tid][1]);
-        ppt[1][desc_slot(tid)] = make_float2(c_desc_pts[1][tid][0], c_desc_pts[1][tid][1]);
+    tests[tid >> 7][tid & 127u] = c_desc_tests[tid >> 7][tid & 127u];
+    if (tid < kSlots) {
+        const uint32_t v = c_desc_ppt8[tid];
+        if constexpr (kPackedPts) {
+            ppt[tid] = v;
+        } else {
+            const auto f = [v](int b) {
+                return static_cast<float>(static_cast<int8_t>((v >> (8 * b)) & 0xFFu));
+            };
+            pptf[0][tid] = make_float2(f(0), f(1));
+            pptf[1][tid] = make_float2(f(2), f(3));
+        }
     }
     __syncthreads();
 
@@ -1401,7 +1417,7 @@ __attribute__((amdgpu_waves_per_eu(LANES == 16 ? 8 : 1))) void k_describe(BatchV
     // coordinates (:116-129) is looked up in the smoothed image, where
     // coordinates past the extension (including wrapped negatives) share the
     // value at w+3 / h+3.  The tests run in two groups: each group's
-    // distinct sample points (c_desc_pts, at most kDescPts) are gathered into
+    // distinct sample points (c_desc_ppt8, at most kDescPts) are gathered into
     // LDS, then every lane evaluates its tests of the group from there.  Two
     // tests that name the same pattern position read the same rotated point,
     // so gathering it once changes no value.  The gather order comes from a
@@ -1434,7 +1450,14 @@ __attribute__((amdgpu_waves_per_eu(LANES == 16 ? 8 : 1))) void k_describe(BatchV
     // rotated by the keypoint's angle (offsets < 4 * sw * sh: 24-bit
     // factors).
     const auto point_offset = [&](uint32_t half, uint32_t q) -> uint32_t {
-        const float2 pp = ppt[half][q];
+        float2 pp;
+        if constexpr (kPackedPts) {
+            const uint32_t pv = ppt[q] >> (16 * half);
+            pp = make_float2(static_cast<float>(static_cast<int8_t>(pv & 0xFFu)),
+                             static_cast<float>(static_cast<int8_t>((pv >> 8) & 0xFFu)));
+        } else {
+            pp = pptf[half][q];
+        }
         const uint32_t ax4 =
             umin(4 * __float_as_uint((pp.x * cs - pp.y * sn) + kRound) + xb4, xmax4);
         const uint32_t ay = umin(__float_as_uint((pp.x * sn + pp.y * cs) + kRound) + yb, sh - 1);
@@ -1448,7 +1471,7 @@ __attribute__((amdgpu_waves_per_eu(LANES == 16 ? 8 : 1))) void k_describe(BatchV
         wave_sync();
 #pragma unroll
         for (uint32_t m = 0; m < TPL / 2; ++m) {
-            const uint32_t e = tst[half][m];
+            const uint32_t e = tests[half][TPL / 2 * sub + m];
             const float sa = *reinterpret_cast<const float*>(ksb + (e & 0xFFFFu));
             const float sb = *reinterpret_cast<const float*>(ksb + ((e >> 16) & 0x3FFu));
             bits |= (sa > sb ? 1u : 0u) << ((e >> 26) & (TPL - 1));
@@ -2294,9 +2317,10 @@ __global__ __launch_bounds__(kRansacThreads) void k_ransac_score(
 // ---------------------------------------------------------------------------
 // Host-side launchers (called from orbfe_api.cpp).
 // ---------------------------------------------------------------------------
-// Tables of the describe kernel (c_desc_pts, c_desc_tests, g_desc_order).
+// Tables of the describe kernel (c_desc_ppt8, c_desc_tests, g_desc_order).
 struct DescTables {
-    int8_t pts[2][kDescPts][2];
+    int8_t pts[2][kDescPts][2];      // position of point id p of group g
+    uint32_t ppt8[kDescSlots];        // the device form, by slot (c_desc_ppt8)
     uint32_t tests[2][128];
     uint8_t order[3][kDescOrderBytes];
 };
@@ -2418,6 +2442,14 @@ static bool build_desc_tables(DescTables* d) {
             d->pts[g][q][1] = d->pts[g][0][1];
         }
     }
+    for (uint32_t q = 0; q < kDescSlots; ++q) d->ppt8[q] = 0;
+    for (uint32_t p = 0; p < kDescPts; ++p) {
+        uint32_t v = 0;
+        for (uint32_t g = 0; g < 2; ++g)
+            for (uint32_t c = 0; c < 2; ++c)
+                v |= static_cast<uint32_t>(static_cast<uint8_t>(d->pts[g][p][c])) << (8 * (2 * g + c));
+        d->ppt8[desc_slot(p)] = v;
+    }
     build_desc_order(d->pts, d->order[0], 8, desc_order_row<8>());
     build_desc_order(d->pts, d->order[1], 16, desc_order_row<16>());
     build_desc_order(d->pts, d->order[2], 32, desc_order_row<32>());
@@ -2433,7 +2465,7 @@ hipError_t upload_pattern() {
     for (uint32_t v = 0; v < 256; ++v) unit[v] = static_cast<float>(v) / 255.0f;
     hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_u8_unit), unit, sizeof(unit));
     if (e == hipSuccess)
-        e = hipMemcpyToSymbol(HIP_SYMBOL(c_desc_pts), tables.pts, sizeof(tables.pts));
+        e = hipMemcpyToSymbol(HIP_SYMBOL(c_desc_ppt8), tables.ppt8, sizeof(tables.ppt8));
     if (e == hipSuccess)
         e = hipMemcpyToSymbol(HIP_SYMBOL(c_desc_tests), tables.tests, sizeof(tables.tests));
     if (e == hipSuccess)
