@@ -2502,9 +2502,9 @@ hipError_t launch_compact(uint32_t frames, BatchView bv, const Job& job0, uint32
     return hipGetLastError();
 }
 
-// Single-frame launches (no batch table) use the variant with both gather
-// halves in flight (latency-bound: a few waves per SIMD); batches keep one
-// half at a time (fewer VGPRs, more resident waves).
+// Single-frame launches (no batch table) use the variant with both test
+// groups' gathers in flight (latency-bound: a few waves per SIMD); batches
+// gather one group at a time (fewer VGPRs, more resident waves).
 template <uint32_t LANES>
 void launch_describe_t(uint32_t blocks, BatchView bv, const Job& job0, const Params& prm,
                        hipStream_t s) {
