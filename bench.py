@@ -613,11 +613,11 @@ def main():
                 "valu": valu_roofline(dom, per_kernel[dom]["avg_us"], peak["wave_instr_per_s"]),
                 "limiter": "priced against HBM (the roofline of its algorithmic bytes); what "
                            "limits it is instruction issue and per-tile latency: f32 bilinear "
-                           "and 9-tap smoothing arithmetic, VALU issue about 0.55 of the probed "
-                           "peak in 32-frame batches and 0.33 on one frame, whose 2775 tiles "
-                           "fill the device 1.35 times at 8 workgroups per CU (the wave-slot "
-                           "limit, 8 waves per SIMD) "
-                           "(profiles/valu_1280x720.md)",
+                           "and 9-tap smoothing arithmetic (roofline.valu: VALU issue against "
+                           "the probed peak, live on one frame and from the profile in "
+                           "32-frame batches); one frame's 2775 tiles fill the device 1.35 "
+                           "times at 8 workgroups per CU (the wave-slot limit, 8 waves per "
+                           "SIMD) (profiles/valu_1280x720.md)",
                 "avg_us": per_kernel[dom]["avg_us"], "bytes_per_launch": kb[dom]}
 
     extra = {}
