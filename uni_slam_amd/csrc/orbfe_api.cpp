@@ -514,7 +514,8 @@ int get_plan(orbfe_ctx* ctx, uint32_t W0, uint32_t H0, uint32_t cap, PlanEntry**
     rc = dalloc(ctx, &pe->dev, sizeof(Plan));
     if (rc) return rc;
     HIP_TRY(ctx, hipMemcpy(pe->dev, &P, sizeof(Plan), hipMemcpyHostToDevice));
-    pe->surv_bytes = align_up(static_cast<size_t>(words) * 8);
+    // survivor words, then one survivor count per detect wave (4 per tile)
+    pe->surv_bytes = align_up(static_cast<size_t>(words) * 8 + static_cast<size_t>(tiles) * 16);
     pe->kp_bytes = align_up(static_cast<size_t>(L) * cap * 4);
     pe->pyr_bytes = align_up(static_cast<size_t>(pyr) * 4);
     pe->feat_bytes = align_up(static_cast<size_t>(L) * P.feat_stride);

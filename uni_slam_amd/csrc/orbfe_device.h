@@ -180,13 +180,21 @@ struct Job {
     uint64_t pitch;
     float* pyr;        // nullable
     uint8_t* feat;     // feature array
-    uint64_t* surv;    // survivor bitmap scratch, plan->words
+    uint64_t* surv;    // survivor bitmap scratch, plan->words, then the survivor
+                       // count of each detect wave (u32, 4 per tile: surv_counts)
     uint32_t* kp;      // keypoint list scratch, L*cap (x | y << 16)
     float* smooth;     // smoothed-image scratch, plan->sm_floats
     // copies of plan fields, so a describe block finds its level's records
     // without waiting for the plan
     uint32_t desc_groups, cap, feat_stride, pad_;
 };
+
+// Survivor counts of the detect waves of a frame: wave w of detect tile t
+// (frame-wide tile index, LevelGeom::tile_base + ty * tiles_x + tx) writes
+// the survivors of its TH / 4 rows at 4 * t + w, after the survivor words.
+__host__ __device__ inline uint32_t* surv_counts(uint64_t* surv, uint32_t words) {
+    return reinterpret_cast<uint32_t*>(surv + words);
+}
 
 struct MatchJob {
     const Plan* plan;

@@ -935,6 +935,11 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
         __syncthreads();  // row0_score is read across waves (block-uniform branch)
     }
 
+    // The wave's survivor count (its rows' popcounts; 0 for extension-only
+    // tiles and rows past the level) goes after the survivor words, so a
+    // compaction part finds the survivors before its rows without
+    // recounting words.
+    uint32_t wave_count = 0;  // wave-uniform
     if (tile_x < G.wpr) {  // extension-only tiles have no corners
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
@@ -959,17 +964,22 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
         }
         const uint64_t bits = __ballot(keep);
         if (lane == 0) gptr(J.surv)[G.word_off + y * G.wpr + tile_x] = bits;
+        wave_count += __popcll(bits);
     }
     }
+    if (lane == 0)
+        gptr(surv_counts(J.surv, P.words))[4 * (G.tile_base + T.tile_y * G.tiles_x + tile_x) + wave] =
+            wave_count;
 }
 
 // ---------------------------------------------------------------------------
 // k_compact: `parts` workgroups of 16 waves per (frame, level), part p
-// taking the p-th slice of the level's survivor words (whole 64-word
-// chunks): 8 for a single frame (the level's work spread over 8 CUs), 1 for
-// batches (enough workgroups already; no recount).  A part first counts the survivors of the words before its slice
-// (a popcount pass over words the detect kernel just wrote) for its base
-// rank, so the parts need nothing from each other.  Its slice is cut into
+// taking the p-th share of the level's detect tile rows (their survivor
+// words): 8 for a single frame (the level's work spread over 8 CUs), 1 for
+// batches (enough workgroups already).  A part first sums the survivor
+// counts the detect waves of the tile rows before its share wrote (one
+// load per thread for a 1280x720 level) for its base rank, so the parts
+// need nothing from each other.  Its slice of words is cut into
 // chunks of 64 words (one per lane, read coalesced); in each round of up to
 // kRoundChunks chunks, chunk c goes to wave c % 16.  A round counts its
 // chunks (words kept in registers), a block scan over the chunk counts gives
@@ -1071,32 +1081,41 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(BatchView bv, Job j
     ConstPlan P = as_const(J.plan);
     const LevelGeom G = copy_const(&P.lv[l]);  // by value: stores below cannot alias it
     const uint32_t words = G.h * G.wpr;
-    const uint32_t nchunks = (words + 63) / 64;
-    const uint32_t part_chunks = (nchunks + parts - 1) / parts;
-    const uint32_t c0 = umin(part * part_chunks, nchunks);
-    const uint32_t c1 = umin(c0 + part_chunks, nchunks);
+    // Part p takes whole detect tile rows (tile_h level rows each): words
+    // [w0, w1) of the level, within the 64-word chunks [c0, c1).
+    const uint32_t th = P.tile_h;
+    const uint32_t trows = (G.h + th - 1) / th;
+    const uint32_t part_rows = (trows + parts - 1) / parts;
+    const uint32_t tr0 = umin(part * part_rows, trows);
+    const uint32_t tr1 = umin(tr0 + part_rows, trows);
+    const uint32_t w0 = tr0 * th * G.wpr;
+    const uint32_t w1 = umin(tr1 * th, G.h) * G.wpr;
+    const uint32_t c0 = w0 / 64, c1 = (w1 + 63) / 64;
     const ORBFE_GLOBAL uint64_t* surv = gptr(J.surv) + G.word_off;
     ORBFE_GLOBAL uint32_t* kp = gptr(J.kp) + G.kp_off;
     const uint32_t cap = P.cap;
     const bool last = part == parts - 1;
 
-    // Survivors before the slice: words [0, 64 * c0).
+    // Survivors before the slice: the detect waves' counts of the tile rows
+    // [0, tr0) (4 per tile, tiles_x tiles per tile row, contiguous).
     uint32_t carry = 0;  // block-uniform
-    if (c0 > 0) {
-        const uint32_t w0 = umin(64 * c0, words);
+    if (tr0 > 0) {
+        const ORBFE_GLOBAL uint32_t* cnt =
+            gptr(surv_counts(J.surv, P.words)) + 4 * G.tile_base;
+        const uint32_t n = 4 * tr0 * G.tiles_x;
         uint32_t pc = 0;
         constexpr uint32_t kU = 4;  // loads in flight per thread
-        for (uint32_t k0 = 0; k0 < w0; k0 += kU * kCompactThreads) {
+        for (uint32_t k0 = 0; k0 < n; k0 += kU * kCompactThreads) {
             // loads at clamped indices, unconditional, so all kU are in
             // flight at once (a load under a branch is waited for inside it)
-            uint64_t v[kU];
+            uint32_t v[kU];
 #pragma unroll
-            for (uint32_t u = 0; u < kU; ++u) v[u] = surv[umin(k0 + u * kCompactThreads + tid, w0 - 1)];
+            for (uint32_t u = 0; u < kU; ++u) v[u] = cnt[umin(k0 + u * kCompactThreads + tid, n - 1)];
 #pragma unroll
             for (uint32_t u = 0; u < kU; ++u)
-                if (k0 + u * kCompactThreads + tid >= w0) v[u] = 0ull;
+                if (k0 + u * kCompactThreads + tid >= n) v[u] = 0u;
 #pragma unroll
-            for (uint32_t u = 0; u < kU; ++u) pc += __popcll(v[u]);
+            for (uint32_t u = 0; u < kU; ++u) pc += v[u];
         }
         const uint32_t t = wave_total(pc);
         if (lane == 0) wsum[wave] = t;
@@ -1116,7 +1135,7 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(BatchView bv, Job j
         for (uint32_t c = 0; c < kCompactChunks; ++c) {
             const uint32_t chunk = r0 + wave + kCompactWaves * c;
             const uint32_t k = chunk * 64 + lane;
-            if (!(chunk < c1 && k < words)) wv[c] = 0ull;
+            if (!(chunk < c1 && k >= w0 && k < w1)) wv[c] = 0ull;
             cc[c] = __popcll(wv[c]);
         }
 #pragma unroll
