@@ -452,7 +452,7 @@ int get_plan(orbfe_ctx* ctx, uint32_t W0, uint32_t H0, uint32_t cap, PlanEntry**
         G.feat_off = l * P.feat_stride;
         G.kp_off = l * cap;
         G.sm_off = sm;
-        sm += (G.w + orbfe::kSmoothExt) * (G.h + orbfe::kSmoothExt);
+        sm += orbfe::smooth_floats(G.w, G.h);
         G.lod0_scale = scale;  // toLod0 (match_features_cpu.cpp:58-64)
         G.texel_x = 1.0f / static_cast<float>(G.w);
         G.texel_y = 1.0f / static_cast<float>(G.h);

@@ -80,6 +80,20 @@ constexpr size_t kBfScratchBytes = size_t(16) << 20;
 // last column, so G(P) = G(w+3); the same holds for rows.
 constexpr uint32_t kSmoothExt = 4;
 
+// Layout of a level's smoothed image: bands of 4 grid rows, each band
+// column-interleaved (the value at (x, y) is float 4 * x + y % 4 of band
+// y / 4), bands smooth_band_floats(w) floats apart (the extended width
+// rounded up to 8 columns).  A 128-byte line then holds an 8 x 4 block of
+// the grid, which a rotated BRIEF pattern's scattered points share more
+// often than 32 x 1 runs of one row, and the 4 rows of one column a detect
+// wave computes are one 16-byte store.
+__host__ __device__ inline uint32_t smooth_band_floats(uint32_t w) {
+    return 4u * ((w + kSmoothExt + 7u) & ~7u);
+}
+__host__ __device__ inline uint32_t smooth_floats(uint32_t w, uint32_t h) {
+    return smooth_band_floats(w) * ((h + kSmoothExt + 3u) / 4u);
+}
+
 // Sampler taps of one coordinate along one axis: i0 | i1 << 16 and the
 // fraction, as axis_taps computes them.
 struct TapEntry {

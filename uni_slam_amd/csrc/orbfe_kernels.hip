@@ -52,7 +52,8 @@ __constant__ float c_u8_unit[256];
 //                        of A's slot | byte offset of B's slot << 16 |
 //                        (test mod 32) << 26;
 //   g_desc_order[v][..]  per lanes-per-keypoint variant v (8, 16, 32), angle
-//                        bin, group and lane: the slots of its PPL =
+//                        bin, keypoint alignment (x mod 8 | (y mod 4) << 3),
+//                        group and lane: the slots of its PPL =
 //                        kDescPts / LANES points (u8, in rows of
 //                        desc_order_row bytes) in load order.
 constexpr uint32_t kDescPts = 192;
@@ -62,6 +63,8 @@ __host__ __device__ constexpr uint32_t desc_slot(uint32_t p) { return p + p / 16
 
 constexpr uint32_t kAngleBins = 64;
 constexpr float kAngleBinScale = 10.185916357881302f;  // kAngleBins / (2 pi)
+// Keypoint positions modulo the 8 x 4 blocks of the smoothed layout.
+constexpr uint32_t kOrderAligns = 32;
 
 // Bytes per lane row of the gather order (PPL rounded up to whole u32).
 template <uint32_t LANES>
@@ -69,7 +72,8 @@ constexpr uint32_t desc_order_row() {
     return ((kDescPts / LANES) + 3u) & ~3u;
 }
 
-constexpr size_t kDescOrderBytes = kAngleBins * 2 * 32 * desc_order_row<32>();  // the largest
+constexpr size_t kDescOrderBytes =
+    size_t(kAngleBins) * kOrderAligns * 2 * 32 * desc_order_row<32>();  // the largest (1 MiB)
 __constant__ uint32_t c_desc_ppt8[kDescSlots];
 __constant__ uint32_t c_desc_tests[2][128];
 __device__ uint8_t g_desc_order[3][kDescOrderBytes];
@@ -187,6 +191,33 @@ __device__ inline uint2 gload_u2(const ORBFE_GLOBAL uint32_t* p) {
 }
 __device__ inline void gstore_u4(ORBFE_GLOBAL uint32_t* p, uint4 v) {
     *reinterpret_cast<ORBFE_GLOBAL u32x4*>(p) = u32x4{v.x, v.y, v.z, v.w};
+}
+
+// Smoothed values of rows y .. y + RPT - 1 of column x (y a multiple of
+// min(RPT, 4)) into a level's smoothed image at float offset `level_off`
+// (smooth_band_floats layout): one 16-byte store per band of 4 rows, 8 bytes
+// for RPT = 2, for each band whose first row is a grid row (< sh).  Rows of
+// such a band past the grid are stored too (the band holds them; nothing
+// reads them).
+template <int RPT>
+__device__ inline void store_smooth_rows(float* sm, uint32_t level_off, uint32_t band_floats,
+                                         uint32_t x, uint32_t y, uint32_t sh,
+                                         const float (&v)[RPT]) {
+    static_assert(RPT == 2 || RPT % 4 == 0, "whole or half bands");
+    ORBFE_GLOBAL uint32_t* g = reinterpret_cast<ORBFE_GLOBAL uint32_t*>(
+        reinterpret_cast<uintptr_t>(sm + level_off));
+    if constexpr (RPT == 2) {
+        if (y < sh)
+            *reinterpret_cast<ORBFE_GLOBAL u32x2*>(g + (y >> 2) * band_floats + 4 * x + (y & 3u)) =
+                u32x2{__float_as_uint(v[0]), __float_as_uint(v[1])};
+    } else {
+#pragma unroll
+        for (int b = 0; b < RPT / 4; ++b)
+            if (y + 4 * b < sh)
+                gstore_u4(g + ((y >> 2) + b) * band_floats + 4 * x,
+                          make_uint4(__float_as_uint(v[4 * b]), __float_as_uint(v[4 * b + 1]),
+                                     __float_as_uint(v[4 * b + 2]), __float_as_uint(v[4 * b + 3])));
+    }
 }
 
 // Element i of a global array with a uniform base (32-bit offsets).
@@ -699,6 +730,10 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
     __syncthreads();
 
     const uint32_t x = static_cast<uint32_t>(x0) + lane;
+    // the wave's first output row (a multiple of RPT) and the level's band
+    // stride (smooth_band_floats)
+    const uint32_t y_wave = static_cast<uint32_t>(y0) + wave * RPT;
+    const uint32_t band_floats = smooth_band_floats(w);
 
     // Smoothed image over the extended grid (generate_features.wgsl:149-172
     // at every integer point): per point, 3 x 3 bilinear samples of the
@@ -753,6 +788,7 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
             if (row_fast) {  // wave-uniform
                 const float frs[3] = {fr0, fr1, fr2};
                 float hv[3][RPT + 4];
+                float out[RPT];
 #pragma unroll
                 for (int m = 0; m < RPT + 4; ++m) {
                     const float* tr = tile[wave * RPT + 1 + m];  // row y - 2 + m
@@ -766,6 +802,7 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
                 for (int k = 0; k < RPT; ++k) {
                     const uint32_t rr = wave * RPT + k;
                     const uint32_t y = static_cast<uint32_t>(y0) + rr;
+                    out[k] = 0.0f;
                     if (y >= sh) continue;
                     const bool b1 = (row_b1 >> k) & 1u;
                     // The shader's sum starts at 0; every term is >= +0 (a
@@ -802,14 +839,17 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
                             sum = jj == 0 && i == 0 ? t : sum + t;  // 0 + t == t
                         }
                     }
-                    store_f32(J.smooth, 4 * (G.sm_off + y * sw + x), sum);
+                    out[k] = sum;
                 }
+                store_smooth_rows<RPT>(J.smooth, G.sm_off, band_floats, x, y_wave, sh, out);
             } else {
                 const float frs[3] = {fr0, fr1, fr2};
+                float out[RPT];
 #pragma unroll
                 for (int k = 0; k < RPT; ++k) {
                     const uint32_t rr = wave * RPT + k;
                     const uint32_t y = static_cast<uint32_t>(y0) + rr;
+                    out[k] = 0.0f;
                     if (y >= sh) continue;
                     float sum = 0.0f;
 #pragma unroll
@@ -832,15 +872,17 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
                             sum = jj == 0 && i == 0 ? t : sum + t;  // 0 + t == t, as above
                         }
                     }
-                    store_f32(J.smooth, 4 * (G.sm_off + y * sw + x), sum);
+                    out[k] = sum;
                 }
+                store_smooth_rows<RPT>(J.smooth, G.sm_off, band_floats, x, y_wave, sh, out);
             }
         }
     } else if (x < sw) {
         for (int k = 0; k < RPT; ++k) {
-            const uint32_t y = static_cast<uint32_t>(y0) + wave * RPT + k;
+            const uint32_t y = y_wave + k;
             if (y < sh)
-                store_f32(J.smooth, 4 * (G.sm_off + y * sw + x), smooth_direct(J, P, G, l, lut, x, y));
+                store_f32(J.smooth, 4 * (G.sm_off + (y >> 2) * band_floats + 4 * x + (y & 3u)),
+                          smooth_direct(J, P, G, l, lut, x, y));
         }
     }
 
@@ -1440,10 +1482,12 @@ __attribute__((amdgpu_waves_per_eu(LANES == 16 ? 8 : 1))) void k_describe(BatchV
     // LDS, then every lane evaluates its tests of the group from there.  Two
     // tests that name the same pattern position read the same rotated point,
     // so gathering it once changes no value.  The gather order comes from a
-    // table per angle bin that sorts the points by their position rotated to
-    // the bin's centre, so the lanes of one load touch few rows; it only
-    // orders the loads, every value is the one at the point's own exact
-    // rotated coordinates.
+    // table per angle bin and keypoint alignment (x mod 8, y mod 4) that
+    // groups the points by the 8 x 4 block of the grid (one 128-byte line of
+    // the smoothed layout) their position rotated to the bin's centre falls
+    // in, so the lanes of one load touch few lines; it only orders the
+    // loads, every value is the one at the point's own exact rotated
+    // coordinates.
     //
     // round() is half-to-even.  For |v| < 2^22, v + 1.5*2^23 lies in
     // [2^23, 2^24), where the float spacing is 1, so the addition rounds v
@@ -1451,23 +1495,26 @@ __attribute__((amdgpu_waves_per_eu(LANES == 16 ? 8 : 1))) void k_describe(BatchV
     // mantissa: i32(round(v)) = bits(v + 1.5*2^23) - bits(1.5*2^23).  The
     // pattern coordinates are at most 13 in magnitude, so |v| <= 19.  The
     // subtraction is folded into the keypoint coordinate (mod 2^32); columns
-    // are clamped as byte offsets, 4 * (x + ax) staying above 4 * (sw - 1)
-    // for the wrapped negatives.
+    // are clamped as byte offsets within a band (16 bytes per column),
+    // 16 * (x + ax) staying above 16 * (sw - 1) for the wrapped negatives.
     constexpr float kRound = 12582912.0f;  // 1.5 * 2^23
     constexpr uint32_t kRoundBits = 0x4B400000u;
     const uint32_t sw = G.w + kSmoothExt, sh = G.h + kSmoothExt;
     const uint8_t* sm = reinterpret_cast<const uint8_t*>(J.smooth + G.sm_off);
     const uint32_t yb = y - kRoundBits;
-    const uint32_t xb4 = 4 * (x - kRoundBits), sw4 = 4 * sw, xmax4 = 4 * (sw - 1);
+    const uint32_t xb16 = 16 * (x - kRoundBits), xmax16 = 16 * (sw - 1);
+    const uint32_t band_bytes = 4 * smooth_band_floats(G.w);
     const uint32_t bin = umin(static_cast<uint32_t>((theta + fm::kPi) * kAngleBinScale),
                               kAngleBins - 1);
+    const uint32_t align = (x & 7u) | (y & 3u) << 3;
     const uint32_t* order = reinterpret_cast<const uint32_t*>(
-        desc_order<LANES>() + (static_cast<size_t>(bin) * 2 * LANES + sub) * ROW);
+        desc_order<LANES>() +
+        ((static_cast<size_t>(bin) * kOrderAligns + align) * 2 * LANES + sub) * ROW);
     float* ks = samp[tid / LANES];
     uint32_t bits = 0;
     // Byte offset in the smoothed image of the point in slot q of a group,
-    // rotated by the keypoint's angle (offsets < 4 * sw * sh: 24-bit
-    // factors).
+    // rotated by the keypoint's angle (band index and band stride below
+    // 2^24: 24-bit factors).
     const auto point_offset = [&](uint32_t half, uint32_t q) -> uint32_t {
         float2 pp;
         if constexpr (kPackedPts) {
@@ -1477,10 +1524,10 @@ __attribute__((amdgpu_waves_per_eu(LANES == 16 ? 8 : 1))) void k_describe(BatchV
         } else {
             pp = pptf[half][q];
         }
-        const uint32_t ax4 =
-            umin(4 * __float_as_uint((pp.x * cs - pp.y * sn) + kRound) + xb4, xmax4);
+        const uint32_t ax16 =
+            umin(16 * __float_as_uint((pp.x * cs - pp.y * sn) + kRound) + xb16, xmax16);
         const uint32_t ay = umin(__float_as_uint((pp.x * sn + pp.y * cs) + kRound) + yb, sh - 1);
-        return __umul24(ay, sw4) + ax4;
+        return __umul24(ay >> 2, band_bytes) + (ax16 | (ay & 3u) << 2);
     };
     // The lane's tests of a group, once the group's samples are in ks: entry
     // e names the byte offsets of A's and B's samples in ks and the test t,
@@ -2398,11 +2445,15 @@ static void split_tests(uint8_t group[256], uint32_t npos[2]) {
     }
 }
 
-// Gather order of one lanes-per-keypoint variant: for each angle bin and
-// group, the group's kDescPts point ids sorted by (row, column) of their
-// position rotated to the bin's centre angle; lane s takes points s,
-// s + LANES, ... of that order (one load of the group's lanes: consecutive
-// points), stored as their slots.  The order only decides which lane loads
+// Gather order of one lanes-per-keypoint variant: for each angle bin,
+// keypoint alignment a (x mod 8 = a & 7, y mod 4 = a >> 3) and group, the
+// group's kDescPts point ids sorted by the 8 x 4 block of the smoothed
+// layout (band, then block column) that their position rotated to the bin's
+// centre angle reaches from a keypoint of that alignment, then by row and
+// column; lane s takes points s, s + LANES, ... of that order (one load of
+// the group's lanes: consecutive points, each run of 16 transposed so that
+// the 4 lanes of a quad take every fourth point of it, which measured
+// faster), stored as their slots.  The order only decides which lane loads
 // which point, never a value.
 static void build_desc_order(const int8_t (*pts)[kDescPts][2], uint8_t* out, uint32_t lanes,
                              uint32_t row) {
@@ -2410,20 +2461,33 @@ static void build_desc_order(const int8_t (*pts)[kDescPts][2], uint8_t* out, uin
     for (uint32_t bin = 0; bin < kAngleBins; ++bin) {
         const double th = -3.14159265358979323846 + (bin + 0.5) * (2.0 * 3.14159265358979323846 / kAngleBins);
         const double c = std::cos(th), sn = std::sin(th);
-        for (uint32_t g = 0; g < 2; ++g) {
-            uint32_t seq[kDescPts];
-            long key[kDescPts];
-            for (uint32_t q = 0; q < kDescPts; ++q) {
-                const double px = pts[g][q][0], py = pts[g][q][1];
-                const long rx = std::lround(px * c - py * sn), ry = std::lround(px * sn + py * c);
-                key[q] = (ry + 64) * 256 + (rx + 64);
-                seq[q] = q;
+        for (uint32_t a = 0; a < kOrderAligns; ++a) {
+            for (uint32_t g = 0; g < 2; ++g) {
+                uint32_t seq[kDescPts];
+                long key[kDescPts];
+                for (uint32_t q = 0; q < kDescPts; ++q) {
+                    const double px = pts[g][q][0], py = pts[g][q][1];
+                    const long rx = std::lround(px * c - py * sn), ry = std::lround(px * sn + py * c);
+                    // grid position from a keypoint at (64 + a mod 8, 64 + a / 8)
+                    const long gx = 64 + (a & 7u) + rx, gy = 64 + (a >> 3) + ry;
+                    key[q] = (((gy >> 2) * 32 + (gx >> 3)) * 256 + (ry + 64)) * 256 + (rx + 64);
+                    seq[q] = q;
+                }
+                std::stable_sort(seq, seq + kDescPts,
+                                 [&](uint32_t u, uint32_t v) { return key[u] < key[v]; });
+                // each run of 16 (8) consecutive ids transposed as a 4 x 4
+                // (4 x 2) block: lane 4i + j of a load takes point i + 4j
+                // (i + 2j) of the run
+                uint32_t tr[kDescPts];
+                const uint32_t run = lanes < 16 ? lanes : 16, per = run / 4;
+                for (uint32_t i = 0; i < kDescPts; ++i)
+                    tr[i] = seq[i - i % run + (i % run) % 4 * per + (i % run) / 4];
+                std::copy(tr, tr + kDescPts, seq);
+                uint8_t* dst = out + ((static_cast<size_t>(bin) * kOrderAligns + a) * 2 + g) * lanes * row;
+                for (uint32_t s = 0; s < lanes; ++s)
+                    for (uint32_t k = 0; k < row; ++k)
+                        dst[s * row + k] = k < ppl ? static_cast<uint8_t>(desc_slot(seq[lanes * k + s])) : 0;
             }
-            std::stable_sort(seq, seq + kDescPts, [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
-            uint8_t* dst = out + (static_cast<size_t>(bin) * 2 + g) * lanes * row;
-            for (uint32_t s = 0; s < lanes; ++s)
-                for (uint32_t k = 0; k < row; ++k)
-                    dst[s * row + k] = k < ppl ? static_cast<uint8_t>(desc_slot(seq[lanes * k + s])) : 0;
         }
     }
 }
