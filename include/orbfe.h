@@ -50,7 +50,13 @@ extern "C" {
 /* Tunables of the reference's passes. */
 typedef struct orbfe_config {
     uint32_t frame_width;   /* W0: WSLAM_FRAME_WIDTH, wslam/include/wslam/common.hpp:11-13 */
-    uint32_t frame_height;  /* H0: WSLAM_FRAME_HEIGHT, common.hpp:15-17 */
+    uint32_t frame_height;  /* H0: WSLAM_FRAME_HEIGHT, common.hpp:15-17.  Frame
+                               sizes (here and in batches) run from 1 to 65535
+                               per side, with each frame's pyramid, smoothed
+                               images and survivor words under 4 GiB (32-bit
+                               offsets; about 19000 x 19000 at 6 levels) and
+                               (height - 1) * pitch + width below 2^32;
+                               ORBFE_ERANGE otherwise */
     uint32_t levels;        /* GPUConst::levels_of_detail = 6, common.hpp:43 */
     double scale_factor;    /* GPUConst::lod_scale_factor = 1.2, common.hpp:51 */
     float fast_threshold;   /* THRESHOLD override, detect_corners.wgsl:22 (0.15) */

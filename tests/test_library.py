@@ -62,6 +62,19 @@ def test_create_rejects_ratio_outside_unit_interval():
         assert not h.value
 
 
+def test_create_rejects_frames_past_32bit_offsets():
+    """The kernels address a frame's pyramid, smoothed images and survivor
+    words with 32-bit byte offsets; sizes whose buffers would pass 4 GiB
+    are refused before any device call (so this runs without a GPU).
+    30000 x 30000: about 1.1e10 bytes of smoothed images over 6 levels."""
+    import ctypes as C
+    for w, h in ((30000, 30000), (65535, 65535), (65535, 20000), (0, 480), (640, 0)):
+        cfg = orbfe.default_config(w, h)
+        hnd = C.c_void_p()
+        assert orbfe.lib().orbfe_create(C.byref(cfg), 0, C.byref(hnd)) == orbfe.ERANGE, (w, h)
+        assert not hnd.value
+
+
 def test_load_features_host_entry_point():
     cap = 5
     feats = np.zeros(6 * (8 + 48 * cap), np.uint8)

@@ -74,6 +74,33 @@ void level_dims(uint32_t W0, uint32_t H0, uint32_t L, double scale, uint32_t* w,
     }
 }
 
+// Whether a W0 x H0 frame's per-frame buffers stay within the 32-bit byte
+// offsets the kernels address them with: the u8 frame (W0 * H0), the
+// pyramid (f32, levels >= 1), the smoothed images (f32, smooth_floats per
+// level) and the survivor words (u64, ceil(w / 64) per row).
+bool frame_fits_offsets(uint32_t W0, uint32_t H0, uint32_t L, double scale) {
+    uint32_t dw[ORBFE_MAX_LEVELS], dh[ORBFE_MAX_LEVELS];
+    level_dims(W0, H0, L, scale, dw, dh);
+    uint64_t pyr = 0, sm = 0, words = 0;
+    for (uint32_t l = 0; l < L; ++l) {
+        if (l > 0) pyr += static_cast<uint64_t>(dw[l]) * dh[l];
+        sm += static_cast<uint64_t>(4u * ((dw[l] + orbfe::kSmoothExt + 7u) / 8u * 8u)) *
+              ((dh[l] + orbfe::kSmoothExt + 3u) / 4u);
+        words += static_cast<uint64_t>(dh[l]) * ((dw[l] + 63u) / 64u);
+    }
+    constexpr uint64_t kMax = UINT32_MAX;
+    return static_cast<uint64_t>(W0) * H0 <= kMax && 4 * pyr <= kMax && 4 * sm <= kMax &&
+           8 * words <= kMax;
+}
+
+// Whether the kernels' 32-bit offsets y * pitch + x reach every byte of a
+// frame's rows (pitch >= width, checked first).
+bool pitch_fits_offsets(size_t pitch, uint32_t w, uint32_t h) {
+    if (h == 0) return true;  // an empty frame is refused with its plan
+    return pitch <= UINT32_MAX &&
+           static_cast<uint64_t>(pitch) * (h - 1) + w <= static_cast<uint64_t>(UINT32_MAX);
+}
+
 // Static part of the corner cull of cull_corners.wgsl for one level, as a
 // bitmap in the survivor-word layout (bit x&63 of word y*wpr + x/64).
 //
@@ -405,6 +432,9 @@ int get_plan(orbfe_ctx* ctx, uint32_t W0, uint32_t H0, uint32_t cap, PlanEntry**
     const uint32_t L = ctx->cfg.levels;
     if (W0 == 0 || H0 == 0 || W0 > 65535 || H0 > 65535)
         return fail(ctx, ORBFE_ERANGE, "frame %ux%u outside 1..65535", W0, H0);
+    if (!frame_fits_offsets(W0, H0, L, ctx->cfg.scale_factor))
+        return fail(ctx, ORBFE_ERANGE, "frame %ux%u: per-frame buffers past 32-bit offsets", W0,
+                    H0);
     if (cap == 0 || cap >= (1u << orbfe::kMatchIdxBits))
         return fail(ctx, ORBFE_ERANGE, "cap %u outside 1..%u", cap,
                     (1u << orbfe::kMatchIdxBits) - 1);
@@ -623,6 +653,8 @@ int layout_detect(orbfe_ctx* ctx, const orbfe_frame* frames, uint32_t n, DetectL
             return fail(ctx, ORBFE_EINVAL, "frame %u: null image or features", i);
         if (f.pitch < f.width)
             return fail(ctx, ORBFE_EINVAL, "frame %u: pitch %zu < width %u", i, f.pitch, f.width);
+        if (!pitch_fits_offsets(f.pitch, f.width, f.height))
+            return fail(ctx, ORBFE_ERANGE, "frame %u: pitch %zu past 32-bit offsets", i, f.pitch);
         PlanEntry* pe = nullptr;
         const int rc = get_plan(ctx, f.width, f.height, f.cap, &pe);
         if (rc) return rc;
@@ -862,6 +894,11 @@ int orbfe_create(const orbfe_config* cfg, int device, orbfe_ctx** out) {
     if ((th != 8 && th != 16 && th != 32) || (dl != 0 && dl != 8 && dl != 16 && dl != 32) ||
         (ml != 0 && ml != 16 && ml != 32 && ml != 64))
         return ORBFE_ERANGE;
+    // the main frame size (checked again, with every other size, per plan)
+    if (cfg->frame_width == 0 || cfg->frame_height == 0 || cfg->frame_width > 65535 ||
+        cfg->frame_height > 65535 ||
+        !frame_fits_offsets(cfg->frame_width, cfg->frame_height, cfg->levels, cfg->scale_factor))
+        return ORBFE_ERANGE;
     auto ctx = std::make_unique<orbfe_ctx>();
     ctx->device = device;
     ctx->cfg = *cfg;
@@ -938,6 +975,8 @@ int orbfe_detect_async(orbfe_ctx* ctx, const uint8_t* image, size_t pitch, float
     if (!image || !features) return fail(ctx, ORBFE_EINVAL, "null image or features");
     const Plan& P = ctx->main->host;
     if (pitch < P.W0) return fail(ctx, ORBFE_EINVAL, "pitch %zu < width %u", pitch, P.W0);
+    if (!pitch_fits_offsets(pitch, P.W0, P.H0))
+        return fail(ctx, ORBFE_ERANGE, "pitch %zu past 32-bit offsets", pitch);
     Job job0{};
     job0.plan = ctx->main->dev;
     job0.tiles = ctx->main->tiles_single;
