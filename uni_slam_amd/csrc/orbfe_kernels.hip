@@ -1387,6 +1387,24 @@ __attribute__((amdgpu_waves_per_eu(LANES == 16 ? 8 : 1))) void k_describe(BatchV
                            prm.border_zero);
         }
         wave_sync();
+    } else if (boxed && !S.base && x >= kBoxR && y >= kBoxR && x + kBoxR + 1 < S.w &&
+               y + kBoxR < S.h) {
+        // A pyramid level's keypoint whose box and the column after it lie
+        // inside the level: lanes 2r and 2r + 1 load row r as two 16-byte
+        // runs (columns x - 3 .. x and x + 1 .. x + 4, the last unused):
+        // ceil(14 / LANES) loads per lane instead of ceil(49 / LANES); the
+        // values are the in-range texels the per-entry form reads.
+        for (uint32_t q = sub; q < 2 * kBoxSide; q += LANES) {
+            const uint32_t r = q >> 1, half = q & 1u;
+            const uint4 v = gload_u4(reinterpret_cast<const ORBFE_GLOBAL uint32_t*>(
+                S.lvl + (y - kBoxR + r) * S.w + x - kBoxR + 4 * half));
+            float* row = bx + r * kBoxSide + 4 * half;
+            row[0] = __uint_as_float(v.x);
+            row[1] = __uint_as_float(v.y);
+            row[2] = __uint_as_float(v.z);
+            if (half == 0) row[3] = __uint_as_float(v.w);
+        }
+        wave_sync();
     } else if (boxed) {
         // every lane's loads first (entries past the box repeat the last),
         // then the values
