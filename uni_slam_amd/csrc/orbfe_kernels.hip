@@ -160,10 +160,12 @@ __device__ inline float load_f32(const uint8_t* base, uint32_t off) {
 }
 
 // Stores an f32 at byte offset `off` of a global buffer whose base is
-// uniform (SGPR base, one VGPR offset; no 64-bit address per lane).
-__device__ inline void store_f32(float* base, uint32_t off, float v) {
+// uniform (SGPR base, one VGPR offset; no 64-bit address per lane), as a
+// nontemporal store: the kernels use it for data that later launches read,
+// not the storing one.
+__device__ inline void store_f32_nt(float* base, uint32_t off, float v) {
     ORBFE_GLOBAL uint8_t* g = reinterpret_cast<ORBFE_GLOBAL uint8_t*>(reinterpret_cast<uintptr_t>(base));
-    *reinterpret_cast<ORBFE_GLOBAL float*>(g + off) = v;
+    __builtin_nontemporal_store(v, reinterpret_cast<ORBFE_GLOBAL float*>(g + off));
 }
 
 // The global-address-space view of a device pointer.  Accesses through it
@@ -198,7 +200,10 @@ __device__ inline void gstore_u4(ORBFE_GLOBAL uint32_t* p, uint4 v) {
 // (smooth_band_floats layout): one 16-byte store per band of 4 rows, 8 bytes
 // for RPT = 2, for each band whose first row is a grid row (< sh).  Rows of
 // such a band past the grid are stored too (the band holds them; nothing
-// reads them).
+// reads them).  The stores are nontemporal: the describe launch reads the
+// image after the whole detect grid (and the compaction) has run, and
+// keeping the detect writes out of L2 measured faster for both kernels in
+// batches and for detect on single frames.
 template <int RPT>
 __device__ inline void store_smooth_rows(float* sm, uint32_t level_off, uint32_t band_floats,
                                          uint32_t x, uint32_t y, uint32_t sh,
@@ -208,15 +213,17 @@ __device__ inline void store_smooth_rows(float* sm, uint32_t level_off, uint32_t
         reinterpret_cast<uintptr_t>(sm + level_off));
     if constexpr (RPT == 2) {
         if (y < sh)
-            *reinterpret_cast<ORBFE_GLOBAL u32x2*>(g + (y >> 2) * band_floats + 4 * x + (y & 3u)) =
-                u32x2{__float_as_uint(v[0]), __float_as_uint(v[1])};
+            __builtin_nontemporal_store(
+                u32x2{__float_as_uint(v[0]), __float_as_uint(v[1])},
+                reinterpret_cast<ORBFE_GLOBAL u32x2*>(g + (y >> 2) * band_floats + 4 * x + (y & 3u)));
     } else {
 #pragma unroll
         for (int b = 0; b < RPT / 4; ++b)
             if (y + 4 * b < sh)
-                gstore_u4(g + ((y >> 2) + b) * band_floats + 4 * x,
-                          make_uint4(__float_as_uint(v[4 * b]), __float_as_uint(v[4 * b + 1]),
-                                     __float_as_uint(v[4 * b + 2]), __float_as_uint(v[4 * b + 3])));
+                __builtin_nontemporal_store(
+                    u32x4{__float_as_uint(v[4 * b]), __float_as_uint(v[4 * b + 1]),
+                          __float_as_uint(v[4 * b + 2]), __float_as_uint(v[4 * b + 3])},
+                    reinterpret_cast<ORBFE_GLOBAL u32x4*>(g + ((y >> 2) + b) * band_floats + 4 * x));
     }
 }
 
@@ -881,8 +888,8 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
         for (int k = 0; k < RPT; ++k) {
             const uint32_t y = y_wave + k;
             if (y < sh)
-                store_f32(J.smooth, 4 * (G.sm_off + (y >> 2) * band_floats + 4 * x + (y & 3u)),
-                          smooth_direct(J, P, G, l, lut, x, y));
+                store_f32_nt(J.smooth, 4 * (G.sm_off + (y >> 2) * band_floats + 4 * x + (y & 3u)),
+                             smooth_direct(J, P, G, l, lut, x, y));
         }
     }
 
@@ -918,7 +925,8 @@ __global__ __launch_bounds__(kDetectThreads) void k_detect(BatchView bv, Job job
         const uint32_t y = static_cast<uint32_t>(y0) + r;
         wscore[k * 64 + lane] = 0;
         const bool inb = x < w && y < h;
-        if (inb && l > 0 && J.pyr) store_f32(J.pyr, 4 * (G.pyr_off + y * w + x), cen[k]);
+        if (inb && l > 0 && J.pyr)  // nontemporal, as the smoothed image (read by later launches)
+            store_f32_nt(J.pyr, 4 * (G.pyr_off + y * w + x), cen[k]);
         bool need = inb && tile_x < G.wpr &&
                     (y == 0 ? (x % 9) < 3 : ((mrow[k] >> lane) & 1ull) != 0);
         if (quick)
