@@ -1319,8 +1319,11 @@ __attribute__((amdgpu_waves_per_eu(LANES == 16 ? 8 : 1))) void k_describe(BatchV
     const uint32_t tid = threadIdx.x;
     // The block's level, its survivor count and keypoint entries are loaded
     // first (from the job alone), then the tables are filled while they
-    // arrive.
-    uint32_t b = bv.n ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    // arrive.  Every XCD takes a contiguous share of the blocks, single
+    // frames included: consecutive keypoints of a level lie in the same image
+    // rows, so an XCD's gathers share the lines its L2 fetched (the smoothed
+    // image reaches describe from memory; detect stores it nontemporally).
+    uint32_t b = xcd_block(blockIdx.x, gridDim.x);
     uint32_t j = 0;
     if (bv.n) {
         j = job_of(bv.desc_job, b);
